@@ -1,0 +1,629 @@
+// C ABI: the hmat.h free functions and the hmat_interface_t function table
+// backed by the MI355X engine (replaces c_default_interface.cpp and
+// c_wrapping.hpp of hmat-oss).  Exceptions never cross the boundary: table
+// entries print the message on stderr and return 1, as c_wrapping.hpp does.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+
+#include "engine.hpp"
+#include "hmat/hmat.h"
+#include "hmat/hmat_mi355x.h"
+
+using hmx::Matrix;
+
+namespace {
+
+thread_local std::string g_last_error;
+int g_rank_cap = 160;
+hmat_settings_t g_settings = {100, 1e-4, 200, 0, 0, 0, 0, 0, 0, 0, 0.0};
+
+struct ClusteringAlgo {
+  int kind = 0;        // median
+  int max_leaf = -1;   // -1: settings.maxLeafSize
+  int divider = 2;
+};
+
+struct Compression {
+  int method;
+  double eps;
+};
+
+struct Admissibility {
+  hmx::AdmissibilityParams p;
+};
+
+int fail(const std::exception& e) {
+  g_last_error = e.what();
+  fprintf(stderr, "%s\n", e.what());
+  return 1;
+}
+
+Matrix* M(hmat_matrix_t* h) { return reinterpret_cast<Matrix*>(h); }
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+}
+
+// ---- table entries -----------------------------------------------------------
+template <int VT>
+hmat_matrix_t* create_empty(const hmat_cluster_tree_t* r, const hmat_cluster_tree_t* c,
+                            int lower_sym, hmat_admissibility_t* cond) {
+  try {
+    const auto* rt = reinterpret_cast<const hmx::ClusterTree*>(r);
+    const auto* ct = reinterpret_cast<const hmx::ClusterTree*>(c);
+    hmx::AdmissibilityParams p;
+    if (cond) p = reinterpret_cast<Admissibility*>(cond)->p;
+    auto* m = new Matrix(rt, ct, lower_sym != 0, p, VT, g_rank_cap);
+    return reinterpret_cast<hmat_matrix_t*>(m);
+  } catch (const std::exception& e) {
+    fail(e);
+    return nullptr;
+  }
+}
+
+void own_cluster_trees(hmat_matrix_t* h, int owns_row, int owns_col) {
+  M(h)->own_rows = owns_row != 0;
+  M(h)->own_cols = owns_col != 0;
+}
+
+void set_low_rank_epsilon(hmat_matrix_t* h, double eps) { M(h)->lowrank_eps = eps; }
+
+int assemble_generic(hmat_matrix_t* h, hmat_assemble_context_t* ctx) {
+  return guard([&] {
+    Matrix* m = M(h);
+    if (!ctx->compression) throw std::runtime_error("No compression algorithm defined in hmat_assemble_context_t");
+    if (ctx->lower_symmetric && !m->lower_sym)
+      throw std::runtime_error("lower_symmetric assembly needs a matrix created lower-symmetric");
+    if (ctx->simple_compute != hmat_mi355x_kernel_compute || ctx->assembly || ctx->block_compute ||
+        ctx->advanced_compute)
+      throw std::runtime_error(
+          "the MI355X engine assembles on the device: set simple_compute = "
+          "hmat_mi355x_kernel_compute with an hmat_mi355x_kernel_t user_context");
+    const auto* k = static_cast<const hmat_mi355x_kernel_t*>(ctx->user_context);
+    hmx::KernelSpec spec;
+    spec.kind = k->kind;
+    spec.length = k->length;
+    spec.nugget = k->nugget;
+    spec.dim = k->dimension;
+    spec.n = k->n;
+    spec.points.assign(k->points, k->points + (size_t)k->n * k->dimension);
+    const auto* c = reinterpret_cast<const Compression*>(ctx->compression);
+    hmx::CompressionSpec cs{c->method, c->eps};
+    m->assemble(spec, cs);
+    if (ctx->factorization != hmat_factorization_none) m->factorize(ctx->factorization, false);
+  });
+}
+
+int destroy(hmat_matrix_t* h) {
+  Matrix* m = M(h);
+  const hmx::ClusterTree* r = m->own_rows ? m->rows : nullptr;
+  const hmx::ClusterTree* c = m->own_cols ? m->cols : nullptr;
+  delete m;
+  if (r) delete r;
+  if (c && c != r) delete c;
+  return 0;
+}
+
+int factorize_generic(hmat_matrix_t* h, hmat_factorization_context_t* ctx) {
+  return guard([&] { M(h)->factorize(ctx->factorization, false); });
+}
+
+int logdet(hmat_matrix_t* h, double* res) {
+  return guard([&] { *res = M(h)->logdet(); });
+}
+
+int solve_systems(hmat_matrix_t* h, void* b, int nrhs) {
+  return guard([&] { M(h)->solve_host(b, nrhs, true); });
+}
+
+int solve_dense(hmat_matrix_t* h, void* b, int nrhs) {
+  return guard([&] { M(h)->solve_host(b, nrhs, false); });
+}
+
+int solve_generic(hmat_matrix_t* h, const struct hmat_solve_context_t* ctx) {
+  return guard([&] {
+    if (ctx->nr_rhs == 0) throw std::runtime_error("solve_generic: H-matrix right-hand sides are not supported");
+    if (ctx->lower || ctx->upper) throw std::runtime_error("solve_generic: lower/upper solves are not supported");
+    M(h)->solve_host(ctx->values, ctx->nr_rhs, ctx->no_permutation == 0);
+  });
+}
+
+template <typename T>
+void reorder_axis(T* v, int nrows, int ncols, const int* idx, int axis, bool restore) {
+  const int n = axis == 0 ? nrows : ncols;
+  std::vector<T> tmp(n);
+  if (axis == 0) {
+    for (int c = 0; c < ncols; ++c) {
+      T* col = v + (size_t)c * nrows;
+      for (int i = 0; i < n; ++i) {
+        if (restore)
+          tmp[idx[i]] = col[i];
+        else
+          tmp[i] = col[idx[i]];
+      }
+      std::memcpy(col, tmp.data(), n * sizeof(T));
+    }
+  } else {
+    for (int r = 0; r < nrows; ++r) {
+      for (int i = 0; i < n; ++i) {
+        if (restore)
+          tmp[idx[i]] = v[r + (size_t)i * nrows];
+        else
+          tmp[i] = v[r + (size_t)idx[i] * nrows];
+      }
+      for (int i = 0; i < n; ++i) v[r + (size_t)i * nrows] = tmp[i];
+    }
+  }
+}
+
+// reorderVector / restoreVectorOrder h_matrix.cpp:75-141 via c_wrapping.hpp:455-500
+template <typename T, bool Restore>
+int vector_order(void* vec, const hmat_cluster_tree_t* rows_ct, int rows,
+                 const hmat_cluster_tree_t* cols_ct, int cols) {
+  return guard([&] {
+    if (!rows_ct && rows == 0) throw std::runtime_error("either row cluster tree or rows must be non null");
+    if (!cols_ct && cols == 0) throw std::runtime_error("either col cluster tree or cols must be non null");
+    const auto* rt = reinterpret_cast<const hmx::ClusterTree*>(rows_ct);
+    const auto* ct = reinterpret_cast<const hmx::ClusterTree*>(cols_ct);
+    const int nr = rt ? rt->n : rows, nc = ct ? ct->n : cols;
+    if (rt) reorder_axis<T>((T*)vec, nr, nc, rt->indices.data(), 0, Restore);
+    if (ct) reorder_axis<T>((T*)vec, nr, nc, ct->indices.data(), 1, Restore);
+  });
+}
+
+int gemv(char trans, void* alpha, hmat_matrix_t* h, void* x, void* beta, void* y, int nrhs) {
+  return guard([&] { M(h)->gemv_host(trans, alpha, x, beta, y, nrhs, true); });
+}
+
+int gemm_scalar(char trans, void* alpha, hmat_matrix_t* h, void* x, void* beta, void* y, int nrhs) {
+  return guard([&] { M(h)->gemv_host(trans, alpha, x, beta, y, nrhs, false); });
+}
+
+int gemm_dense(char trans_b, char trans_x, char side, const void* alpha, hmat_matrix_t* h,
+               void* x, const void* beta, void* y, int nrhs) {
+  return guard([&] {
+    if (side != 'L' || trans_x != 'N')
+      throw std::runtime_error("gemm_dense: only side 'L' with trans_x 'N' is supported");
+    M(h)->gemv_host(trans_b, alpha, x, beta, y, nrhs, false);
+  });
+}
+
+int init_engine() {
+  return guard([] {
+    int n = 0;
+    HMX_CHECK(hipGetDeviceCount(&n));
+    if (n < 1) throw std::runtime_error("no HIP device");
+  });
+}
+int finalize_engine() { return 0; }
+
+int get_info(hmat_matrix_t* h, hmat_info_t* info) {
+  return guard([&] { M(h)->info(*info); });
+}
+
+int get_cluster_trees(hmat_matrix_t* h, const hmat_cluster_tree_t** r, const hmat_cluster_tree_t** c) {
+  if (r) *r = reinterpret_cast<const hmat_cluster_tree_t*>(M(h)->rows);
+  if (c) *c = reinterpret_cast<const hmat_cluster_tree_t*>(M(h)->cols);
+  return 0;
+}
+
+// ---- unsupported slots -------------------------------------------------------
+int unsupported(const char* what) {
+  g_last_error = std::string(what) + ": not provided by the MI355X engine";
+  fprintf(stderr, "%s\n", g_last_error.c_str());
+  return 1;
+}
+hmat_matrix_t* u_copy(hmat_matrix_t*) { unsupported("copy"); return nullptr; }
+double u_norm(hmat_matrix_t*) { unsupported("norm"); return NAN; }
+int u_inverse(hmat_matrix_t*) { return unsupported("inverse"); }
+hmat_matrix_t* u_get_child(hmat_matrix_t*, int, int) { unsupported("get_child"); return nullptr; }
+int u_h(hmat_matrix_t*) { return unsupported("operation"); }
+int u_solve_mat(hmat_matrix_t*, hmat_matrix_t*) { return unsupported("solve_mat"); }
+int u_scale(void*, hmat_matrix_t*) { return unsupported("scale"); }
+int u_gemm(char, char, const void*, hmat_matrix_t*, hmat_matrix_t*, const void*, hmat_matrix_t*) { return unsupported("gemm"); }
+int u_axpy(void*, hmat_matrix_t*, hmat_matrix_t*) { return unsupported("axpy"); }
+int u_trsm(char, char, char, char, int, int, void*, hmat_matrix_t*, int, void*) { return unsupported("trsm"); }
+int u_full_gemm(char, char, int, int, void*, void*, void*, hmat_matrix_t*, void*) { return unsupported("full_gemm"); }
+int u_add_identity(hmat_matrix_t*, void*) { return unsupported("add_identity"); }
+int u_profile(hmat_matrix_t*, hmat_profile_t*) { return unsupported("get_profile"); }
+int u_ratio(hmat_matrix_t*, hmat_FPCompressionRatio_t*) { return unsupported("get_ratio"); }
+hmat_fp_settings_t u_get_fp(hmat_matrix_t*) { unsupported("GetFPCompressionSettings"); return hmat_fp_settings_t{}; }
+int u_set_fp(hmat_matrix_t*, hmat_fp_settings_t) { return unsupported("SetFPCompressionSettings"); }
+int u_set_fp2(hmat_matrix_t*, float, int, hmat_FPcompress_t, bool, bool) { return unsupported("SetFPCompressionSettingsParams"); }
+int u_dump(hmat_matrix_t*, char*) { return unsupported("dump_info"); }
+int u_set_ct(hmat_matrix_t*, const hmat_cluster_tree_t*, const hmat_cluster_tree_t*) { return unsupported("set_cluster_trees"); }
+int u_diag(hmat_matrix_t*, void*, int) { return unsupported("extract_diagonal"); }
+int u_slt(hmat_matrix_t*, int, void*, int) { return unsupported("solve_lower_triangular"); }
+int u_values(struct hmat_get_values_context_t*) { return unsupported("get_block/get_values"); }
+int u_walk(hmat_matrix_t*, hmat_procedure_t*) { return unsupported("walk"); }
+int u_leaf(hmat_matrix_t*, const hmat_leaf_procedure_t*) { return unsupported("apply_on_leaf"); }
+hmat_matrix_t* u_read_struct(hmat_iostream, void*) { unsupported("read_struct"); return nullptr; }
+void u_write(hmat_matrix_t*, hmat_iostream, void*) { unsupported("write_struct/read_data/write_data"); }
+void u_progress(hmat_matrix_t*, hmat_progress_t*) {}
+int u_diag_block(hmat_matrix_t*, int, void*) { return unsupported("extract_diagonal_block"); }
+
+template <int VT, typename T>
+void fill(hmat_interface_t* i) {
+  std::memset(i, 0, sizeof(*i));
+  i->create_empty_hmatrix_admissibility = create_empty<VT>;
+  i->own_cluster_trees = own_cluster_trees;
+  i->set_low_rank_epsilon = set_low_rank_epsilon;
+  i->assemble_generic = assemble_generic;
+  i->copy = u_copy;
+  i->copy_struct = u_copy;
+  i->norm = u_norm;
+  i->logdet = logdet;
+  i->inverse = u_inverse;
+  i->factorize_generic = factorize_generic;
+  i->destroy = destroy;
+  i->get_child = u_get_child;
+  i->destroy_child = u_h;
+  i->solve_generic = solve_generic;
+  i->solve_mat = u_solve_mat;
+  i->solve_systems = solve_systems;
+  i->solve_dense = solve_dense;
+  i->transpose = u_h;
+  i->scale = u_scale;
+  i->truncate = u_h;
+  i->vector_reorder = vector_order<T, false>;
+  i->vector_restore = vector_order<T, true>;
+  i->gemm = u_gemm;
+  i->axpy = u_axpy;
+  i->trsm = u_trsm;
+  i->gemv = gemv;
+  i->gemm_scalar = gemm_scalar;
+  i->full_gemm = u_full_gemm;
+  i->gemm_dense = gemm_dense;
+  i->add_identity = u_add_identity;
+  i->init = init_engine;
+  i->finalize = finalize_engine;
+  i->get_info = get_info;
+  i->get_profile = u_profile;
+  i->get_ratio = u_ratio;
+  i->FPcompress = u_h;
+  i->FPdecompress = u_h;
+  i->GetFPCompressionSettings = u_get_fp;
+  i->SetFPCompressionSettings = u_set_fp;
+  i->SetFPCompressionSettingsParams = u_set_fp2;
+  i->dump_info = u_dump;
+  i->get_cluster_trees = get_cluster_trees;
+  i->set_cluster_trees = u_set_ct;
+  i->extract_diagonal = u_diag;
+  i->solve_lower_triangular = u_slt;
+  i->solve_lower_triangular_dense = u_slt;
+  i->get_block = u_values;
+  i->get_values = u_values;
+  i->walk = u_walk;
+  i->apply_on_leaf = u_leaf;
+  i->value_type = (hmat_value_t)VT;
+  i->internal = nullptr;
+  i->read_struct = u_read_struct;
+  i->write_struct = u_write;
+  i->read_data = u_write;
+  i->write_data = u_write;
+  i->set_progressbar = u_progress;
+  i->extract_diagonal_block = u_diag_block;
+}
+
+}  // namespace
+
+extern "C" {
+
+void hmat_init_mi355x_interface(hmat_interface_t* i, hmat_value_t type) {
+  switch (type) {
+    case HMAT_SIMPLE_PRECISION: fill<HMAT_SIMPLE_PRECISION, float>(i); break;
+    case HMAT_DOUBLE_PRECISION: fill<HMAT_DOUBLE_PRECISION, double>(i); break;
+    default:
+      // complex types: the table is filled, operations report the limitation
+      fill<HMAT_DOUBLE_PRECISION, double>(i);
+      i->value_type = type;
+      i->create_empty_hmatrix_admissibility = [](const hmat_cluster_tree_t*, const hmat_cluster_tree_t*, int,
+                                                 hmat_admissibility_t*) -> hmat_matrix_t* {
+        unsupported("complex arithmetic");
+        return nullptr;
+      };
+  }
+}
+
+void hmat_mi355x_kernel_compute(void* ctx, int row, int col, void* result) {
+  const auto* k = static_cast<const hmat_mi355x_kernel_t*>(ctx);
+  double d[3] = {0, 0, 0};
+  for (int q = 0; q < k->dimension && q < 3; ++q)
+    d[q] = k->points[(size_t)row * k->dimension + q] - k->points[(size_t)col * k->dimension + q];
+  hmx::KernelParams kp{k->kind, k->length, k->nugget};
+  *(double*)result = hmx::kernel_value(kp, d[0], d[1], d[2], row == col);
+}
+
+// ---- clustering / trees -------------------------------------------------------
+hmat_clustering_algorithm_t* hmat_create_clustering_median(void) {
+  return reinterpret_cast<hmat_clustering_algorithm_t*>(new ClusteringAlgo());
+}
+hmat_clustering_algorithm_t* hmat_create_clustering_max_dof(const hmat_clustering_algorithm_t* a, int max_dof) {
+  auto* r = new ClusteringAlgo(*reinterpret_cast<const ClusteringAlgo*>(a));
+  r->max_leaf = max_dof;
+  return reinterpret_cast<hmat_clustering_algorithm_t*>(r);
+}
+void hmat_delete_clustering(hmat_clustering_algorithm_t* a) { delete reinterpret_cast<ClusteringAlgo*>(a); }
+void hmat_set_clustering_divider(hmat_clustering_algorithm_t* a, int divider) {
+  if (divider != 2) fprintf(stderr, "hmat_set_clustering_divider: only binary median trees are supported\n");
+  reinterpret_cast<ClusteringAlgo*>(a)->divider = 2;
+}
+
+hmat_cluster_tree_t* hmat_create_cluster_tree(double* coord, int dimension, int size,
+                                              hmat_clustering_algorithm_t* algo) {
+  try {
+    const auto* a = reinterpret_cast<const ClusteringAlgo*>(algo);
+    const int leaf = a->max_leaf >= 0 ? a->max_leaf : g_settings.maxLeafSize;
+    return reinterpret_cast<hmat_cluster_tree_t*>(hmx::build_cluster_tree(coord, dimension, size, leaf).release());
+  } catch (const std::exception& e) {
+    fail(e);
+    return nullptr;
+  }
+}
+
+struct hmat_cluster_tree_builder {
+  ClusteringAlgo algo;
+};
+hmat_cluster_tree_builder_t* hmat_create_cluster_tree_builder(const hmat_clustering_algorithm_t* a) {
+  return new hmat_cluster_tree_builder{*reinterpret_cast<const ClusteringAlgo*>(a)};
+}
+void hmat_delete_cluster_tree_builder(hmat_cluster_tree_builder_t* b) { delete b; }
+
+hmat_cluster_tree_t* hmat_create_cluster_tree_generic(struct hmat_cluster_tree_create_context_t* ctx) {
+  try {
+    if (ctx->span_offsets || ctx->group_index)
+      throw std::runtime_error("hmat_create_cluster_tree_generic: spans and groups are not supported");
+    const int leaf = ctx->builder && ctx->builder->algo.max_leaf >= 0 ? ctx->builder->algo.max_leaf
+                                                                      : g_settings.maxLeafSize;
+    return reinterpret_cast<hmat_cluster_tree_t*>(
+        hmx::build_cluster_tree(ctx->coordinates, (int)ctx->dimension, (int)ctx->number_of_dof, leaf).release());
+  } catch (const std::exception& e) {
+    fail(e);
+    return nullptr;
+  }
+}
+
+void hmat_delete_cluster_tree(const hmat_cluster_tree_t* t) {
+  delete reinterpret_cast<const hmx::ClusterTree*>(t);
+}
+int hmat_tree_nodes_count(const hmat_cluster_tree_t* t) {
+  return reinterpret_cast<const hmx::ClusterTree*>(t)->node_count();
+}
+int hmat_cluster_get_info(hmat_cluster_tree_t* t, hmat_cluster_info_t* info) {
+  const auto* ct = reinterpret_cast<const hmx::ClusterTree*>(t);
+  info->spatial_dimension = ct->dim;
+  info->dimension = ct->n;
+  info->nr_tree_nodes = ct->node_count();
+  return 0;
+}
+const int* hmat_cluster_get_indices(const hmat_cluster_tree_t* t) {
+  return reinterpret_cast<const hmx::ClusterTree*>(t)->indices.data();
+}
+
+// ---- admissibility / compression ---------------------------------------------
+void hmat_init_admissibility_param(hmat_admissibility_param_t* p) {
+  p->eta = 2;
+  p->ratio = 0.0;
+  p->max_width = (size_t)-1L;
+  p->scale = -1;
+}
+hmat_admissibility_t* hmat_create_admissibility(hmat_admissibility_param_t* p) {
+  auto* a = new Admissibility();
+  a->p.eta = p->eta;
+  a->p.ratio = p->ratio;
+  a->p.max_width = p->max_width;
+  return reinterpret_cast<hmat_admissibility_t*>(a);
+}
+void hmat_update_admissibility(hmat_admissibility_t* c, hmat_admissibility_param_t* p) {
+  auto* a = reinterpret_cast<Admissibility*>(c);
+  a->p.ratio = p->ratio;
+  a->p.max_width = p->max_width;
+}
+hmat_admissibility_t* hmat_create_admissibility_standard(double eta) {
+  auto* a = new Admissibility();
+  a->p.eta = eta;
+  return reinterpret_cast<hmat_admissibility_t*>(a);
+}
+void hmat_delete_admissibility(hmat_admissibility_t* c) { delete reinterpret_cast<Admissibility*>(c); }
+
+hmat_compression_algorithm_t* hmat_create_compression_svd(double eps) {
+  return reinterpret_cast<hmat_compression_algorithm_t*>(new Compression{hmat_compress_svd, eps});
+}
+hmat_compression_algorithm_t* hmat_create_compression_aca_partial(double eps) {
+  return reinterpret_cast<hmat_compression_algorithm_t*>(new Compression{hmat_compress_aca_partial, eps});
+}
+hmat_compression_algorithm_t* hmat_create_compression_aca_plus(double eps) {
+  return reinterpret_cast<hmat_compression_algorithm_t*>(new Compression{hmat_compress_aca_plus, eps});
+}
+void hmat_delete_compression(const hmat_compression_algorithm_t* c) {
+  delete reinterpret_cast<const Compression*>(c);
+}
+
+// ---- contexts / settings -------------------------------------------------------
+void hmat_assemble_context_init(hmat_assemble_context_t* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->factorization = hmat_factorization_none;
+  c->progress = nullptr;
+}
+void hmat_factorization_context_init(hmat_factorization_context_t* c) {
+  c->factorization = hmat_factorization_lu;
+  c->progress = nullptr;
+}
+void hmat_solve_context_init(struct hmat_solve_context_t* c) { std::memset(c, 0, sizeof(*c)); }
+
+static hmat_progress_t g_progress = {0, 0, nullptr, nullptr};
+hmat_progress_t* hmat_default_progress(void) { return &g_progress; }
+
+void hmat_get_parameters(hmat_settings_t* s) { *s = g_settings; }
+int hmat_set_parameters(hmat_settings_t* s) {
+  if (s->compressionMinLeafSize != 100) {
+    fprintf(stderr, "hmat_set_parameters: compressionMinLeafSize is fixed at 100 in this engine\n");
+    return 1;
+  }
+  g_settings = *s;
+  return 0;
+}
+const char* hmat_get_version(void) { return "hmat-mi355x 0.1"; }
+
+// ---- flat helpers ----------------------------------------------------------------
+const char* hmx_last_error(void) { return g_last_error.c_str(); }
+
+int hmx_gemv_device(hmat_matrix_t* h, int precision, double alpha, const void* x, double beta,
+                    void* y, int nrhs) {
+  return guard([&] {
+    Matrix* m = M(h);
+    m->device_init();
+    if (precision == 0)
+      m->gemv_device<double>(alpha, (const double*)x, beta, (double*)y, nrhs, m->n_cols(), m->n_rows(), m->s64);
+    else
+      m->gemv_device<float>(alpha, (const float*)x, beta, (float*)y, nrhs, m->n_cols(), m->n_rows(), m->s32);
+    if (!m->user_stream) HMX_CHECK(hipStreamSynchronize(m->stream));
+  });
+}
+
+int hmx_solve_device(hmat_matrix_t* h, int precision, void* b, int nrhs) {
+  return guard([&] {
+    Matrix* m = M(h);
+    if (precision == 0)
+      m->solve_device<double>((double*)b, nrhs, m->n_rows(), m->s64);
+    else
+      m->solve_device<float>((float*)b, nrhs, m->n_rows(), m->s32);
+    if (!m->user_stream) HMX_CHECK(hipStreamSynchronize(m->stream));
+  });
+}
+
+int hmx_factorize_mixed(hmat_matrix_t* h, int algo) {
+  return guard([&] { M(h)->factorize(algo, true); });
+}
+
+int hmx_solve_refined_device(hmat_matrix_t* h, double* b, int nrhs, double tol, int max_iter,
+                             double* resid) {
+  int it = -1;
+  const int rc = guard([&] { it = M(h)->solve_refined(b, nrhs, M(h)->n_rows(), tol, max_iter, resid); });
+  return rc ? -1 : it;
+}
+
+int hmx_partition(hmat_matrix_t* h, int* out) {
+  int count = 0;
+  std::function<void(const hmx::Block*)> rec = [&](const hmx::Block* b) {
+    if (out) {
+      int* o = out + 5 * count;
+      o[0] = b->rows->offset;
+      o[1] = b->rows->size;
+      o[2] = b->cols->offset;
+      o[3] = b->cols->size;
+      o[4] = (int)b->kind;
+    }
+    ++count;
+    if (!b->is_leaf())
+      for (int i = 0; i < b->nrow * b->ncol; ++i)
+        if (b->child[i]) rec(b->child[i]);
+  };
+  rec(M(h)->bt->root);
+  return count;
+}
+
+int hmx_leaf_count(hmat_matrix_t* h) { return (int)M(h)->bt->leaves.size(); }
+
+int hmx_leaf_info(hmat_matrix_t* h, int leaf, int precision, int* info) {
+  return guard([&] {
+    Matrix* m = M(h);
+    const hmx::LeafDesc& L = m->s64.hdesc.at(leaf);
+    info[0] = L.m;
+    info[1] = L.n;
+    info[2] = L.r0;
+    info[3] = L.c0;
+    info[4] = L.kind;
+    int r = -1;
+    const hmx::DevBuf& rk = precision == 0 ? m->s64.ranks : m->s32.ranks;
+    if (L.kind == 1 && rk.p)
+      HMX_CHECK(hipMemcpy(&r, rk.as<int>() + leaf, sizeof(int), hipMemcpyDeviceToHost));
+    info[5] = r;
+  });
+}
+
+int hmx_leaf_table(hmat_matrix_t* h, int precision, int* out) {
+  return guard([&] {
+    Matrix* m = M(h);
+    const auto& D = m->s64.hdesc;
+    std::vector<int> rk(D.size(), -1);
+    const hmx::DevBuf& rb = precision == 0 ? m->s64.ranks : m->s32.ranks;
+    if (rb.p) HMX_CHECK(hipMemcpy(rk.data(), rb.p, rk.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < D.size(); ++i) {
+      int* o = out + 6 * i;
+      o[0] = D[i].m;
+      o[1] = D[i].n;
+      o[2] = D[i].r0;
+      o[3] = D[i].c0;
+      o[4] = D[i].kind;
+      o[5] = D[i].kind == 0 ? -1 : rk[i];
+    }
+  });
+}
+
+int hmx_leaf_data(hmat_matrix_t* h, int leaf, int precision, double* a, double* b) {
+  return guard([&] {
+    Matrix* m = M(h);
+    const hmx::LeafDesc& L = m->s64.hdesc.at(leaf);
+    auto fetch = [&](int64_t off, size_t count, double* dst) {
+      if (precision == 0) {
+        if (!m->s64.arena.p) throw std::runtime_error("no FP64 data");
+        HMX_CHECK(hipMemcpy(dst, m->s64.data() + off, count * sizeof(double), hipMemcpyDeviceToHost));
+      } else {
+        if (!m->s32.arena.p) throw std::runtime_error("no FP32 data");
+        std::vector<float> tmp(count);
+        HMX_CHECK(hipMemcpy(tmp.data(), m->s32.data() + off, count * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < count; ++i) dst[i] = tmp[i];
+      }
+    };
+    if (L.kind == 0) {
+      fetch(L.a, (size_t)L.m * L.n, a);
+    } else {
+      int r = 0;
+      const hmx::DevBuf& rk = precision == 0 ? m->s64.ranks : m->s32.ranks;
+      HMX_CHECK(hipMemcpy(&r, rk.as<int>() + leaf, sizeof(int), hipMemcpyDeviceToHost));
+      if (r > 0) {
+        fetch(L.a, (size_t)L.m * r, a);
+        fetch(L.b, (size_t)L.n * r, b);
+      }
+    }
+  });
+}
+
+int hmx_stats(hmat_matrix_t* h, double* out) {
+  const hmx::Stats& s = M(h)->stats;
+  out[0] = s.t_assemble;
+  out[1] = s.t_factor;
+  out[2] = s.t_solve;
+  out[3] = (double)s.tasks;
+  out[4] = (double)s.waves;
+  out[5] = (double)s.launches;
+  out[6] = (double)s.segments;
+  out[7] = (double)s.temp_bytes;
+  return 0;
+}
+
+int hmx_set_stream(hmat_matrix_t* h, void* s) {
+  return guard([&] {
+    Matrix* m = M(h);
+    m->device_init();
+    HMX_CHECK(hipStreamSynchronize(m->stream));
+    m->stream = s ? (hipStream_t)s : m->own_stream;
+    m->user_stream = s != nullptr;
+  });
+}
+
+void hmx_set_rank_capacity(int cap) { g_rank_cap = cap > 0 ? cap : 160; }
+
+}  // extern "C"

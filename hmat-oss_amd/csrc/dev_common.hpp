@@ -1,0 +1,136 @@
+// Device-side helpers shared by the HIP kernels: wavefront/workgroup
+// reductions for 64-wide wavefronts and the built-in covariance kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#define HMX_CHECK(expr)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                               __FILE__ + ":" + std::to_string(__LINE__));                 \
+  } while (0)
+
+namespace hmx {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;  // threads per workgroup for the one-workgroup-per-task kernels
+constexpr int kWaves = kBlock / kWave;
+
+// Built-in interaction kernels (user numbering for the nugget).
+enum KernelKind : int { kExponential = 0, kGaussian = 1 };
+
+struct KernelParams {
+  int kind;
+  double length;
+  double nugget;
+};
+
+__host__ __device__ inline double kernel_value(const KernelParams& k, double dx, double dy,
+                                               double dz, bool same) {
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double v;
+  if (k.kind == kGaussian) {
+    v = exp(-r2 / (k.length * k.length));
+  } else {
+    v = exp(-sqrt(r2) / k.length);
+  }
+  if (same) v += k.nugget;
+  return v;
+}
+
+// ---- reductions -------------------------------------------------------------
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Sum over the whole workgroup (blockDim.x == kBlock).  All threads get the result.
+template <typename T>
+__device__ inline T block_sum(T v, T* red /* >= kWaves */) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) s += red[i];
+  return s;
+}
+
+// Arg-max of |value| with the lowest index winning ties (BLAS i_amax rule).
+struct ArgMax {
+  double v;
+  int i;
+};
+
+__device__ inline ArgMax better_max(ArgMax a, ArgMax b) {
+  if (b.v > a.v || (b.v == a.v && b.i >= 0 && (a.i < 0 || b.i < a.i))) return b;
+  return a;
+}
+
+__device__ inline ArgMax wave_argmax(ArgMax a) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ArgMax b;
+    b.v = __shfl_xor(a.v, o, kWave);
+    b.i = __shfl_xor(a.i, o, kWave);
+    a = better_max(a, b);
+  }
+  return a;
+}
+
+__device__ inline ArgMax block_argmax(ArgMax a, double* redv, int* redi) {
+  a = wave_argmax(a);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) {
+    redv[w] = a.v;
+    redi[w] = a.i;
+  }
+  __syncthreads();
+  ArgMax r{-1.0, -1};
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) r = better_max(r, ArgMax{redv[i], redi[i]});
+  return r;
+}
+
+// Arg-min with the lowest index winning ties (strict '<' scan order).
+__device__ inline ArgMax better_min(ArgMax a, ArgMax b) {
+  if (b.i < 0) return a;
+  if (a.i < 0) return b;
+  if (b.v < a.v || (b.v == a.v && b.i < a.i)) return b;
+  return a;
+}
+
+__device__ inline ArgMax block_argmin(ArgMax a, double* redv, int* redi) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ArgMax b;
+    b.v = __shfl_xor(a.v, o, kWave);
+    b.i = __shfl_xor(a.i, o, kWave);
+    a = better_min(a, b);
+  }
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) {
+    redv[w] = a.v;
+    redi[w] = a.i;
+  }
+  __syncthreads();
+  ArgMax r{0.0, -1};
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) r = better_min(r, ArgMax{redv[i], redi[i]});
+  return r;
+}
+
+}  // namespace hmx

@@ -1,0 +1,415 @@
+// Matrix construction, HBM storage layout, GPU assembly and the H-gemv driver.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+
+namespace hmx {
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+void DevBuf::alloc(size_t nbytes) {
+  release();
+  if (nbytes == 0) nbytes = 256;
+  HMX_CHECK(hipMalloc(&p, nbytes));
+  bytes = nbytes;
+}
+
+void DevBuf::ensure(size_t nbytes) {
+  if (nbytes > bytes) alloc(nbytes);
+}
+
+void upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
+  b.ensure(bytes);
+  if (bytes) HMX_CHECK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+}
+
+template <>
+Store<double>& Matrix::store<double>() {
+  return s64;
+}
+template <>
+Store<float>& Matrix::store<float>() {
+  return s32;
+}
+
+Matrix::Matrix(const ClusterTree* r, const ClusterTree* c, bool sym,
+               const AdmissibilityParams& adm, int vt, int rank_cap)
+    : value_type(vt), rows(r), cols(c), lower_sym(sym), rk_cap_max(rank_cap) {
+  if (sym && r != c) throw std::runtime_error("lower-symmetric storage needs rows tree == cols tree");
+  bt = build_block_tree(*r, *c, adm, sym);
+  build_storage();
+}
+
+void Matrix::device_init() {
+  if (own_stream) return;
+  HMX_CHECK(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
+  if (!stream) stream = own_stream;
+  overflow_.alloc(sizeof(int));
+}
+
+Matrix::~Matrix() {
+  fstate_.reset();
+  if (own_stream) {
+    (void)hipStreamSynchronize(own_stream);
+    (void)hipStreamDestroy(own_stream);
+  }
+}
+
+// One arena layout shared by both precisions: full leaves m*n, Rk leaves
+// (m + n) * cap with cap = min(m, n, rk_cap_max); every leaf 64-element aligned.
+void Matrix::build_storage() {
+  std::vector<LeafDesc> d(bt->leaves.size());
+  int64_t off = 0;
+  auto align = [](int64_t x) { return (x + 63) / 64 * 64; };
+  for (size_t i = 0; i < bt->leaves.size(); ++i) {
+    const Block* b = bt->leaves[i];
+    LeafDesc& L = d[i];
+    L.m = b->m();
+    L.n = b->n();
+    L.r0 = b->rows->offset;
+    L.c0 = b->cols->offset;
+    if (b->kind == Kind::Full) {
+      L.kind = 0;
+      L.cap = 0;
+      L.a = off;
+      L.b = -1;
+      off = align(off + (int64_t)L.m * L.n);
+    } else {
+      L.kind = 1;
+      L.cap = std::min(std::min(L.m, L.n), rk_cap_max);
+      L.a = off;
+      off = align(off + (int64_t)L.m * L.cap);
+      L.b = off;
+      off = align(off + (int64_t)L.n * L.cap);
+    }
+  }
+  s64.hdesc = d;
+  s32.hdesc = d;
+  s64.elems = s32.elems = (size_t)off;
+}
+
+template <typename T>
+void Matrix::init_store(Store<T>& st) {
+  if (!st.arena.p) {
+    st.arena.alloc(st.elems * sizeof(T));
+    st.ranks.alloc(st.hdesc.size() * sizeof(int));
+    upload(st.desc, st.hdesc.data(), st.hdesc.size() * sizeof(LeafDesc), stream);
+    std::vector<int> rk(st.hdesc.size());
+    for (size_t i = 0; i < rk.size(); ++i) rk[i] = st.hdesc[i].kind == 0 ? -1 : 0;
+    HMX_CHECK(hipMemcpyAsync(st.ranks.p, rk.data(), rk.size() * sizeof(int),
+                             hipMemcpyHostToDevice, stream));
+    HMX_CHECK(hipStreamSynchronize(stream));
+  }
+}
+template void Matrix::init_store<double>(Store<double>&);
+template void Matrix::init_store<float>(Store<float>&);
+
+namespace {
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void upload_points(const KernelSpec& k, const ClusterTree& t, DevBuf& pts, DevBuf& user,
+                   hipStream_t s) {
+  if (k.n != t.n) throw std::runtime_error("kernel point count differs from the cluster tree size");
+  std::vector<double> soa(3 * (size_t)t.n, 0.0);
+  for (int i = 0; i < t.n; ++i) {
+    const int u = t.indices[i];
+    for (int d = 0; d < k.dim && d < 3; ++d) soa[(size_t)d * t.n + i] = k.points[(size_t)u * k.dim + d];
+  }
+  upload(pts, soa.data(), soa.size() * sizeof(double), s);
+  upload(user, t.indices.data(), t.n * sizeof(int), s);
+  HMX_CHECK(hipStreamSynchronize(s));
+}
+}  // namespace
+
+void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
+  const double t0 = now();
+  device_init();
+  upload_points(k, *rows, pts_rows, user_rows, stream);
+  if (cols != rows) upload_points(k, *cols, pts_cols, user_cols, stream);
+  PointSet rp{pts_rows.as<double>(), user_rows.as<int>(), rows->n};
+  PointSet cp = (cols == rows) ? rp : PointSet{pts_cols.as<double>(), user_cols.as<int>(), cols->n};
+  KernelParams kp{k.kind, k.length, k.nugget};
+  init_store(s64);
+  HMX_CHECK(hipMemsetAsync(overflow_.p, 0, sizeof(int), stream));
+
+  std::vector<int> full, svd;
+  std::vector<int> aca;
+  for (size_t i = 0; i < bt->leaves.size(); ++i) {
+    const LeafDesc& L = s64.hdesc[i];
+    if (L.kind == 0) {
+      full.push_back((int)i);
+    } else if (std::max(L.m, L.n) < 100 || comp.method == 0 /*svd*/) {
+      if (std::max(L.m, L.n) >= 100)
+        throw std::runtime_error("SVD compression is implemented for blocks below 100 rows/cols");
+      svd.push_back((int)i);
+    } else {
+      aca.push_back((int)i);
+    }
+  }
+  DevBuf dfull, dsvd;
+  upload(dfull, full.data(), full.size() * sizeof(int), stream);
+  upload(dsvd, svd.data(), svd.size() * sizeof(int), stream);
+  launch_assemble_full<double>(s64.desc.as<LeafDesc>(), dfull.as<int>(), (int)full.size(),
+                               s64.data(), rp, cp, kp, stream);
+  launch_svd_compress<double>(s64.desc.as<LeafDesc>(), dsvd.as<int>(), (int)svd.size(), s64.data(),
+                              s64.ranks.as<int>(), overflow_.as<int>(), rp, cp, kp, comp.eps,
+                              stream);
+  // ACA blocks in batches bounded by a workspace budget.
+  const int64_t budget = (int64_t)4 << 27;  // doubles (4 GiB)
+  const int partial = comp.method == 2;
+  size_t i = 0;
+  DevBuf ws, dt;
+  while (i < aca.size()) {
+    std::vector<AcaTask> batch;
+    int64_t used = 0;
+    while (i < aca.size()) {
+      const LeafDesc& L = s64.hdesc[aca[i]];
+      const int kmax = std::min(std::min(L.m, L.n), 1024);
+      const int64_t need = aca_workspace(L.m, L.n, kmax);
+      if (!batch.empty() && used + need > budget) break;
+      batch.push_back(AcaTask{aca[i], kmax, used});
+      used += need;
+      ++i;
+    }
+    HMX_CHECK(hipStreamSynchronize(stream));
+    ws.ensure((size_t)used * sizeof(double));
+    upload(dt, batch.data(), batch.size() * sizeof(AcaTask), stream);
+    launch_aca<double>(s64.desc.as<LeafDesc>(), dt.as<AcaTask>(), (int)batch.size(), s64.data(),
+                       s64.ranks.as<int>(), overflow_.as<int>(), ws.as<double>(), rp, cp, kp,
+                       comp.eps, lowrank_eps, partial, stream);
+  }
+  HMX_CHECK(hipStreamSynchronize(stream));
+  int ov = 0;
+  HMX_CHECK(hipMemcpy(&ov, overflow_.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (ov) throw std::runtime_error("assembly: an Rk rank exceeded the panel capacity");
+  s64.valid = true;
+  if (value_type == HMAT_SIMPLE_PRECISION) {
+    init_store(s32);
+    launch_convert<double, float>(s64.data(), s32.data(), s64.elems, stream);
+    HMX_CHECK(hipMemcpyAsync(s32.ranks.p, s64.ranks.p, s64.hdesc.size() * sizeof(int),
+                             hipMemcpyDeviceToDevice, stream));
+    HMX_CHECK(hipStreamSynchronize(stream));
+    s32.valid = true;
+    s64.arena.release();
+    s64.valid = false;
+  }
+  assembled = true;
+  plan_.nrhs = 0;
+  stats.t_assemble = now() - t0;
+}
+
+// ---------------------------------------------------------------------------
+// H-gemv plan (host) and driver
+// ---------------------------------------------------------------------------
+void Matrix::build_gemv_plan(int nrhs) {
+  GemvPlan& P = plan_;
+  const auto& D = s64.hdesc;
+  const int nleaf = (int)D.size();
+  const int chunk = 1024;
+  std::vector<ZItem> zi;
+  std::vector<ZReduce> zr;
+  std::vector<int64_t> zoff_side0(nleaf, -1), zoff_side1(nleaf, -1);
+  int64_t zpart = 0, zbuf = 0;
+  for (int l = 0; l < nleaf; ++l) {
+    const LeafDesc& L = D[l];
+    if (L.kind != 1) continue;
+    const bool mir = bt->leaves[l]->mirrored() && lower_sym;
+    for (int side = 0; side < (mir ? 2 : 1); ++side) {
+      const int len = side == 0 ? L.n : L.m;
+      ZReduce red{l, (int)zi.size(), 0, 0, zbuf};
+      for (int r = 0; r < len; r += chunk) {
+        zi.push_back(ZItem{l, side, r, std::min(chunk, len - r), zpart});
+        zpart += (int64_t)L.cap * nrhs;
+      }
+      red.end = (int)zi.size();
+      zr.push_back(red);
+      (side == 0 ? zoff_side0 : zoff_side1)[l] = zbuf;
+      zbuf += (int64_t)L.cap * nrhs;
+    }
+  }
+  // y pass: contributions grouped by destination leaf cluster of the rows tree
+  const auto& RL = rows->leaves;
+  std::vector<std::vector<YContrib>> per(RL.size());
+  std::vector<std::vector<int64_t>> slots_for(RL.size());
+  int64_t slot = 0;
+  for (int l = 0; l < nleaf; ++l) {
+    const Block* b = bt->leaves[l];
+    const LeafDesc& L = D[l];
+    const bool mir = b->mirrored() && lower_sym;
+    if (L.kind == 0) {
+      if (!b->rows->is_leaf() || !b->cols->is_leaf())
+        throw std::runtime_error("gemv: full leaf spanning several leaf clusters");
+      YContrib c{l, 0, 0, 0, -1};
+      if (mir) {
+        c.z = slot;
+        slots_for[b->cols->leaf_index].push_back(slot);
+        slot += (int64_t)L.n * nrhs;
+      }
+      per[b->rows->leaf_index].push_back(c);
+    } else {
+      for (int q = b->rows->first_leaf; q < b->rows->first_leaf + b->rows->nleaves; ++q)
+        per[q].push_back(YContrib{l, 2, RL[q]->offset - L.r0, 0, zoff_side0[l]});
+      if (mir)
+        for (int q = b->cols->first_leaf; q < b->cols->first_leaf + b->cols->nleaves; ++q)
+          per[q].push_back(YContrib{l, 3, RL[q]->offset - L.c0, 0, zoff_side1[l]});
+    }
+  }
+  std::vector<YItem> yi;
+  std::vector<YContrib> contrib;
+  std::vector<int64_t> slist;
+  for (size_t q = 0; q < RL.size(); ++q) {
+    if (RL[q]->size > 64 * 4) throw std::runtime_error("gemv: leaf clusters above 256 rows");
+    YItem it{RL[q]->offset, RL[q]->size, (int)contrib.size(), 0, (int)slist.size(), 0};
+    contrib.insert(contrib.end(), per[q].begin(), per[q].end());
+    slist.insert(slist.end(), slots_for[q].begin(), slots_for[q].end());
+    it.end = (int)contrib.size();
+    it.send = (int)slist.size();
+    yi.push_back(it);
+  }
+  upload(P.zitems, zi.data(), zi.size() * sizeof(ZItem), stream);
+  upload(P.zred, zr.data(), zr.size() * sizeof(ZReduce), stream);
+  upload(P.yitems, yi.data(), yi.size() * sizeof(YItem), stream);
+  upload(P.contrib, contrib.data(), contrib.size() * sizeof(YContrib), stream);
+  upload(P.slotlist, slist.data(), slist.size() * sizeof(int64_t), stream);
+  P.zpart.ensure(std::max<int64_t>(zpart, 1) * sizeof(double));
+  P.z.ensure(std::max<int64_t>(zbuf, 1) * sizeof(double));
+  P.slots.ensure(std::max<int64_t>(slot, 1) * sizeof(double));
+  P.acc.ensure((size_t)rows->n * nrhs * sizeof(double));
+  P.nz = (int)zi.size();
+  P.nzred = (int)zr.size();
+  P.ny = (int)yi.size();
+  P.nrhs = nrhs;
+  HMX_CHECK(hipStreamSynchronize(stream));
+}
+
+template <typename T>
+void Matrix::gemv_device(double alpha, const T* x, double beta, T* y, int nrhs, int ldx, int ldy,
+                         const Store<T>& st) {
+  if (!st.valid) throw std::runtime_error("gemv: matrix data not available in this precision");
+  if (plan_.nrhs != nrhs) build_gemv_plan(nrhs);
+  GemvPlan& P = plan_;
+  const LeafDesc* desc = st.desc.template as<LeafDesc>();
+  const int* ranks = st.ranks.template as<int>();
+  launch_gemv_zpass<T>(desc, ranks, st.data(), P.zitems.as<ZItem>(), P.nz, x, ldx, nrhs,
+                       P.zpart.as<double>(), stream);
+  launch_gemv_zreduce(P.zred.as<ZReduce>(), P.nzred, ranks, P.zpart.as<double>(),
+                      P.zitems.as<ZItem>(), nrhs, desc, P.z.as<double>(), stream);
+  launch_gemv_ypass<T>(desc, ranks, st.data(), P.yitems.as<YItem>(), P.ny,
+                       P.contrib.as<YContrib>(), P.z.as<double>(), P.slots.as<double>(), x, ldx,
+                       nrhs, P.acc.as<double>(), rows->n, stream);
+  launch_gemv_yfinal<T>(P.yitems.as<YItem>(), P.ny, P.slotlist.as<int64_t>(),
+                        P.slots.as<double>(), P.acc.as<double>(), rows->n, y, ldy, nrhs, alpha,
+                        beta, stream);
+}
+template void Matrix::gemv_device<double>(double, const double*, double, double*, int, int, int,
+                                          const Store<double>&);
+template void Matrix::gemv_device<float>(double, const float*, double, float*, int, int, int,
+                                         const Store<float>&);
+
+namespace {
+template <typename T>
+void permute_in(const T* user, T* hm, const std::vector<int>& idx, int n, int nrhs) {
+  for (int r = 0; r < nrhs; ++r)
+    for (int i = 0; i < n; ++i) hm[i + (size_t)r * n] = user[idx[i] + (size_t)r * n];
+}
+template <typename T>
+void permute_out(const T* hm, T* user, const std::vector<int>& idx, int n, int nrhs) {
+  for (int r = 0; r < nrhs; ++r)
+    for (int i = 0; i < n; ++i) user[idx[i] + (size_t)r * n] = hm[i + (size_t)r * n];
+}
+
+template <typename T>
+void gemv_host_t(Matrix& M, const void* alpha, void* x, const void* beta, void* y, int nrhs,
+                 bool permute) {
+  const int n = M.n_rows(), nc = M.n_cols();
+  std::vector<T> hx((size_t)nc * nrhs), hy((size_t)n * nrhs);
+  const T a = *(const T*)alpha, b = *(const T*)beta;
+  if (permute) {
+    permute_in((const T*)x, hx.data(), M.cols->indices, nc, nrhs);
+    permute_in((const T*)y, hy.data(), M.rows->indices, n, nrhs);
+  } else {
+    std::memcpy(hx.data(), x, hx.size() * sizeof(T));
+    std::memcpy(hy.data(), y, hy.size() * sizeof(T));
+  }
+  DevBuf dx, dy;
+  upload(dx, hx.data(), hx.size() * sizeof(T), M.stream);
+  upload(dy, hy.data(), hy.size() * sizeof(T), M.stream);
+  M.gemv_device<T>((double)a, dx.as<T>(), (double)b, dy.as<T>(), nrhs, nc, n, M.store<T>());
+  HMX_CHECK(hipMemcpyAsync(hy.data(), dy.p, hy.size() * sizeof(T), hipMemcpyDeviceToHost, M.stream));
+  HMX_CHECK(hipStreamSynchronize(M.stream));
+  if (permute)
+    permute_out(hy.data(), (T*)y, M.rows->indices, n, nrhs);
+  else
+    std::memcpy(y, hy.data(), hy.size() * sizeof(T));
+}
+}  // namespace
+
+void Matrix::gemv_host(char trans, const void* alpha, void* x, const void* beta, void* y, int nrhs,
+                       bool permute) {
+  if (!assembled) throw std::runtime_error("gemv: matrix not assembled");
+  if (factorization != -1) throw std::runtime_error("gemv: matrix has been factorised");
+  if (trans != 'N' && !lower_sym) throw std::runtime_error("gemv: only 'N' for non-symmetric matrices");
+  if (value_type == HMAT_DOUBLE_PRECISION)
+    gemv_host_t<double>(*this, alpha, x, beta, y, nrhs, permute);
+  else if (value_type == HMAT_SIMPLE_PRECISION)
+    gemv_host_t<float>(*this, alpha, x, beta, y, nrhs, permute);
+  else
+    throw std::runtime_error("gemv: complex types are not supported by this engine");
+}
+
+void Matrix::info(hmat_info_t& out) const {
+  // HMatrix::info h_matrix.cpp:464-502
+  std::memset(&out, 0, sizeof(out));
+  std::vector<int> rk(bt->leaves.size(), 0);
+  const DevBuf& rb = s64.ranks.p ? s64.ranks : s32.ranks;
+  if (rb.p) HMX_CHECK(hipMemcpy(rk.data(), rb.p, rk.size() * sizeof(int), hipMemcpyDeviceToHost));
+  std::function<void(const Block*)> rec;
+  int count_nodes = 0;
+  rec = [&](const Block* b) {
+    ++count_nodes;
+    const size_t r = b->m(), c = b->n();
+    if (r == 0 || c == 0) return;
+    if (b->is_leaf()) {
+      const size_t s = r * c;
+      out.uncompressed_size += s;
+      if (b->kind == Kind::Rk) {
+        const size_t k = std::max(rk[b->leaf], 0);
+        const size_t mem = k * (r + c);
+        out.compressed_size += mem;
+        if ((int)(r + c) > out.largest_rk_dim_cols + out.largest_rk_dim_rows) {
+          out.largest_rk_dim_cols = (int)c;
+          out.largest_rk_dim_rows = (int)r;
+        }
+        const size_t old = ((size_t)out.largest_rk_mem_cols + out.largest_rk_mem_rows) * out.largest_rk_mem_rank;
+        if (mem > old) {
+          out.largest_rk_mem_cols = (int)c;
+          out.largest_rk_mem_rows = (int)r;
+          out.largest_rk_mem_rank = (int)k;
+        }
+        out.rk_count++;
+        out.rk_size += s;
+      } else {
+        out.compressed_size += s;
+        out.full_count++;
+        out.full_size += s;
+      }
+    } else {
+      for (int i = 0; i < b->nrow * b->ncol; ++i)
+        if (b->child[i]) rec(b->child[i]);
+    }
+  };
+  rec(bt->root);
+  out.nr_block_clusters = count_nodes;
+}
+
+}  // namespace hmx

@@ -1,0 +1,144 @@
+// The MI355X engine behind the C API: one Matrix object per hmat_matrix_t.
+// The block tree lives on the host; every leaf lives in one HBM arena per
+// precision; the hot operations run as batched HIP kernels on the matrix's
+// stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "hmat/hmat.h"
+#include "kernels.hpp"
+#include "tree.hpp"
+
+namespace hmx {
+
+// Owning device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release();
+  void alloc(size_t nbytes);          // discards old content
+  void ensure(size_t nbytes);         // grows (discards) if too small
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+void upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s);
+
+// Leaf storage of one precision.
+template <typename T>
+struct Store {
+  DevBuf arena;
+  DevBuf ranks;     // int per leaf (full: -1)
+  DevBuf desc;      // LeafDesc per leaf
+  std::vector<LeafDesc> hdesc;
+  size_t elems = 0;
+  bool valid = false;
+  T* data() const { return arena.as<T>(); }
+};
+
+struct CompressionSpec {
+  int method = 3;     // hmat_compress_t
+  double eps = 1e-4;
+};
+
+struct KernelSpec {
+  int kind = kExponential;
+  double length = 1.0;
+  double nugget = 0.0;
+  int dim = 3;
+  int n = 0;
+  std::vector<double> points;  // user numbering
+};
+
+struct GemvPlan {
+  int nrhs = 0;
+  DevBuf zitems, zred, yitems, contrib, slotlist;
+  int nz = 0, nzred = 0, ny = 0;
+  DevBuf zpart, z, slots, acc;
+  size_t zpart_elems = 0, z_elems = 0, slot_elems = 0;
+};
+
+struct Stats {
+  double t_assemble = 0, t_factor = 0, t_solve = 0;
+  long tasks = 0, waves = 0, launches = 0, segments = 0;
+  size_t temp_bytes = 0;
+};
+
+class Matrix {
+ public:
+  Matrix(const ClusterTree* rows, const ClusterTree* cols, bool lower_sym,
+         const AdmissibilityParams& adm, int value_type, int rank_cap);
+  ~Matrix();
+
+  int value_type;        // hmat_value_t
+  const ClusterTree* rows;
+  const ClusterTree* cols;
+  bool own_rows = false, own_cols = false;
+  bool lower_sym;
+  double lowrank_eps = 1e-4;
+  int rk_cap_max = 160;   // capacity limit of Rk panels
+  std::unique_ptr<BlockTree> bt;
+  hipStream_t stream = nullptr;       // stream every launch goes to
+  hipStream_t own_stream = nullptr;   // the engine's own stream
+  bool user_stream = false;
+
+  Store<double> s64;
+  Store<float> s32;
+  bool assembled = false;
+  int factorization = -1;    // hmat_factorization_t
+  bool factor32 = false;      // factor stored in s32 (mixed precision)
+  int overflow = 0;
+  Stats stats;
+
+  // points of the kernel, hmat order (rows tree), SoA, plus user indices
+  DevBuf pts_rows, user_rows;
+  DevBuf pts_cols, user_cols;
+
+  void device_init();   // stream + device scratch, created on first GPU use
+  void assemble(const KernelSpec& k, const CompressionSpec& c);
+  // y = alpha * A * x + beta * y, device pointers in hmat numbering, ld = N.
+  template <typename T>
+  void gemv_device(double alpha, const T* x, double beta, T* y, int nrhs, int ldx, int ldy,
+                   const Store<T>& st);
+  void gemv_host(char trans, const void* alpha, void* x, const void* beta, void* y, int nrhs,
+                 bool permute);
+
+  // factorisation / solve (factor.cpp)
+  void factorize(int algo, bool fp32);
+  template <typename T>
+  void solve_device(T* b, int nrhs, int ldb, const Store<T>& st);
+  void solve_host(void* b, int nrhs, bool permute);
+  double logdet();
+  // Iterative refinement (FP32 factor, FP64 residual with the FP64 matrix).
+  int solve_refined(double* b_dev, int nrhs, int ldb, double tol, int max_iter, double* resid_out);
+
+  void info(hmat_info_t& out) const;
+
+  int n_rows() const { return rows->n; }
+  int n_cols() const { return cols->n; }
+
+  template <typename T>
+  Store<T>& store();
+
+ private:
+  void build_storage();
+  template <typename T>
+  void init_store(Store<T>& st);
+  void build_gemv_plan(int nrhs);
+  GemvPlan plan_;
+  DevBuf overflow_;
+  std::shared_ptr<struct FactorState> fstate_;
+  friend struct FactorState;
+};
+
+}  // namespace hmx

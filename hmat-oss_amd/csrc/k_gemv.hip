@@ -1,0 +1,244 @@
+// Leaf-level H-gemv (h_matrix.cpp:792-857, rk_matrix.cpp:156-206) as four
+// deterministic passes over HBM-resident leaves:
+//   zpass   : z = B^T x[cols] (and z' = A^T x[rows] for mirrored Rk) per panel chunk
+//   zreduce : sum the chunk partials in a fixed order
+//   ypass   : one workgroup per destination leaf cluster: full leaves of that
+//             block row (M x, plus M^T x into a slot when mirrored), Rk A z and
+//             mirrored Rk B z' restricted to the cluster's rows
+//   yfinal  : y = beta y + alpha (acc + mirrored full slots), fixed order
+// Every stored leaf is streamed once per pass that needs it; accumulation is
+// in FP64 whatever the storage type.
+#include "kernels.hpp"
+
+namespace hmx {
+
+namespace {
+
+constexpr int kMaxRowsPerLane = 4;  // destination clusters of up to 256 rows
+constexpr int kRhsChunk = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_zpass(const LeafDesc* desc, const int* ranks,
+                                                  const T* arena, const ZItem* items, const T* x,
+                                                  int ldx, int nrhs, double* zpart) {
+  const ZItem it = items[blockIdx.x];
+  const LeafDesc d = desc[it.leaf];
+  const int k = ranks[it.leaf];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int ld = it.side == 0 ? d.n : d.m;
+  const T* P = arena + (it.side == 0 ? d.b : d.a) + it.row0;
+  const T* xs = x + (it.side == 0 ? d.c0 : d.r0) + it.row0;
+  double* z = zpart + it.zslot;
+  for (int l = w; l < k; l += kWaves) {
+    const T* col = P + (size_t)l * ld;
+    for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk) {
+      const int nr = nrhs - r0 < kRhsChunk ? nrhs - r0 : kRhsChunk;
+      double s[kRhsChunk];
+#pragma unroll
+      for (int r = 0; r < kRhsChunk; ++r) s[r] = 0.0;
+      for (int i = lane; i < it.rows; i += kWave) {
+        const double p = (double)col[i];
+#pragma unroll
+        for (int r = 0; r < kRhsChunk; ++r)
+          if (r < nr) s[r] += p * (double)xs[i + (size_t)(r0 + r) * ldx];
+      }
+#pragma unroll
+      for (int r = 0; r < kRhsChunk; ++r) {
+        if (r < nr) {
+          const double v = wave_sum(s[r]);
+          if (lane == 0) z[l + (size_t)(r0 + r) * d.cap] = v;
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_zreduce(const ZReduce* red, const int* ranks, const double* zpart,
+                          const ZItem* items, int nrhs, const LeafDesc* desc, double* z) {
+  const ZReduce rd = red[blockIdx.x];
+  const int k = ranks[rd.leaf];
+  const int cap = desc[rd.leaf].cap;
+  for (int e = threadIdx.x; e < k * nrhs; e += blockDim.x) {
+    const int l = e % k, r = e / k;
+    double s = 0.0;
+    for (int q = rd.beg; q < rd.end; ++q) s += zpart[items[q].zslot + l + (size_t)r * cap];
+    z[rd.z + l + (size_t)r * cap] = s;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const int* ranks,
+                                                  const T* arena, const YItem* items,
+                                                  const YContrib* contrib, const double* z,
+                                                  double* slots, const T* x, int ldx, int nrhs,
+                                                  double* acc_out, int ldacc) {
+  __shared__ double red[kWaves][kWave * kMaxRowsPerLane];
+  const YItem it = items[blockIdx.x];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  for (int rc = 0; rc < nrhs; rc += kRhsChunk) {
+    const int nr = nrhs - rc < kRhsChunk ? nrhs - rc : kRhsChunk;
+    double acc[kMaxRowsPerLane][kRhsChunk];
+#pragma unroll
+    for (int q = 0; q < kMaxRowsPerLane; ++q)
+#pragma unroll
+      for (int r = 0; r < kRhsChunk; ++r) acc[q][r] = 0.0;
+    for (int ci = it.beg; ci < it.end; ++ci) {
+      const YContrib c = contrib[ci];
+      const LeafDesc d = desc[c.leaf];
+      if (c.kind == 0) {
+        const T* M = arena + d.a;
+        const bool mir = c.z >= 0;
+        for (int j = w; j < d.n; j += kWaves) {
+          const T* col = M + (size_t)j * d.m;
+          double t[kRhsChunk];
+#pragma unroll
+          for (int r = 0; r < kRhsChunk; ++r) t[r] = 0.0;
+#pragma unroll
+          for (int q = 0; q < kMaxRowsPerLane; ++q) {
+            const int i = lane + q * kWave;
+            if (i < d.m) {
+              const double v = (double)col[i];
+#pragma unroll
+              for (int r = 0; r < kRhsChunk; ++r) {
+                if (r < nr) {
+                  acc[q][r] += v * (double)x[d.c0 + j + (size_t)(rc + r) * ldx];
+                  if (mir) t[r] += v * (double)x[d.r0 + i + (size_t)(rc + r) * ldx];
+                }
+              }
+            }
+          }
+          if (mir) {
+#pragma unroll
+            for (int r = 0; r < kRhsChunk; ++r) {
+              if (r < nr) {
+                const double s = wave_sum(t[r]);
+                if (lane == 0) slots[c.z + j + (size_t)(rc + r) * d.n] = s;
+              }
+            }
+          }
+        }
+      } else {
+        const int k = ranks[c.leaf];
+        const bool isA = c.kind == 2;
+        const int ld = isA ? d.m : d.n;
+        const T* P = arena + (isA ? d.a : d.b) + c.row0;
+        const double* zz = z + c.z;
+        for (int l = w; l < k; l += kWaves) {
+          const T* col = P + (size_t)l * ld;
+          double zl[kRhsChunk];
+#pragma unroll
+          for (int r = 0; r < kRhsChunk; ++r) zl[r] = (r < nr) ? zz[l + (size_t)(rc + r) * d.cap] : 0.0;
+#pragma unroll
+          for (int q = 0; q < kMaxRowsPerLane; ++q) {
+            const int i = lane + q * kWave;
+            if (i < it.len) {
+              const double v = (double)col[i];
+#pragma unroll
+              for (int r = 0; r < kRhsChunk; ++r) acc[q][r] += v * zl[r];
+            }
+          }
+        }
+      }
+    }
+    // fixed-order reduction over the waves
+    for (int r = 0; r < nr; ++r) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kMaxRowsPerLane; ++q) red[w][lane + q * kWave] = acc[q][r];
+      __syncthreads();
+      for (int i = threadIdx.x; i < it.len; i += kBlock) {
+        double s = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ++ww) s += red[ww][i];
+        acc_out[it.r0 + i + (size_t)(rc + r) * ldacc] = s;
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_yfinal(const YItem* items, const int64_t* slot_list, const double* slots,
+                         const double* acc, T* y, int ldy, int ldacc, int nrhs, double alpha,
+                         double beta) {
+  const YItem it = items[blockIdx.x];
+  for (int e = threadIdx.x; e < it.len * nrhs; e += blockDim.x) {
+    const int i = e % it.len, r = e / it.len;
+    double s = acc[it.r0 + i + (size_t)r * ldacc];
+    for (int q = it.sbeg; q < it.send; ++q) s += slots[slot_list[q] + i + (size_t)r * it.len];
+    T* py = y + it.r0 + i + (size_t)r * ldy;
+    const double old = (beta == 0.0) ? 0.0 : beta * (double)(*py);
+    *py = (T)(old + alpha * s);
+  }
+}
+
+template <typename S, typename D>
+__global__ void k_convert(const S* src, D* dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (D)src[i];
+}
+
+}  // namespace
+
+template <typename S, typename D>
+void launch_convert(const S* src, D* dst, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL((k_convert<S, D>), dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
+  HMX_CHECK(hipGetLastError());
+}
+template void launch_convert<double, float>(const double*, float*, size_t, hipStream_t);
+template void launch_convert<float, double>(const float*, double*, size_t, hipStream_t);
+
+template <typename T>
+void launch_gemv_zpass(const LeafDesc* desc, const int* ranks, const T* arena, const ZItem* items,
+                       int count, const T* x, int ldx, int nrhs, double* zpart, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_zpass<T>, dim3(count), dim3(kBlock), 0, s, desc, ranks, arena, items, x,
+                     ldx, nrhs, zpart);
+  HMX_CHECK(hipGetLastError());
+}
+
+void launch_gemv_zreduce(const ZReduce* red, int count, const int* ranks, const double* zpart,
+                         const ZItem* items, int nrhs, const LeafDesc* desc, double* z,
+                         hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_zreduce, dim3(count), dim3(128), 0, s, red, ranks, zpart, items, nrhs,
+                     desc, z);
+  HMX_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_gemv_ypass(const LeafDesc* desc, const int* ranks, const T* arena, const YItem* items,
+                       int count, const YContrib* contrib, const double* z, double* slots,
+                       const T* x, int ldx, int nrhs, double* acc, int ldacc, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_ypass<T>, dim3(count), dim3(kBlock), 0, s, desc, ranks, arena, items,
+                     contrib, z, slots, x, ldx, nrhs, acc, ldacc);
+  HMX_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_gemv_yfinal(const YItem* items, int count, const int64_t* slot_list,
+                        const double* slots, const double* acc, int ldacc, T* y, int ldy,
+                        int nrhs, double alpha, double beta, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_yfinal<T>, dim3(count), dim3(128), 0, s, items, slot_list, slots, acc, y,
+                     ldy, ldacc, nrhs, alpha, beta);
+  HMX_CHECK(hipGetLastError());
+}
+
+#define HMX_INST(T)                                                                            \
+  template void launch_gemv_zpass<T>(const LeafDesc*, const int*, const T*, const ZItem*, int, \
+                                     const T*, int, int, double*, hipStream_t);                 \
+  template void launch_gemv_ypass<T>(const LeafDesc*, const int*, const T*, const YItem*, int, \
+                                     const YContrib*, const double*, double*, const T*, int,   \
+                                     int, double*, int, hipStream_t);                           \
+  template void launch_gemv_yfinal<T>(const YItem*, int, const int64_t*, const double*,        \
+                                      const double*, int, T*, int, int, double, double,         \
+                                      hipStream_t);
+HMX_INST(double)
+HMX_INST(float)
+
+}  // namespace hmx

@@ -1,0 +1,114 @@
+// Host-visible declarations of the batched HIP kernels (launch wrappers).
+// Every wrapper enqueues on the given stream and never synchronises.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dev_common.hpp"
+
+namespace hmx {
+
+// Per-leaf descriptor in HBM.  Full leaves: column-major m x n at 'a' (ld = m).
+// Rk leaves: A panel m x cap at 'a' (ld = m), B panel n x cap at 'b' (ld = n),
+// current rank in the separate rank array.
+struct LeafDesc {
+  int m, n;
+  int r0, c0;   // offsets in hmat numbering
+  int kind;     // 0 full, 1 rk
+  int cap;
+  int64_t a, b;
+};
+
+// Points of one cluster tree, hmat order, structure of arrays (x | y | z).
+struct PointSet {
+  const double* xyz;   // 3 * n
+  const int* user;     // hmat index -> user index (for the nugget test)
+  int n;
+};
+
+// ---- assembly ---------------------------------------------------------------
+template <typename T>
+void launch_assemble_full(const LeafDesc* desc, const int* list, int count, T* arena,
+                          PointSet rp, PointSet cp, KernelParams kp, hipStream_t s);
+
+// Admissible leaves with max(m, n) < 100: dense evaluation + Jacobi SVD +
+// truncation (compression.cpp:224-251).  ranks[leaf] set; overflow[0] |= 1 if
+// a rank exceeds the leaf capacity.
+template <typename T>
+void launch_svd_compress(const LeafDesc* desc, const int* list, int count, T* arena, int* ranks,
+                         int* overflow, PointSet rp, PointSet cp, KernelParams kp, double eps,
+                         hipStream_t s);
+
+struct AcaTask {
+  int leaf;
+  int kmax;       // capacity of the ACA panels
+  int64_t ws;     // offset (doubles) of this task's workspace
+};
+
+// Workspace (doubles) needed by one ACA task followed by its truncation.
+int64_t aca_workspace(int m, int n, int kmax);
+
+// ACA+ (compression.cpp:555-724; delegates to ACA partial when n*100 < m,
+// or always when partial != 0), then RkMatrix::truncate (rk_matrix.cpp:490-585)
+// into the leaf storage.
+template <typename T>
+void launch_aca(const LeafDesc* desc, const AcaTask* tasks, int count, T* arena, int* ranks,
+                int* overflow, double* ws, PointSet rp, PointSet cp, KernelParams kp,
+                double aca_eps, double trunc_eps, int partial, hipStream_t s);
+
+// ---- H-gemv -----------------------------------------------------------------
+// Work items of the z pass: z_part = P^T x over a row range of a panel.
+struct ZItem {
+  int leaf;
+  int side;       // 0: B panel against x[cols]  (z),  1: A panel against x[rows] (z')
+  int row0, rows; // local row range inside the panel
+  int64_t zslot;  // output slot (cap * nrhs doubles)
+};
+// y pass: one work item per destination leaf cluster.  Contributions:
+//   kind 0: full leaf with rows == destination: acc += M x[cols]; if the leaf is
+//           mirrored (symmetric storage) also slot[fslot] = M^T x[rows]
+//   kind 2: Rk A panel rows [row0, row0+len): acc += A z
+//   kind 3: Rk B panel rows [row0, row0+len) (mirrored Rk): acc += B z'
+// and the final pass adds the mirrored-full slots that target the cluster:
+//   y[r0:r0+len] = beta y + alpha (acc + sum slots)
+struct YItem {
+  int r0, len;
+  int beg, end;     // contribution range
+  int sbeg, send;   // slot-list range (indices into slot_list)
+};
+struct YContrib {
+  int leaf;
+  int kind;
+  int row0;
+  int pad;
+  int64_t z;       // z / z' offset (doubles) for rk, slot offset (doubles) for mirrored full, else -1
+};
+
+template <typename T>
+void launch_gemv_zpass(const LeafDesc* desc, const int* ranks, const T* arena, const ZItem* items,
+                       int count, const T* x, int ldx, int nrhs, double* zpart, hipStream_t s);
+// Sums z partial slots per (leaf, side): z[z] = sum of slots of items [beg, end).
+struct ZReduce {
+  int leaf;
+  int beg, end;
+  int pad;
+  int64_t z;
+};
+void launch_gemv_zreduce(const ZReduce* red, int count, const int* ranks, const double* zpart,
+                         const ZItem* items, int nrhs, const LeafDesc* desc, double* z,
+                         hipStream_t s);
+template <typename T>
+void launch_gemv_ypass(const LeafDesc* desc, const int* ranks, const T* arena, const YItem* items,
+                       int count, const YContrib* contrib, const double* z, double* slots,
+                       const T* x, int ldx, int nrhs, double* acc, int ldacc, hipStream_t s);
+template <typename T>
+void launch_gemv_yfinal(const YItem* items, int count, const int64_t* slot_list,
+                        const double* slots, const double* acc, int ldacc, T* y, int ldy,
+                        int nrhs, double alpha, double beta, hipStream_t s);
+
+// Elementwise conversion of a whole arena (same layout in both precisions).
+template <typename S, typename D>
+void launch_convert(const S* src, D* dst, size_t n, hipStream_t s);
+
+}  // namespace hmx

@@ -1,0 +1,363 @@
+"""Thin ctypes binding over libhmat_mi355x.so (the C ABI in include/hmat).
+
+The structures below mirror include/hmat/hmat.h and include/hmat/hmat_mi355x.h
+field by field.  ``HMatrix`` drives the same sequence of calls a C application
+makes (see examples/c-cholesky.c of hmat-oss): clustering -> cluster tree ->
+admissibility -> create_empty_hmatrix_admissibility -> assemble_generic ->
+factorize_generic -> solve / gemv.
+
+The binding never computes anything itself: when the shared library is
+missing, ``load()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libhmat_mi355x.so")
+
+HMAT_SIMPLE_PRECISION, HMAT_DOUBLE_PRECISION = 0, 1
+FACT_NONE, FACT_LU, FACT_LDLT, FACT_LLT = -1, 0, 1, 2
+KERNEL_EXPONENTIAL, KERNEL_GAUSSIAN = 0, 1
+
+_vp = C.c_void_p
+
+
+class hmat_info_t(C.Structure):
+    _fields_ = [("full_zeros", C.c_int), ("full_size", C.c_size_t), ("rk_size", C.c_size_t),
+                ("full_count", C.c_size_t), ("rk_count", C.c_size_t),
+                ("compressed_size", C.c_size_t), ("uncompressed_size", C.c_size_t),
+                ("nr_block_clusters", C.c_int), ("largest_rk_dim_rows", C.c_int),
+                ("largest_rk_dim_cols", C.c_int), ("largest_rk_mem_rows", C.c_int),
+                ("largest_rk_mem_cols", C.c_int), ("largest_rk_mem_rank", C.c_int)]
+
+
+class hmat_assemble_context_t(C.Structure):
+    _fields_ = [("assembly", _vp), ("simple_compute", _vp), ("block_compute", _vp),
+                ("advanced_compute", _vp), ("user_context", _vp), ("prepare", _vp),
+                ("compression", _vp), ("lower_symmetric", C.c_int), ("factorization", C.c_int),
+                ("progress", _vp)]
+
+
+class hmat_factorization_context_t(C.Structure):
+    _fields_ = [("factorization", C.c_int), ("progress", _vp)]
+
+
+class hmat_solve_context_t(C.Structure):
+    _fields_ = [("values", _vp), ("nr_rhs", C.c_int), ("no_permutation", C.c_int),
+                ("lower", C.c_int), ("upper", C.c_int), ("progress", _vp)]
+
+
+class hmat_mi355x_kernel_t(C.Structure):
+    _fields_ = [("kind", C.c_int), ("dimension", C.c_int), ("n", C.c_int),
+                ("points", C.POINTER(C.c_double)), ("length", C.c_double),
+                ("nugget", C.c_double)]
+
+
+_CREATE = C.CFUNCTYPE(_vp, _vp, _vp, C.c_int, _vp)
+_P_INT = C.CFUNCTYPE(C.c_int, _vp)
+_ASSEMBLE = C.CFUNCTYPE(C.c_int, _vp, C.POINTER(hmat_assemble_context_t))
+_FACTOR = C.CFUNCTYPE(C.c_int, _vp, C.POINTER(hmat_factorization_context_t))
+_LOGDET = C.CFUNCTYPE(C.c_int, _vp, C.POINTER(C.c_double))
+_SOLVE = C.CFUNCTYPE(C.c_int, _vp, _vp, C.c_int)
+_SOLVE_GEN = C.CFUNCTYPE(C.c_int, _vp, C.POINTER(hmat_solve_context_t))
+_REORDER = C.CFUNCTYPE(C.c_int, _vp, _vp, C.c_int, _vp, C.c_int)
+_GEMV = C.CFUNCTYPE(C.c_int, C.c_char, _vp, _vp, _vp, _vp, _vp, C.c_int)
+_GEMM_DENSE = C.CFUNCTYPE(C.c_int, C.c_char, C.c_char, C.c_char, _vp, _vp, _vp, _vp, _vp, C.c_int)
+_INFO = C.CFUNCTYPE(C.c_int, _vp, C.POINTER(hmat_info_t))
+_VOID_EPS = C.CFUNCTYPE(None, _vp, C.c_double)
+_OWN = C.CFUNCTYPE(None, _vp, C.c_int, C.c_int)
+_INIT = C.CFUNCTYPE(C.c_int)
+
+# slot order of hmat_interface_t (hmat.h:717-1103); unused slots are plain pointers
+_SLOTS = [
+    ("create_empty_hmatrix_admissibility", _CREATE), ("own_cluster_trees", _OWN),
+    ("set_low_rank_epsilon", _VOID_EPS), ("assemble_generic", _ASSEMBLE), ("copy", _vp),
+    ("copy_struct", _vp), ("norm", _vp), ("logdet", _LOGDET), ("inverse", _vp),
+    ("factorize_generic", _FACTOR), ("destroy", _P_INT), ("get_child", _vp),
+    ("destroy_child", _vp), ("solve_generic", _SOLVE_GEN), ("solve_mat", _vp),
+    ("solve_systems", _SOLVE), ("solve_dense", _SOLVE), ("transpose", _vp), ("scale", _vp),
+    ("truncate", _vp), ("vector_reorder", _REORDER), ("vector_restore", _REORDER),
+    ("gemm", _vp), ("axpy", _vp), ("trsm", _vp), ("gemv", _GEMV), ("gemm_scalar", _GEMV),
+    ("full_gemm", _vp), ("gemm_dense", _GEMM_DENSE), ("add_identity", _vp), ("init", _INIT),
+    ("finalize", _INIT), ("get_info", _INFO), ("get_profile", _vp), ("get_ratio", _vp),
+    ("FPcompress", _vp), ("FPdecompress", _vp), ("GetFPCompressionSettings", _vp),
+    ("SetFPCompressionSettings", _vp), ("SetFPCompressionSettingsParams", _vp),
+    ("dump_info", _vp), ("get_cluster_trees", _vp), ("set_cluster_trees", _vp),
+    ("extract_diagonal", _vp), ("solve_lower_triangular", _vp),
+    ("solve_lower_triangular_dense", _vp), ("get_block", _vp), ("get_values", _vp),
+    ("walk", _vp), ("apply_on_leaf", _vp), ("value_type", C.c_int), ("internal", _vp),
+    ("read_struct", _vp), ("write_struct", _vp), ("read_data", _vp), ("write_data", _vp),
+    ("set_progressbar", _vp), ("extract_diagonal_block", _vp),
+]
+
+
+class hmat_interface_t(C.Structure):
+    _fields_ = _SLOTS
+
+
+# exported symbols the binding uses, with their signatures
+_EXPORTS = {
+    "hmat_init_mi355x_interface": (None, [C.POINTER(hmat_interface_t), C.c_int]),
+    "hmat_create_clustering_median": (_vp, []),
+    "hmat_create_clustering_max_dof": (_vp, [_vp, C.c_int]),
+    "hmat_delete_clustering": (None, [_vp]),
+    "hmat_create_cluster_tree": (_vp, [_vp, C.c_int, C.c_int, _vp]),
+    "hmat_delete_cluster_tree": (None, [_vp]),
+    "hmat_cluster_get_indices": (C.POINTER(C.c_int), [_vp]),
+    "hmat_tree_nodes_count": (C.c_int, [_vp]),
+    "hmat_create_admissibility_standard": (_vp, [C.c_double]),
+    "hmat_delete_admissibility": (None, [_vp]),
+    "hmat_create_compression_aca_plus": (_vp, [C.c_double]),
+    "hmat_create_compression_aca_partial": (_vp, [C.c_double]),
+    "hmat_create_compression_svd": (_vp, [C.c_double]),
+    "hmat_delete_compression": (None, [_vp]),
+    "hmat_assemble_context_init": (None, [C.POINTER(hmat_assemble_context_t)]),
+    "hmat_factorization_context_init": (None, [C.POINTER(hmat_factorization_context_t)]),
+    "hmat_solve_context_init": (None, [C.POINTER(hmat_solve_context_t)]),
+    "hmat_get_version": (C.c_char_p, []),
+    "hmat_mi355x_kernel_compute": (None, [_vp, C.c_int, C.c_int, _vp]),
+    "hmx_last_error": (C.c_char_p, []),
+    "hmx_gemv_device": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_double, _vp, C.c_int]),
+    "hmx_solve_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int]),
+    "hmx_factorize_mixed": (C.c_int, [_vp, C.c_int]),
+    "hmx_solve_refined_device": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int,
+                                           C.POINTER(C.c_double)]),
+    "hmx_partition": (C.c_int, [_vp, _vp]),
+    "hmx_leaf_count": (C.c_int, [_vp]),
+    "hmx_leaf_info": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "hmx_leaf_data": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
+    "hmx_leaf_table": (C.c_int, [_vp, C.c_int, _vp]),
+    "hmx_stats": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "hmx_set_rank_capacity": (None, [C.c_int]),
+    "hmx_set_stream": (C.c_int, [_vp, _vp]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run __graft_entry__.build() first")
+        lib = C.CDLL(path)
+        for name, (res, args) in _EXPORTS.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise EngineError(f"{what} failed: {load().hmx_last_error().decode()}")
+
+
+def interface(value_type=HMAT_DOUBLE_PRECISION):
+    lib = load()
+    it = hmat_interface_t()
+    lib.hmat_init_mi355x_interface(C.byref(it), value_type)
+    return it
+
+
+class ClusterTree:
+    def __init__(self, points, max_leaf=128):
+        lib = load()
+        self.points = np.ascontiguousarray(points, dtype=np.float64)
+        self.n, self.dim = self.points.shape
+        algo = lib.hmat_create_clustering_median()
+        algo2 = lib.hmat_create_clustering_max_dof(algo, int(max_leaf))
+        self.handle = lib.hmat_create_cluster_tree(self.points.ctypes.data, self.dim, self.n, algo2)
+        lib.hmat_delete_clustering(algo2)
+        lib.hmat_delete_clustering(algo)
+        if not self.handle:
+            raise EngineError(lib.hmx_last_error().decode())
+
+    @property
+    def indices(self):
+        p = load().hmat_cluster_get_indices(self.handle)
+        return np.ctypeslib.as_array(p, shape=(self.n,)).copy()
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.hmat_delete_cluster_tree(self.handle)
+            self.handle = None
+
+
+class HMatrix:
+    """An H-matrix created and driven through hmat_interface_t."""
+
+    def __init__(self, tree: ClusterTree, eta=2.0, lower_symmetric=True,
+                 value_type=HMAT_DOUBLE_PRECISION, lowrank_eps=1e-4):
+        lib = load()
+        self.tree = tree
+        self.iface = interface(value_type)
+        self.value_type = value_type
+        adm = lib.hmat_create_admissibility_standard(float(eta))
+        self.handle = self.iface.create_empty_hmatrix_admissibility(
+            tree.handle, tree.handle, 1 if lower_symmetric else 0, adm)
+        lib.hmat_delete_admissibility(adm)
+        if not self.handle:
+            raise EngineError(lib.hmx_last_error().decode())
+        self.lower_symmetric = lower_symmetric
+        self.iface.set_low_rank_epsilon(self.handle, float(lowrank_eps))
+        self._kernel = None
+
+    @property
+    def n(self):
+        return self.tree.n
+
+    def assemble(self, kind=KERNEL_EXPONENTIAL, length=0.5, nugget=1e-2, eps=1e-4,
+                 method="aca_plus", factorization=FACT_NONE):
+        lib = load()
+        k = hmat_mi355x_kernel_t()
+        k.kind, k.dimension, k.n = kind, self.tree.dim, self.tree.n
+        k.points = self.tree.points.ctypes.data_as(C.POINTER(C.c_double))
+        k.length, k.nugget = float(length), float(nugget)
+        self._kernel = k
+        ctx = hmat_assemble_context_t()
+        lib.hmat_assemble_context_init(C.byref(ctx))
+        comp = {"aca_plus": lib.hmat_create_compression_aca_plus,
+                "aca_partial": lib.hmat_create_compression_aca_partial,
+                "svd": lib.hmat_create_compression_svd}[method](float(eps))
+        ctx.compression = comp
+        ctx.simple_compute = C.cast(lib.hmat_mi355x_kernel_compute, _vp).value
+        ctx.user_context = C.cast(C.pointer(k), _vp).value
+        ctx.lower_symmetric = 1 if self.lower_symmetric else 0
+        ctx.factorization = factorization
+        rc = self.iface.assemble_generic(self.handle, C.byref(ctx))
+        lib.hmat_delete_compression(comp)
+        _check(rc, "assemble_generic")
+
+    def factorize(self, algo=FACT_LLT, mixed=False):
+        if mixed:
+            _check(load().hmx_factorize_mixed(self.handle, algo), "hmx_factorize_mixed")
+            return
+        ctx = hmat_factorization_context_t()
+        load().hmat_factorization_context_init(C.byref(ctx))
+        ctx.factorization = algo
+        _check(self.iface.factorize_generic(self.handle, C.byref(ctx)), "factorize_generic")
+
+    def info(self):
+        inf = hmat_info_t()
+        _check(self.iface.get_info(self.handle, C.byref(inf)), "get_info")
+        return {k: getattr(inf, k) for k, _ in hmat_info_t._fields_}
+
+    def partition(self):
+        lib = load()
+        cnt = lib.hmx_partition(self.handle, None)
+        out = np.zeros(cnt * 5, dtype=np.int32)
+        lib.hmx_partition(self.handle, out.ctypes.data)
+        return out.reshape(cnt, 5)
+
+    def leaf_table(self, precision=0):
+        """(count, 6) int array: m, n, row offset, col offset, kind, rank."""
+        lib = load()
+        cnt = lib.hmx_leaf_count(self.handle)
+        out = np.zeros(cnt * 6, dtype=np.int32)
+        _check(lib.hmx_leaf_table(self.handle, precision, out.ctypes.data), "hmx_leaf_table")
+        return out.reshape(cnt, 6)
+
+    def leaf_shapes(self):
+        for m, n, r0, c0, kind, _ in self.leaf_table():
+            yield int(m), int(n), int(r0), int(c0), int(kind)
+
+    def leaves(self, precision=0):
+        """Yields (m, n, r0, c0, kind, rank, a, b) with host copies of the data."""
+        lib = load()
+        for leaf, row in enumerate(self.leaf_table(precision)):
+            m, n, r0, c0, kind, rank = (int(v) for v in row)
+            if kind == 0:
+                a = np.zeros(m * n)
+                _check(lib.hmx_leaf_data(self.handle, leaf, precision, a.ctypes.data, None), "leaf")
+                yield m, n, r0, c0, kind, rank, a.reshape(n, m).T, None
+            else:
+                k = max(rank, 0)
+                a = np.zeros(m * max(k, 1))
+                b = np.zeros(n * max(k, 1))
+                _check(lib.hmx_leaf_data(self.handle, leaf, precision, a.ctypes.data, b.ctypes.data), "leaf")
+                yield m, n, r0, c0, kind, rank, a[:m * k].reshape(k, m).T, b[:n * k].reshape(k, n).T
+
+    def to_dense(self, precision=0, mirror=True):
+        """Dense matrix (hmat numbering) from the stored leaves."""
+        d = np.zeros((self.n, self.n))
+        for m, n, r0, c0, kind, rank, a, b in self.leaves(precision):
+            blk = a if kind == 0 else (a @ b.T if rank > 0 else np.zeros((m, n)))
+            d[r0:r0 + m, c0:c0 + n] = blk
+            if mirror and self.lower_symmetric and (r0 != c0 or m != n):
+                d[c0:c0 + n, r0:r0 + m] = blk.T
+        return d
+
+    # host-pointer API entries (user numbering unless stated)
+    def gemv(self, x, alpha=1.0, beta=0.0, y=None, permute=True):
+        x = np.asfortranarray(x, dtype=np.float64 if self.value_type else np.float32)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        if y is None:
+            y = np.zeros_like(x)
+        y = np.asfortranarray(y)
+        T = C.c_double if self.value_type else C.c_float
+        a, b = T(alpha), T(beta)
+        fn = self.iface.gemv if permute else self.iface.gemm_scalar
+        _check(fn(b"N", C.cast(C.byref(a), _vp), self.handle, x.ctypes.data,
+                  C.cast(C.byref(b), _vp), y.ctypes.data, nrhs), "gemv")
+        return y
+
+    def solve(self, b, permute=True):
+        x = np.array(b, dtype=np.float64 if self.value_type else np.float32, order="F", copy=True)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        fn = self.iface.solve_systems if permute else self.iface.solve_dense
+        _check(fn(self.handle, x.ctypes.data, nrhs), "solve")
+        return x
+
+    def logdet(self):
+        r = C.c_double()
+        _check(self.iface.logdet(self.handle, C.byref(r)), "logdet")
+        return r.value
+
+    # device-pointer entries (hmat numbering), e.g. torch tensors' data_ptr()
+    def gemv_device(self, x_ptr, y_ptr, nrhs, alpha=1.0, beta=0.0, precision=0):
+        _check(load().hmx_gemv_device(self.handle, precision, alpha, x_ptr, beta, y_ptr, nrhs),
+               "hmx_gemv_device")
+
+    def solve_device(self, b_ptr, nrhs, precision=0):
+        _check(load().hmx_solve_device(self.handle, precision, b_ptr, nrhs), "hmx_solve_device")
+
+    def solve_refined_device(self, b_ptr, nrhs, tol=1e-10, max_iter=30):
+        r = C.c_double()
+        it = load().hmx_solve_refined_device(self.handle, b_ptr, nrhs, tol, max_iter, C.byref(r))
+        if it < 0:
+            raise EngineError(f"refined solve failed: {load().hmx_last_error().decode()}")
+        return it, r.value
+
+    def set_stream(self, stream_ptr):
+        """Launch on a caller's HIP stream (int handle, e.g.
+        torch.cuda.current_stream().cuda_stream); None restores the engine's."""
+        _check(load().hmx_set_stream(self.handle, stream_ptr), "hmx_set_stream")
+
+    def stats(self):
+        out = (C.c_double * 8)()
+        load().hmx_stats(self.handle, out)
+        keys = ["t_assemble", "t_factor", "t_solve", "tasks", "waves", "launches", "segments",
+                "temp_bytes"]
+        return dict(zip(keys, list(out)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.iface.destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

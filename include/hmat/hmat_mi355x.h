@@ -1,0 +1,105 @@
+/*
+ * MI355X engine selection and extensions.
+ *
+ * hmat_init_mi355x_interface() fills an hmat_interface_t exactly like
+ * hmat_init_default_interface() (c_default_interface.cpp:242-252 of hmat-oss)
+ * but with the GPU engine behind every slot it provides.
+ *
+ * Assembly on the GPU needs the matrix entries as a device-evaluable kernel.
+ * A caller keeps the ordinary hmat_assemble_context_t and sets
+ *     ctx.simple_compute = hmat_mi355x_kernel_compute;
+ *     ctx.user_context   = &kernel;      (an hmat_mi355x_kernel_t)
+ * hmat_mi355x_kernel_compute is a plain host implementation of the same
+ * interaction (so the context also works with any other engine); this engine
+ * recognises it and evaluates the kernel on the device instead.
+ *
+ * The hmx_* functions take and return plain pointers and sizes.  The *_device
+ * variants work on device pointers in hmat numbering on the engine's stream
+ * and synchronise before returning.
+ */
+#ifndef HMAT_MI355X_EXT_H
+#define HMAT_MI355X_EXT_H
+
+#include "hmat/hmat.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces hmat_init_default_interface (c_default_interface.cpp:242). */
+HMAT_API void hmat_init_mi355x_interface(hmat_interface_t *i, hmat_value_t type);
+
+typedef enum {
+  HMAT_MI355X_KERNEL_EXPONENTIAL = 0, /* exp(-r / length) + nugget * delta_ij */
+  HMAT_MI355X_KERNEL_GAUSSIAN = 1     /* exp(-r^2 / length^2) + nugget * delta_ij */
+} hmat_mi355x_kernel_kind_t;
+
+typedef struct {
+  int kind;              /* hmat_mi355x_kernel_kind_t */
+  int dimension;         /* spatial dimension of points (1..3) */
+  int n;                 /* number of points (== DoFs) */
+  const double *points;  /* n * dimension, user numbering */
+  double length;         /* correlation length */
+  double nugget;         /* added on the diagonal (user indices equal) */
+} hmat_mi355x_kernel_t;
+
+/* hmat_interaction_func_t for an hmat_mi355x_kernel_t user context (the
+ * simple_compute callback of assembly.cpp:75-91); writes a double. */
+HMAT_API void hmat_mi355x_kernel_compute(void *user_context, int row, int col, void *result);
+
+/* ---- flat helpers (no counterpart in hmat-oss; used by the test-suite, the
+ *      benchmark and the Python binding) ---------------------------------- */
+
+/* Last error message of a failing call on this thread ("" if none). */
+HMAT_API const char *hmx_last_error(void);
+
+/* y = alpha A x + beta y on device pointers (hmat numbering, ld = N).
+ * 'precision' 0 = the FP64 matrix, 1 = the FP32 copy. */
+HMAT_API int hmx_gemv_device(hmat_matrix_t *m, int precision, double alpha, const void *x,
+                             double beta, void *y, int nrhs);
+
+/* Solves with the factor in place on device data (hmat numbering). */
+HMAT_API int hmx_solve_device(hmat_matrix_t *m, int precision, void *b, int nrhs);
+
+/* Mixed precision: factorise an FP32 copy of the FP64 matrix (LLt), keeping
+ * the FP64 matrix for residuals. */
+HMAT_API int hmx_factorize_mixed(hmat_matrix_t *m, int algo);
+
+/* Iterative refinement on device FP64 data: x <- A^{-1} b with the FP32
+ * factor as the inner solver and the FP64 H-matrix for the residual, until
+ * max_col ||b - A x|| / ||b|| <= tol or max_iter sweeps.  Returns the number
+ * of sweeps (< 0 on error); *resid receives the final relative residual. */
+HMAT_API int hmx_solve_refined_device(hmat_matrix_t *m, double *b, int nrhs, double tol,
+                                      int max_iter, double *resid);
+
+/* Partition in preorder: 5 ints per block node (row_offset, row_size,
+ * col_offset, col_size, kind 0 full / 1 rk / 2 inner).  Returns the node
+ * count; writes only when out != NULL. */
+HMAT_API int hmx_partition(hmat_matrix_t *m, int *out);
+
+HMAT_API int hmx_leaf_count(hmat_matrix_t *m);
+/* info[6] = m, n, row_offset, col_offset, kind, rank (-1 for full) */
+HMAT_API int hmx_leaf_info(hmat_matrix_t *m, int leaf, int precision, int *info);
+/* All leaves at once: 6 ints per leaf as in hmx_leaf_info. */
+HMAT_API int hmx_leaf_table(hmat_matrix_t *m, int precision, int *out);
+/* Copies a leaf to the host as doubles: full -> a (m x n); rk -> a (m x k), b (n x k). */
+HMAT_API int hmx_leaf_data(hmat_matrix_t *m, int leaf, int precision, double *a, double *b);
+
+/* Timing / scheduling counters of the last assembly / factorisation / solve:
+ * out[0..7] = t_assemble, t_factor, t_solve, tasks, waves, launches,
+ *             segments, temp_bytes. */
+HMAT_API int hmx_stats(hmat_matrix_t *m, double *out);
+
+/* Makes the engine launch on the caller's HIP stream (hipStream_t passed as
+ * void*; NULL restores the engine's own stream).  While a caller stream is
+ * set, the *_device entry points are stream-ordered and do not synchronise. */
+HMAT_API int hmx_set_stream(hmat_matrix_t *m, void *stream);
+
+/* Upper bound on Rk panel capacity used for new matrices (default 160). */
+HMAT_API void hmx_set_rank_capacity(int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

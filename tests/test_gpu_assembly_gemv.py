@@ -1,0 +1,153 @@
+"""GPU parity: assembly (full-leaf evaluation, SVD compression, ACA+ and Rk
+truncation) and the leaf-level H-gemv against the oracle on the same seeded
+inputs."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import hmat_oracle as ho
+
+hmat_amd = pytest.importorskip("hmat_amd")
+
+EPS = 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def build(n, height, seed, leaf=128, kind=hmat_amd.KERNEL_EXPONENTIAL, length=0.5, nugget=1e-2,
+          method="aca_plus", vt=hmat_amd.HMAT_DOUBLE_PRECISION):
+    pts = ho.cylinder_points(n, 1.0, height, seed)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    h = hmat_amd.HMatrix(t, value_type=vt)
+    h.assemble(kind=kind, length=length, nugget=nugget, eps=EPS, method=method)
+    return pts, t, h
+
+
+def oracle_build(pts, leaf=128, length=0.5, nugget=1e-2, kernel=ho.ExpKernel):
+    root, idx = ho.build_cluster_tree(pts, leaf)
+    bt = ho.build_block_tree(root, root, 2.0, True)
+    k = kernel(pts, length, nugget)
+    ho.assemble(bt, idx, k)
+    return bt, idx, k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,height,seed,leaf", [(3000, 1.2, 3, 128), (6000, 2.4, 11, 64)])
+def test_assembly_leaves_match_oracle(n, height, seed, leaf):
+    say(f"assembly parity n={n} leaf={leaf} seed={seed}")
+    pts, t, h = build(n, height, seed, leaf)
+    bt, idx, k = oracle_build(pts, leaf)
+    assert np.array_equal(t.indices, idx)
+    oleaves = ho.leaves(bt)
+    gleaves = list(h.leaves())
+    assert len(oleaves) == len(gleaves)
+    worst_full = 0.0
+    worst_rk = 0.0
+    rank_diff = []
+    for ob, (m, nn, r0, c0, kind, rank, a, b) in zip(oleaves, gleaves):
+        assert (ob.rows.offset, ob.m, ob.cols.offset, ob.n) == (r0, m, c0, nn)
+        exact = k.block(idx[r0:r0 + m], idx[c0:c0 + nn])
+        if kind == 0:
+            assert ob.kind == ho.FULL
+            worst_full = max(worst_full, np.abs(a - exact).max() / np.abs(exact).max())
+        else:
+            assert ob.kind == ho.RK
+            g = a @ b.T if rank > 0 else np.zeros((m, nn))
+            o = ob.rk.eval(m, nn, np.float64)
+            nrm = np.linalg.norm(exact)
+            # both are eps-approximations of the same block
+            worst_rk = max(worst_rk, np.linalg.norm(g - exact) / nrm)
+            assert np.linalg.norm(g - o) <= 4 * EPS * nrm
+            rank_diff.append(abs(rank - ob.rk.rank))
+    say("worst full", worst_full, "worst rk", worst_rk, "max rank diff", max(rank_diff))
+    assert worst_full < 1e-14
+    assert worst_rk < 4 * EPS
+    assert max(rank_diff) <= 2
+    inf = h.info()
+    oinf = ho.info(bt)
+    assert inf["full_count"] == oinf["full_count"] and inf["rk_count"] == oinf["rk_count"]
+    assert inf["nr_block_clusters"] == oinf["nr_block_clusters"]
+    assert abs(inf["compressed_size"] - oinf["compressed_size"]) <= 0.02 * oinf["compressed_size"]
+    h.close()
+
+
+@pytest.mark.gpu
+def test_aca_partial_and_svd_methods():
+    say("aca partial / svd methods n=2500")
+    pts = ho.cylinder_points(2500, 1.0, 1.0, 21)
+    for method in ("aca_partial",):
+        t = hmat_amd.ClusterTree(pts, 128)
+        h = hmat_amd.HMatrix(t)
+        h.assemble(eps=EPS, method=method)
+        k = ho.ExpKernel(pts, 0.5, 1e-2)
+        idx = t.indices
+        A = k.block(idx, idx)
+        D = h.to_dense()
+        assert np.linalg.norm(D - A) / np.linalg.norm(A) < 4 * EPS
+        h.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nrhs", [1, 3, 9])
+def test_gemv_device_matches_dense(nrhs):
+    import torch
+    say(f"gemv device nrhs={nrhs}")
+    pts, t, h = build(4000, 1.6, 5)
+    D = h.to_dense()
+    rng = np.random.default_rng(nrhs)
+    x = rng.standard_normal((h.n, nrhs))
+    y0 = rng.standard_normal((h.n, nrhs))
+    xd = torch.tensor(x.T.copy(), dtype=torch.float64, device="cuda").contiguous()  # column-major
+    yd = torch.tensor(y0.T.copy(), dtype=torch.float64, device="cuda").contiguous()
+    h.gemv_device(xd.data_ptr(), yd.data_ptr(), nrhs, alpha=0.7, beta=-0.3)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy().T
+    ref = 0.7 * (D @ x) - 0.3 * y0
+    err = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+    say("gemv rel err", err)
+    assert err < 1e-13
+    h.close()
+
+
+@pytest.mark.gpu
+def test_gemv_host_api_user_numbering():
+    say("gemv through hmat_interface_t.gemv (user numbering)")
+    pts, t, h = build(2500, 1.0, 8)
+    k = ho.ExpKernel(pts, 0.5, 1e-2)
+    A = k.block(np.arange(h.n), np.arange(h.n))
+    x = np.random.default_rng(2).standard_normal((h.n, 2))
+    y = h.gemv(x)
+    err = np.linalg.norm(y - A @ x) / np.linalg.norm(A @ x)
+    say("gemv vs dense kernel", err)
+    assert err < 4 * EPS
+    # oracle H-gemv on the oracle's own assembly agrees to the same tolerance
+    bt, idx, _ = oracle_build(pts)
+    xo = x[idx]
+    yo = np.zeros_like(xo)
+    ho.gemv(bt, "N", 1.0, xo, 0.0, yo)
+    yo_user = np.empty_like(yo)
+    yo_user[idx] = yo
+    assert np.linalg.norm(y - yo_user) / np.linalg.norm(yo_user) < 4 * EPS
+    h.close()
+
+
+@pytest.mark.gpu
+def test_gemv_fp32_storage():
+    import torch
+    say("gemv fp32 matrix")
+    pts, t, h = build(3000, 1.2, 4, vt=hmat_amd.HMAT_SIMPLE_PRECISION)
+    D = h.to_dense(precision=1)
+    x = np.random.default_rng(3).standard_normal((h.n, 1)).astype(np.float32)
+    xd = torch.tensor(x[:, 0], device="cuda")
+    yd = torch.zeros_like(xd)
+    h.gemv_device(xd.data_ptr(), yd.data_ptr(), 1, precision=1)
+    torch.cuda.synchronize()
+    ref = D @ x.astype(np.float64)
+    err = np.linalg.norm(yd.cpu().numpy()[:, None] - ref) / np.linalg.norm(ref)
+    say("fp32 gemv rel err", err)
+    assert err < 1e-6
+    h.close()
