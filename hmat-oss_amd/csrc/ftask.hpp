@@ -1,0 +1,70 @@
+// Task records of the factorisation / solve executor.  The host recursion
+// (factor.cpp) turns the H-LLt of hmat-oss (recursion.cpp:349-390 and the
+// H-gemm / H-trsm it calls) into a list of leaf-level tasks; tasks of one
+// dependency wave and one type run as one batched kernel launch.
+//
+// All operands are device pointers.  Sizes are static capacities; a dynamic
+// size (the rank of an Rk panel, known only on the device) is given by a
+// pointer to its rank slot and the effective size is min(static, *slot).
+#pragma once
+#include <cstdint>
+
+namespace hmx {
+
+enum TaskType : int {
+  T_GEMM = 0,     // C = beta C + alpha op(A) op(B)
+  T_POTRF,        // A <- chol(A) lower, upper triangle zeroed
+  T_TRSM,         // mode: 0 X <- X L^{-T}, 1 X <- L^{-1} X, 2 X <- L^{-T} X
+  T_FADD,         // Rk target (+)= sum_i alpha_i part_i, then truncate
+  T_FADD_DENSE,   // Rk target += alpha * dense part, eval + SVD
+  T_RKRK,         // Rk product core: (A1 B1^T)(A2 B2^T)
+  T_ACAFULL,      // dense -> Rk by full-pivot ACA
+  T_ZERO,         // C = 0
+  T_AXPY,         // C += alpha op(A)
+  T_NTYPES
+};
+
+struct Dim {
+  int s;             // static size / capacity
+  const int* d;      // dynamic size slot or nullptr
+};
+
+struct RkRef {
+  void* a;           // m x cap, ld la
+  void* b;           // n x cap, ld lb
+  int la, lb;
+  int m, n;
+  int cap;
+  int* rank;         // rank slot
+};
+
+struct Part {
+  RkRef rk;          // rows/cols are a subset of the target at (roff, coff)
+  int roff, coff;
+  double alpha;
+};
+
+struct FTask {
+  int type;
+  int mode;          // trsm mode, gemm beta (0/1)
+  int ta, tb;        // gemm transposes
+  Dim m, n, p;
+  double alpha;
+  double eps;
+  void* A;
+  void* B;
+  void* C;
+  int lda, ldb, ldc;
+  int nparts;
+  int pad;
+  RkRef tgt;         // FADD / FADD_DENSE / RKRK / ACAFULL output
+  Part parts[4];     // FADD inputs; RKRK uses parts[0] (A1 B1^T) and parts[1] (A2 B2^T)
+  int64_t ws;        // workspace offset (doubles)
+};
+
+struct GemmTile {
+  int task;
+  int tm, tn;
+};
+
+}  // namespace hmx

@@ -8,6 +8,8 @@
 //   yfinal  : y = beta y + alpha (acc + mirrored full slots), fixed order
 // Every stored leaf is streamed once per pass that needs it; accumulation is
 // in FP64 whatever the storage type.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace hmx {
@@ -189,6 +191,31 @@ void launch_convert(const S* src, D* dst, size_t n, hipStream_t s) {
   HMX_CHECK(hipGetLastError());
 }
 template void launch_convert<double, float>(const double*, float*, size_t, hipStream_t);
+
+namespace {
+__global__ void k_add_f2d(double* x, const float* d, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] += (double)d[i];
+}
+__global__ void k_col_sumsq(const double* v, int n, double* out) {
+  __shared__ double red[kWaves];
+  const double* c = v + (size_t)blockIdx.x * n;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) s += c[i] * c[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+}  // namespace
+
+void launch_vec_add_f2d(double* x, const float* d, size_t n, hipStream_t s) {
+  size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_add_f2d, dim3((unsigned)blocks), dim3(256), 0, s, x, d, n);
+  HMX_CHECK(hipGetLastError());
+}
+void launch_col_sumsq(const double* v, int n, int nrhs, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_col_sumsq, dim3(nrhs), dim3(kBlock), 0, s, v, n, out);
+  HMX_CHECK(hipGetLastError());
+}
 template void launch_convert<float, double>(const float*, double*, size_t, hipStream_t);
 
 template <typename T>

@@ -107,6 +107,10 @@ void launch_gemv_yfinal(const YItem* items, int count, const int64_t* slot_list,
                         const double* slots, const double* acc, int ldacc, T* y, int ldy,
                         int nrhs, double alpha, double beta, hipStream_t s);
 
+// Refinement helpers on column-major N x nrhs vectors (ld = n).
+void launch_vec_add_f2d(double* x, const float* d, size_t n, hipStream_t s);   // x += d
+void launch_col_sumsq(const double* v, int n, int nrhs, double* out, hipStream_t s);  // out[r] = sum v^2
+
 // Elementwise conversion of a whole arena (same layout in both precisions).
 template <typename S, typename D>
 void launch_convert(const S* src, D* dst, size_t n, hipStream_t s);

@@ -370,10 +370,10 @@ def aca_full(m, eps):
         est += 2.0 * new
         ab = float(np.dot(va, va) * np.dot(vb, vb))
         est += ab
+        if ab < eps * eps * est:
+            break                 # the pair that meets the criterion is dropped (resize(nu))
         acols.append(va)
         bcols.append(vb)
-        if ab < eps * eps * est:
-            break
     if not acols:
         return Rk(None, None)
     return Rk(np.stack(acols, axis=1), np.stack(bcols, axis=1))

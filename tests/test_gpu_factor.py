@@ -1,0 +1,143 @@
+"""GPU parity of the H-LLt factorisation, the H-triangular solves and logdet
+against the oracle (same seeded inputs, same partition)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import hmat_oracle as ho
+
+hmat_amd = pytest.importorskip("hmat_amd")
+
+EPS = 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dense_lower_factor(h, precision=0):
+    return np.tril(h.to_dense(precision=precision, mirror=False))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,height,seed,leaf", [(1500, 0.6, 3, 64), (4000, 1.6, 9, 128)])
+def test_llt_fp64_matches_oracle(n, height, seed, leaf):
+    say(f"llt fp64 n={n} leaf={leaf} seed={seed}")
+    pts = ho.cylinder_points(n, 1.0, height, seed)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    h = hmat_amd.HMatrix(t)
+    h.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    H = h.to_dense()
+    h.factorize(hmat_amd.FACT_LLT)
+    st = h.stats()
+    say("stats", st)
+    L = dense_lower_factor(h)
+    rec = np.linalg.norm(L @ L.T - H) / np.linalg.norm(H)
+    say("||LL^T - H|| / ||H||", rec)
+    assert rec < 20 * EPS
+    # oracle on the same inputs
+    root, idx = ho.build_cluster_tree(pts, leaf)
+    bt = ho.build_block_tree(root, root, 2.0, True)
+    k = ho.ExpKernel(pts, 0.5, 1e-2)
+    ho.assemble(bt, idx, k)
+    ho.llt(bt, EPS)
+    ld_o = ho.logdet(bt)
+    ld_g = h.logdet()
+    say("logdet gpu", ld_g, "oracle", ld_o)
+    assert abs(ld_g - ld_o) <= 1e-5 * abs(ld_o)
+    sign, ld_dense = np.linalg.slogdet(H)
+    assert abs(ld_g - 0.5 * ld_dense) <= 1e-4 * abs(ld_dense)
+    # solve: user numbering through solve_systems
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal((n, 2))
+    x = h.solve(b)
+    A = k.block(np.arange(n), np.arange(n))
+    res = np.linalg.norm(A @ x - b) / np.linalg.norm(b)
+    xo = b[idx].copy()
+    ho.solve_llt(bt, xo)
+    xou = np.empty_like(xo)
+    xou[idx] = xo
+    res_o = np.linalg.norm(A @ xou - b) / np.linalg.norm(b)
+    rel = np.linalg.norm(x - xou) / np.linalg.norm(xou)
+    say("residual gpu", res, "oracle", res_o, "solution rel diff", rel)
+    assert res < 5 * max(res_o, 1e-6)
+    assert rel < 50 * EPS * max(1.0, np.linalg.norm(xou) / np.linalg.norm(b))
+    # solve against the H-matrix itself: H x = b holds to the factor accuracy
+    xh = h.solve(b[idx], permute=False)
+    res_h = np.linalg.norm(H @ xh - b[idx]) / np.linalg.norm(b)
+    say("residual vs H", res_h)
+    h.close()
+
+
+@pytest.mark.gpu
+def test_llt_fp32_type():
+    say("llt fp32 n=2500")
+    n = 2500
+    pts = ho.cylinder_points(n, 1.0, 1.0, 17)
+    t = hmat_amd.ClusterTree(pts, 128)
+    h = hmat_amd.HMatrix(t, value_type=hmat_amd.HMAT_SIMPLE_PRECISION)
+    h.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    H = h.to_dense(precision=1)
+    h.factorize(hmat_amd.FACT_LLT)
+    L = dense_lower_factor(h, precision=1)
+    rec = np.linalg.norm(L @ L.T - H) / np.linalg.norm(H)
+    say("fp32 ||LL^T - H|| / ||H||", rec)
+    assert rec < 50 * EPS
+    b = np.random.default_rng(0).standard_normal((n, 1)).astype(np.float32)
+    x = h.solve(b)
+    k = ho.ExpKernel(pts, 0.5, 1e-2)
+    A = k.block(np.arange(n), np.arange(n))
+    res = np.linalg.norm(A @ x.astype(np.float64) - b) / np.linalg.norm(b)
+    say("fp32 residual", res)
+    assert res < 1e-2
+    h.close()
+
+
+@pytest.mark.gpu
+def test_not_spd_reports_error():
+    """A negative nugget large enough to break positive definiteness makes the
+    factorisation fail with an error (as potrf's info > 0 in the reference)."""
+    say("non-SPD input")
+    pts = ho.cylinder_points(800, 1.0, 0.4, 1)
+    t = hmat_amd.ClusterTree(pts, 64)
+    h = hmat_amd.HMatrix(t)
+    h.assemble(length=0.5, nugget=-5.0, eps=EPS)
+    with pytest.raises(hmat_amd.EngineError, match="positive definite"):
+        h.factorize(hmat_amd.FACT_LLT)
+    h.close()
+
+
+@pytest.mark.gpu
+def test_mixed_precision_refinement_reaches_1e10():
+    """FP32 H-LLt + FP64 iterative refinement: the residual against the FP64
+    H-matrix reaches 1e-10 and the solution agrees with the plain FP64 path."""
+    import torch
+    say("mixed precision refinement n=3000")
+    n = 3000
+    pts = ho.cylinder_points(n, 1.0, 1.2, 23)
+    t = hmat_amd.ClusterTree(pts, 128)
+    h = hmat_amd.HMatrix(t)
+    h.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    H = h.to_dense()
+    h.factorize(hmat_amd.FACT_LLT, mixed=True)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal((n, 3))
+    bd = torch.tensor(b.T.copy(), dtype=torch.float64, device="cuda").contiguous()
+    it, rel = h.solve_refined_device(bd.data_ptr(), 3, tol=1e-10, max_iter=40)
+    torch.cuda.synchronize()
+    x = bd.cpu().numpy().T
+    res = np.linalg.norm(H @ x - b, axis=0) / np.linalg.norm(b, axis=0)
+    say("IR sweeps", it, "reported", rel, "checked", res.max())
+    assert res.max() <= 1e-10
+    xe = np.linalg.solve(H, b)
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-8
+    # plain FP64 factor on a second matrix: both solutions approximate H^{-1} b
+    h2 = hmat_amd.HMatrix(t)
+    h2.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    h2.factorize(hmat_amd.FACT_LLT)
+    x64 = h2.solve(b, permute=False)
+    say("fp64 direct vs refined", np.linalg.norm(x64 - x) / np.linalg.norm(x))
+    h.close()
+    h2.close()
