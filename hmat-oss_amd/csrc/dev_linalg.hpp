@@ -159,7 +159,9 @@ __device__ inline void sorted_norms(const double* G, int ldg, int rows, int cols
     double s = 0.0;
     for (int i = lane; i < rows; i += kWave) s += g[i] * g[i];
     s = wave_sum(s);
-    if (lane == 0) sig[j] = sqrt(s);
+    // a non-finite norm (NaN input) is ordered as zero so that 'order' is
+    // always a permutation of [0, cols)
+    if (lane == 0) sig[j] = (s >= 0.0 && s <= 1.79e308) ? sqrt(s) : 0.0;
   }
   __syncthreads();
   for (int j = threadIdx.x; j < cols; j += kBlock) {

@@ -944,6 +944,7 @@ class Exec {
       throw std::runtime_error("task segment limit reached while temporaries are live");
     if (tasks_.empty()) return;
     run_segment();
+    check_flags();  // stop at the first failing pivot, as potrf's info > 0 does
     tasks_.clear();
     waves_.clear();
     units_.clear();

@@ -151,7 +151,8 @@ __device__ inline bool any_nonzero(const double* v, int len, double* red) {
 __device__ inline int amax(const double* v, int len, double* rv, int* ri) {
   ArgMax a{-1.0, -1};
   for (int t = threadIdx.x; t < len; t += kBlock) a = better_max(a, ArgMax{fabs(v[t]), t});
-  return block_argmax(a, rv, ri).i;
+  const int i = block_argmax(a, rv, ri).i;
+  return i < 0 ? 0 : i;  // all-NaN vector: any in-range index
 }
 __device__ inline double dot(const double* a, const double* b, int len, double* red) {
   double s = 0.0;

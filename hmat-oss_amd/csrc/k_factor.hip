@@ -25,9 +25,11 @@ __device__ inline int rank_of(const RkRef& r) {
 constexpr int TM = 64, TN = 64, TK = 16;
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_gemm(const FTask* tasks, const GemmTile* tiles) {
+__global__ void __launch_bounds__(256) k_gemm(const FTask* tasks, const GemmTile* tiles,
+                                              const int* fail) {
   __shared__ T As[TK][TM + 1];
   __shared__ T Bs[TK][TN + 1];
+  if (*fail) return;
   const GemmTile tl = tiles[blockIdx.x];
   const FTask& t = tasks[tl.task];
   const int M = eff(t.m), N = eff(t.n), P = eff(t.p);
@@ -147,8 +149,9 @@ __global__ void __launch_bounds__(kBlock) k_potrf(const FTask* tasks, int* info)
 //   mode 2: X (n x k) <- L^{-T} X     (solveUpperTriangularLeft, Uplo LOWER)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks) {
+__global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* fail) {
   __shared__ T Ls[kLdsDense * kLdsDense];
+  if (*fail) return;
   const FTask& t = tasks[blockIdx.x];
   const int n = t.n.s;  // order of L
   const T* Lg = (const T*)t.A;
@@ -209,7 +212,8 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks) {
 // zero fill and axpy
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_zero(const FTask* tasks) {
+__global__ void k_zero(const FTask* tasks, const int* fail) {
+  if (*fail) return;
   const FTask& t = tasks[blockIdx.x];
   T* C = (T*)t.C;
   const int m = t.m.s, n = t.n.s;
@@ -217,7 +221,8 @@ __global__ void k_zero(const FTask* tasks) {
 }
 
 template <typename T>
-__global__ void k_axpy(const FTask* tasks) {
+__global__ void k_axpy(const FTask* tasks, const int* fail) {
+  if (*fail) return;
   const FTask& t = tasks[blockIdx.x];
   T* C = (T*)t.C;
   const T* A = (const T*)t.A;
@@ -342,6 +347,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsbase, int* overflow) {
   __shared__ double red[kWaves];
   __shared__ int flag;
+  if (overflow[1]) return;
   const FTask& t = tasks[blockIdx.x];
   const RkRef tg = t.tgt;
   const int m = tg.m, n = tg.n;
@@ -445,6 +451,7 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fadd_dense(const FTask* tasks, double* wsbase, int* overflow) {
   __shared__ int flag;
+  if (overflow[1]) return;
   const FTask& t = tasks[blockIdx.x];
   const RkRef tg = t.tgt;
   const int m = tg.m, n = tg.n, kt = rank_of(tg);
@@ -474,6 +481,7 @@ __global__ void __launch_bounds__(kBlock) k_fadd_dense(const FTask* tasks, doubl
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_rkrk(const FTask* tasks, double* wsbase, int* overflow) {
   __shared__ int flag;
+  if (overflow[1]) return;
   const FTask& t = tasks[blockIdx.x];
   const RkRef r1 = t.parts[0].rk, r2 = t.parts[1].rk;
   RkRef out = t.tgt;
@@ -547,10 +555,11 @@ __global__ void __launch_bounds__(kBlock) k_rkrk(const FTask* tasks, double* wsb
 // ACAFULL: acaFull (compression.cpp:270-355) on a dense temp (modified)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_acafull(const FTask* tasks) {
+__global__ void __launch_bounds__(kBlock) k_acafull(const FTask* tasks, const int* fail) {
   __shared__ double rv[kWaves];
   __shared__ int ri[kWaves];
   __shared__ double red[kWaves];
+  if (*fail) return;
   const FTask& t = tasks[blockIdx.x];
   RkRef out = t.tgt;
   T* D = (T*)t.A;
@@ -566,6 +575,7 @@ __global__ void __launch_bounds__(kBlock) k_acafull(const FTask* tasks) {
     for (int e = threadIdx.x; e < m * n; e += kBlock)
       a = better_max(a, ArgMax{fabs((double)D[e % m + (size_t)(e / m) * ld]), e});
     a = block_argmax(a, rv, ri);
+    if (a.i < 0) break;  // all-NaN block
     const int i = a.i % m, j = a.i / m;
     const double delta = (double)D[i + (size_t)j * ld];
     if (delta == 0.0) break;
@@ -623,13 +633,13 @@ void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile*
   if (count == 0) return;
   switch (type) {
     case T_GEMM:
-      hipLaunchKernelGGL(k_gemm<T>, dim3(ntiles), dim3(256), 0, s, tasks, tiles);
+      hipLaunchKernelGGL(k_gemm<T>, dim3(ntiles), dim3(256), 0, s, tasks, tiles, flags + 1);
       break;
     case T_POTRF:
       hipLaunchKernelGGL(k_potrf<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
       break;
     case T_TRSM:
-      hipLaunchKernelGGL(k_trsm<T>, dim3(count), dim3(kBlock), 0, s, tasks);
+      hipLaunchKernelGGL(k_trsm<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
       break;
     case T_FADD:
       hipLaunchKernelGGL(k_fadd<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags);
@@ -641,13 +651,13 @@ void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile*
       hipLaunchKernelGGL(k_rkrk<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags);
       break;
     case T_ACAFULL:
-      hipLaunchKernelGGL(k_acafull<T>, dim3(count), dim3(kBlock), 0, s, tasks);
+      hipLaunchKernelGGL(k_acafull<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
       break;
     case T_ZERO:
-      hipLaunchKernelGGL(k_zero<T>, dim3(count), dim3(256), 0, s, tasks);
+      hipLaunchKernelGGL(k_zero<T>, dim3(count), dim3(256), 0, s, tasks, flags + 1);
       break;
     case T_AXPY:
-      hipLaunchKernelGGL(k_axpy<T>, dim3(count), dim3(256), 0, s, tasks);
+      hipLaunchKernelGGL(k_axpy<T>, dim3(count), dim3(256), 0, s, tasks, flags + 1);
       break;
     default:
       throw std::runtime_error("unknown task type");
