@@ -89,9 +89,11 @@ struct SegmentPlan {
 using PlanList = std::vector<std::shared_ptr<SegmentPlan>>;
 
 struct FactorState {
-  // temp arena and temp rank slots for the current segment
-  DevBuf temp, ranks, ws, flags;
-  size_t temp_cap = 0, temp_used = 0;
+  // temporaries of the current segment: chunks that grow on demand up to
+  // 'temp_budget' bytes and are reused by the next segment
+  std::vector<std::unique_ptr<DevBuf>> chunks;
+  size_t chunk = 0, chunk_used = 0, temp_total = 0, temp_budget = 0, temp_used = 0;
+  DevBuf ranks, ws, flags;
   int rank_cap = 0, rank_used = 0;
   int64_t next_temp = 0;
   // cached solve plans, keyed by (precision, nrhs), and their right-hand side
@@ -122,13 +124,10 @@ class Exec {
       for (int i = RL[q]->offset; i < RL[q]->offset + RL[q]->size; ++i) row2leaf_[i] = (int)q;
     eps_ = M.lowrank_eps;
     cap_max_ = M.rk_cap_max;
-    if (fs_.temp_cap == 0) {
+    if (fs_.temp_budget == 0) {
       size_t freeb = 0, total = 0;
       HMX_CHECK(hipMemGetInfo(&freeb, &total));
-      size_t budget = (size_t)(0.4 * (double)freeb);
-      budget = std::min(budget, (size_t)64 << 30);
-      fs_.temp.alloc(budget);
-      fs_.temp_cap = budget;
+      fs_.temp_budget = std::min((size_t)(0.5 * (double)freeb), (size_t)96 << 30);
     }
     if (fs_.rank_cap < (1 << 22)) {
       fs_.ranks.alloc(sizeof(int) << 22);
@@ -136,8 +135,14 @@ class Exec {
     }
     fs_.flags.alloc(2 * sizeof(int));
     HMX_CHECK(hipMemsetAsync(fs_.flags.p, 0, 2 * sizeof(int), M.stream));
-    fs_.temp_used = 0;
+    reset_temps();
     fs_.rank_used = 0;
+  }
+
+  void reset_temps() {
+    fs_.chunk = 0;
+    fs_.chunk_used = 0;
+    fs_.temp_used = 0;
   }
 
   // ---- operands from the tree ---------------------------------------------
@@ -170,12 +175,38 @@ class Exec {
 
   // ---- temporaries ------------------------------------------------------------
   void* talloc(size_t elems) {
-    size_t bytes = (elems * sizeof(T) + 255) / 256 * 256;
-    if (fs_.temp_used + bytes > fs_.temp_cap) flush(true);
-    if (fs_.temp_used + bytes > fs_.temp_cap) throw std::runtime_error("temporary arena too small");
-    void* p = (char*)fs_.temp.p + fs_.temp_used;
-    fs_.temp_used += bytes;
-    return p;
+    const size_t bytes = (elems * sizeof(T) + 255) / 256 * 256;
+    for (;;) {
+      if (fs_.chunk < fs_.chunks.size()) {
+        DevBuf& c = *fs_.chunks[fs_.chunk];
+        if (fs_.chunk_used + bytes <= c.bytes) {
+          void* p = (char*)c.p + fs_.chunk_used;
+          fs_.chunk_used += bytes;
+          fs_.temp_used += bytes;
+          return p;
+        }
+        ++fs_.chunk;
+        fs_.chunk_used = 0;
+        continue;
+      }
+      // no chunk left: grow, or run what is queued and start over
+      const size_t want = std::max(bytes, fs_.chunks.empty() ? ((size_t)256 << 20)
+                                                             : fs_.chunks.back()->bytes * 2);
+      if (fs_.temp_total + want <= fs_.temp_budget || fs_.chunks.empty()) {
+        auto b = std::make_unique<DevBuf>();
+        b->alloc(want);
+        fs_.temp_total += want;
+        fs_.chunks.push_back(std::move(b));
+        continue;
+      }
+      flush(true);  // throws if temporaries are live
+      if (bytes > fs_.chunks.back()->bytes) {
+        auto b = std::make_unique<DevBuf>();
+        b->alloc(bytes);
+        fs_.temp_total += bytes;
+        fs_.chunks.push_back(std::move(b));
+      }
+    }
   }
   int* ralloc() {
     if (fs_.rank_used >= fs_.rank_cap) flush(true);
@@ -936,7 +967,7 @@ class Exec {
   // ---- execution -----------------------------------------------------------------------
   void maybe_flush() {
     if (live_ != 0) return;
-    if (tasks_.size() >= kMaxTasks || fs_.temp_used > fs_.temp_cap / 2) flush(false);
+    if (tasks_.size() >= kMaxTasks || fs_.temp_used > fs_.temp_budget / 2) flush(false);
   }
 
   void flush(bool forced) {
@@ -949,7 +980,7 @@ class Exec {
     waves_.clear();
     units_.clear();
     max_wave_ = 0;
-    fs_.temp_used = 0;
+    reset_temps();
     fs_.rank_used = 0;
   }
 
