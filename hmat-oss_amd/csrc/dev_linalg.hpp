@@ -77,18 +77,31 @@ __device__ inline void apply_q(const double* V, int ldv, int m, int nref, const 
   }
 }
 
+// Sum over an aligned group of g lanes (g a power of two <= 64).
+template <typename T>
+__device__ inline T group_sum(T v, int g) {
+  for (int o = g >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
 // One-sided Jacobi SVD of G (rows x cols, ldg): on exit G = U * Sigma
 // (columns mutually orthogonal, unsorted) and V (cols x cols, ldv) holds the
 // accumulated rotations so that G_in = (U Sigma) V^T.  'flag' is a shared int.
+// Column pairs of one round (round-robin ordering) are independent; each is
+// handled by a group of g lanes sized to the column length, so that a
+// wavefront works on 64 / g pairs at once.
 __device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
                                   int* flag, int max_sweeps = 40) {
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   for (int e = threadIdx.x; e < cols * cols; e += kBlock) {
     const int i = e % cols, j = e / cols;
     V[i + (size_t)j * ldv] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
   if (cols < 2) return;
+  const int longest = rows > cols ? rows : cols;
+  int g = 8;
+  while (g < 64 && g * 4 < longest) g <<= 1;
+  const int gl = threadIdx.x % g, grp = threadIdx.x / g, ngrp = kBlock / g;
   const int np = (cols + 1) & ~1;  // players, padded to even
   const int rounds = np - 1, half = np / 2;
   const double tol = 1e-15;
@@ -96,50 +109,55 @@ __device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double
     if (threadIdx.x == 0) *flag = 0;
     __syncthreads();
     for (int rd = 0; rd < rounds; ++rd) {
-      for (int pk = w; pk < half; pk += kWaves) {
-        int p, q;
-        if (pk == 0) {
-          p = np - 1;
-          q = rd;
-        } else {
-          p = (rd + pk) % (np - 1);
-          q = (rd - pk + (np - 1)) % (np - 1);
+      for (int pk0 = 0; pk0 < half; pk0 += ngrp) {
+        const int pk = pk0 + grp;
+        int p = -1, q = -1;
+        if (pk < half) {
+          if (pk == 0) {
+            p = np - 1;
+            q = rd;
+          } else {
+            p = (rd + pk) % (np - 1);
+            q = (rd - pk + (np - 1)) % (np - 1);
+          }
+          if (p > q) {
+            const int tmp = p;
+            p = q;
+            q = tmp;
+          }
         }
-        if (p >= cols || q >= cols) continue;
-        if (p > q) {
-          const int tmp = p;
-          p = q;
-          q = tmp;
-        }
-        double* gp = G + (size_t)p * ldg;
-        double* gq = G + (size_t)q * ldg;
-        double a = 0.0, b = 0.0, g = 0.0;
-        for (int i = lane; i < rows; i += kWave) {
-          const double x = gp[i], y = gq[i];
-          a += x * x;
-          b += y * y;
-          g += x * y;
-        }
-        a = wave_sum(a);
-        b = wave_sum(b);
-        g = wave_sum(g);
-        if (g == 0.0 || !(fabs(g) > tol * sqrt(a * b))) continue;
-        const double zeta = (b - a) / (2.0 * g);
+        const bool active = pk < half && q < cols;
+        double a = 0.0, b = 0.0, c2 = 0.0;
+        double* gp = G + (size_t)(active ? p : 0) * ldg;
+        double* gq = G + (size_t)(active ? q : 0) * ldg;
+        if (active)
+          for (int i = gl; i < rows; i += g) {
+            const double x = gp[i], y = gq[i];
+            a += x * x;
+            b += y * y;
+            c2 += x * y;
+          }
+        // every lane of the wavefront takes part in the shuffles
+        a = group_sum(a, g);
+        b = group_sum(b, g);
+        c2 = group_sum(c2, g);
+        if (!active || c2 == 0.0 || !(fabs(c2) > tol * sqrt(a * b))) continue;
+        const double zeta = (b - a) / (2.0 * c2);
         const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-        for (int i = lane; i < rows; i += kWave) {
+        for (int i = gl; i < rows; i += g) {
           const double x = gp[i], y = gq[i];
           gp[i] = c * x - s * y;
           gq[i] = s * x + c * y;
         }
         double* vp = V + (size_t)p * ldv;
         double* vq = V + (size_t)q * ldv;
-        for (int i = lane; i < cols; i += kWave) {
+        for (int i = gl; i < cols; i += g) {
           const double x = vp[i], y = vq[i];
           vp[i] = c * x - s * y;
           vq[i] = s * x + c * y;
         }
-        if (lane == 0) *flag = 1;
+        if (gl == 0) *flag = 1;
       }
       __syncthreads();
     }

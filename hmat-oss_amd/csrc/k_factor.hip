@@ -90,30 +90,48 @@ __global__ void __launch_bounds__(256) k_gemm(const FTask* tasks, const GemmTile
 }
 
 // ---------------------------------------------------------------------------
-// POTRF (lower, right-looking, LDS for n <= 128)
+// POTRF (lower, left-looking): column j is updated by one thread per row with
+// the already factorised columns (row of L read from LDS, L[j,:] broadcast),
+// then scaled.  Two barriers per column.  LDS holds the block when n <= 128.
 // ---------------------------------------------------------------------------
 constexpr int kLdsDense = 128;
 
+constexpr int kNb = 16;  // panel width of the blocked POTRF / TRSM
+
+// Right-looking blocked Cholesky: each 16-column panel is factorised
+// left-looking (short dots over the panel only), then the trailing lower
+// triangle gets the panel's rank-16 update with all threads.
 template <typename T>
 __device__ void potrf_body(T* L, int ld, int n, int* info) {
-  for (int j = 0; j < n; ++j) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
+  for (int jb = 0; jb < n; jb += kNb) {
+    const int je = jb + kNb < n ? jb + kNb : n;
+    for (int j = jb; j < je; ++j) {
+      for (int i = j + threadIdx.x; i < n; i += kBlock) {
+        T s = L[i + (size_t)j * ld];
+        for (int l = jb; l < j; ++l) s -= L[i + (size_t)l * ld] * L[j + (size_t)l * ld];
+        L[i + (size_t)j * ld] = s;
+      }
+      __syncthreads();
       const T d = L[j + (size_t)j * ld];
-      if (!(d > (T)0)) atomicOr(info, 1);
-      L[j + (size_t)j * ld] = sqrt(d);
+      const T piv = sqrt(d);
+      for (int i = j + 1 + threadIdx.x; i < n; i += kBlock) L[i + (size_t)j * ld] /= piv;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        if (!(d > (T)0)) atomicOr(info, 1);
+        L[j + (size_t)j * ld] = piv;
+      }
     }
     __syncthreads();
-    const T piv = L[j + (size_t)j * ld];
-    for (int i = j + 1 + threadIdx.x; i < n; i += kBlock) L[i + (size_t)j * ld] /= piv;
-    __syncthreads();
-    const int r = n - j - 1;
+    const int r = n - je;
     for (int e = threadIdx.x; e < r * r; e += kBlock) {
-      const int i = j + 1 + e % r, c = j + 1 + e / r;
-      if (c <= i) L[i + (size_t)c * ld] -= L[i + (size_t)j * ld] * L[c + (size_t)j * ld];
+      const int i = je + e % r, c = je + e / r;
+      if (c > i) continue;
+      T s = 0;
+      for (int l = jb; l < je; ++l) s += L[i + (size_t)l * ld] * L[c + (size_t)l * ld];
+      L[i + (size_t)c * ld] -= s;
     }
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 template <typename T>
@@ -128,6 +146,7 @@ __global__ void __launch_bounds__(kBlock) k_potrf(const FTask* tasks, int* info)
       const int i = e % n, j = e / n;
       lds[i + j * n] = A[i + (size_t)j * lda];
     }
+    __syncthreads();
     potrf_body(lds, n, n, info);
     for (int e = threadIdx.x; e < n * n; e += kBlock) {
       const int i = e % n, j = e / n;
@@ -143,67 +162,115 @@ __global__ void __launch_bounds__(kBlock) k_potrf(const FTask* tasks, int* info)
 }
 
 // ---------------------------------------------------------------------------
-// TRSM with a lower-triangular leaf L (n x n):
+// TRSM with a lower-triangular leaf L (n x n), L staged in LDS:
 //   mode 0: X (m x n) <- X L^{-T}     (solveUpperTriangularRight, LLt)
 //   mode 1: X (n x k) <- L^{-1} X     (solveLowerTriangularLeft)
 //   mode 2: X (n x k) <- L^{-T} X     (solveUpperTriangularLeft, Uplo LOWER)
+// Rows (mode 0) / columns (modes 1, 2) of X are independent: one thread runs
+// the substitution of one of them, reading L from LDS (broadcast) and its own
+// row / column from LDS when it fits, so no barrier is needed inside.
 // ---------------------------------------------------------------------------
+constexpr int kShDoubles = 19456;  // 152 KiB of LDS per workgroup
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* fail) {
-  __shared__ T Ls[kLdsDense * kLdsDense];
+  __shared__ double sh[kShDoubles];
   if (*fail) return;
   const FTask& t = tasks[blockIdx.x];
   const int n = t.n.s;  // order of L
+  const int other = eff(t.m);
+  if (other <= 0) return;
   const T* Lg = (const T*)t.A;
   const int ldl = t.lda;
-  const bool lds = n <= kLdsDense;
-  if (lds)
-    for (int e = threadIdx.x; e < n * n; e += kBlock) {
-      const int i = e % n, j = e / n;
-      Ls[i + j * n] = Lg[i + (size_t)j * ldl];
-    }
-  __syncthreads();
-  const T* L = lds ? Ls : Lg;
-  const int ll = lds ? n : ldl;
   T* X = (T*)t.C;
   const int ldx = t.ldc;
+  const bool lds_l = n * n <= kShDoubles;
+  double* Ls = sh;
+  if (lds_l)
+    for (int e = threadIdx.x; e < n * n; e += kBlock) Ls[e] = (double)Lg[e % n + (size_t)(e / n) * ldl];
+  // X block staged in LDS behind L when it fits: rows = n for modes 1/2
+  const int xr = t.mode == 0 ? other : n, xc = t.mode == 0 ? n : other;
+  const int xl = xr | 1;  // odd leading dimension: fewer bank conflicts for column walks
+  const bool lds_x = lds_l && n * n + xl * xc <= kShDoubles;
+  double* Xs = sh + n * n;
+  if (lds_x)
+    for (int e = threadIdx.x; e < xr * xc; e += kBlock) {
+      const int i = e % xr, j = e / xr;
+      Xs[i + j * xl] = (double)X[i + (size_t)j * ldx];
+    }
+  __syncthreads();
+  auto Lat = [&](int i, int j) -> double { return lds_l ? Ls[i + j * n] : (double)Lg[i + (size_t)j * ldl]; };
+  auto Xg = [&](int i, int j) -> double { return lds_x ? Xs[i + j * xl] : (double)X[i + (size_t)j * ldx]; };
+  auto Xp = [&](int i, int j, double v) {
+    if (lds_x)
+      Xs[i + j * xl] = v;
+    else
+      X[i + (size_t)j * ldx] = (T)v;
+  };
+  // Blocked substitution: a 16-wide diagonal block is solved with one thread per
+  // independent row / column, then the rest gets a rank-16 update from all threads.
   if (t.mode == 0) {
-    const int m = eff(t.m);
-    for (int j = 0; j < n; ++j) {
-      const T d = L[j + (size_t)j * ll];
-      for (int r = threadIdx.x; r < m; r += kBlock) X[r + (size_t)j * ldx] /= d;
+    for (int jb = 0; jb < n; jb += kNb) {
+      const int je = jb + kNb < n ? jb + kNb : n;
+      for (int r = threadIdx.x; r < other; r += kBlock)
+        for (int c = jb; c < je; ++c) {
+          double s = Xg(r, c);
+          for (int l = jb; l < c; ++l) s -= Lat(c, l) * Xg(r, l);
+          Xp(r, c, s / Lat(c, c));
+        }
       __syncthreads();
-      const int nc = n - j - 1;
-      for (int e = threadIdx.x; e < m * nc; e += kBlock) {
-        const int r = e % m, c = j + 1 + e / m;
-        X[r + (size_t)c * ldx] -= X[r + (size_t)j * ldx] * L[c + (size_t)j * ll];
+      const int nc = n - je;
+      for (int e = threadIdx.x; e < other * nc; e += kBlock) {
+        const int r = e % other, c = je + e / other;
+        double s = Xg(r, c);
+        for (int l = jb; l < je; ++l) s -= Xg(r, l) * Lat(c, l);
+        Xp(r, c, s);
       }
       __syncthreads();
     }
   } else if (t.mode == 1) {
-    const int k = eff(t.m);
-    for (int j = 0; j < n; ++j) {
-      const T d = L[j + (size_t)j * ll];
-      for (int c = threadIdx.x; c < k; c += kBlock) X[j + (size_t)c * ldx] /= d;
+    for (int jb = 0; jb < n; jb += kNb) {
+      const int je = jb + kNb < n ? jb + kNb : n;
+      for (int c = threadIdx.x; c < other; c += kBlock)
+        for (int j = jb; j < je; ++j) {
+          double s = Xg(j, c);
+          for (int l = jb; l < j; ++l) s -= Lat(j, l) * Xg(l, c);
+          Xp(j, c, s / Lat(j, j));
+        }
       __syncthreads();
-      const int nr = n - j - 1;
-      for (int e = threadIdx.x; e < nr * k; e += kBlock) {
-        const int i = j + 1 + e % nr, c = e / nr;
-        X[i + (size_t)c * ldx] -= L[i + (size_t)j * ll] * X[j + (size_t)c * ldx];
+      const int nr = n - je;
+      for (int e = threadIdx.x; e < nr * other; e += kBlock) {
+        const int i = je + e % nr, c = e / nr;
+        double s = Xg(i, c);
+        for (int l = jb; l < je; ++l) s -= Lat(i, l) * Xg(l, c);
+        Xp(i, c, s);
       }
       __syncthreads();
     }
   } else {
-    const int k = eff(t.m);
-    for (int j = n - 1; j >= 0; --j) {
-      const T d = L[j + (size_t)j * ll];
-      for (int c = threadIdx.x; c < k; c += kBlock) X[j + (size_t)c * ldx] /= d;
+    for (int je = n; je > 0; je -= kNb) {
+      const int jb = je - kNb > 0 ? je - kNb : 0;
+      for (int c = threadIdx.x; c < other; c += kBlock)
+        for (int j = je - 1; j >= jb; --j) {
+          double s = Xg(j, c);
+          for (int l = j + 1; l < je; ++l) s -= Lat(l, j) * Xg(l, c);
+          Xp(j, c, s / Lat(j, j));
+        }
       __syncthreads();
-      for (int e = threadIdx.x; e < j * k; e += kBlock) {
-        const int i = e % j, c = e / j;
-        X[i + (size_t)c * ldx] -= L[j + (size_t)i * ll] * X[j + (size_t)c * ldx];
+      for (int e = threadIdx.x; e < jb * other; e += kBlock) {
+        const int i = e % jb, c = e / jb;
+        double s = Xg(i, c);
+        for (int l = jb; l < je; ++l) s -= Lat(l, i) * Xg(l, c);
+        Xp(i, c, s);
       }
       __syncthreads();
+    }
+  }
+  if (lds_x) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < xr * xc; e += kBlock) {
+      const int i = e % xr, j = e / xr;
+      X[i + (size_t)j * ldx] = (T)Xs[i + j * xl];
     }
   }
 }
@@ -236,29 +303,14 @@ __global__ void k_axpy(const FTask* tasks, const int* fail) {
 }
 
 // ---------------------------------------------------------------------------
-// Recompression helpers (FP64 workspace)
+// Recompression helpers (FP64; operands in LDS when they fit)
 // ---------------------------------------------------------------------------
-// Dense D (m x n, ld m) -> truncated SVD (eps) -> out Rk (cap, overflow).
+// G (gr x gc, ld gr) holds D (tr == false) or D^T (tr == true) of the dense
+// block to compress; V, sig, order scratch (gc^2, gc, gc).  Truncated SVD at
+// eps into out (A = U sqrt(S) on the row side of D).
 template <typename T>
-__device__ int dense_to_rk(double* D, int m, int n, double* ws, double eps, RkRef out,
-                           int* overflow, int* flag) {
-  const bool tr = m < n;
-  const int gr = tr ? n : m, gc = tr ? m : n;
-  double* G = D;
-  if (tr) {
-    G = ws;
-    ws += (size_t)gr * gc;
-    for (int e = threadIdx.x; e < m * n; e += kBlock) {
-      const int i = e % m, j = e / m;
-      G[j + (size_t)i * gr] = D[e];
-    }
-    __syncthreads();
-  }
-  double* V = ws;
-  ws += (size_t)gc * gc;
-  double* sig = ws;
-  ws += gc;
-  int* order = (int*)ws;
+__device__ int dense_to_rk(double* G, int gr, int gc, bool tr, double* V, double* sig, int* order,
+                           double eps, RkRef out, int* overflow, int* flag) {
   jacobi_svd(G, gr, gr, gc, V, gc, flag);
   sorted_norms(G, gr, gr, gc, sig, order);
   int k = find_k(sig, order, gc, eps);
@@ -289,55 +341,70 @@ __device__ int dense_to_rk(double* D, int m, int n, double* ws, double eps, RkRe
   return k;
 }
 
+struct QrScratch {
+  double *taua, *taub, *g, *v, *sig;
+  int* order;
+  double* y;  // max(m, n) x K, reused for A then B
+};
+
 // Panels Ac (m x K, ld m), Bc (n x K, ld n), K < min(m, n): QR both, SVD of
 // Ra Rb^T, cut at eps, write Qa U sqrt(S), Qb V sqrt(S) to out.
 template <typename T>
-__device__ void qr_truncate(double* Ac, int m, double* Bc, int n, int K, double* ws, double eps,
-                           RkRef out, int* overflow, double* red, int* flag) {
-  double* taua = ws; ws += K;
-  double* taub = ws; ws += K;
-  double* g = ws; ws += (size_t)K * K;
-  double* v = ws; ws += (size_t)K * K;
-  double* sig = ws; ws += K;
-  int* order = (int*)ws; ws += K;
-  double* ya = ws; ws += (size_t)m * K;
-  double* yb = ws; ws += (size_t)n * K;
-  house_qr(Ac, m, m, K, taua, red);
-  house_qr(Bc, n, n, K, taub, red);
+__device__ void qr_truncate(double* Ac, int m, double* Bc, int n, int K, const QrScratch& w,
+                            double eps, RkRef out, int* overflow, double* red, int* flag) {
+  house_qr(Ac, m, m, K, w.taua, red);
+  house_qr(Bc, n, n, K, w.taub, red);
   for (int e = threadIdx.x; e < K * K; e += kBlock) {
     const int i = e % K, j = e / K;
     double s = 0.0;
     const int lo = i > j ? i : j;
     for (int l = lo; l < K; ++l) s += Ac[i + (size_t)l * m] * Bc[j + (size_t)l * n];
-    g[i + (size_t)j * K] = s;
+    w.g[i + (size_t)j * K] = s;
   }
   __syncthreads();
-  jacobi_svd(g, K, K, K, v, K, flag);
-  sorted_norms(g, K, K, K, sig, order);
-  int nk = find_k(sig, order, K, eps);
+  jacobi_svd(w.g, K, K, K, w.v, K, flag);
+  sorted_norms(w.g, K, K, K, w.sig, w.order);
+  int nk = find_k(w.sig, w.order, K, eps);
   if (nk > out.cap) {
     if (threadIdx.x == 0) atomicOr(overflow, 1);
     nk = out.cap;
   }
-  for (int e = threadIdx.x; e < nk * m; e += kBlock) {
-    const int i = e % m, c = e / m, j = order[c];
-    ya[e] = (i < K) ? g[i + (size_t)j * K] / sqrt(sig[j]) : 0.0;
-  }
-  for (int e = threadIdx.x; e < nk * n; e += kBlock) {
-    const int i = e % n, c = e / n, j = order[c];
-    yb[e] = (i < K) ? v[i + (size_t)j * K] * sqrt(sig[j]) : 0.0;
-  }
-  __syncthreads();
-  apply_q(Ac, m, m, K, taua, ya, m, nk);
-  apply_q(Bc, n, n, K, taub, yb, n, nk);
   T* A = (T*)out.a;
   T* B = (T*)out.b;
-  for (int e = threadIdx.x; e < nk * m; e += kBlock)
-    A[e % m + (size_t)(e / m) * out.la] = (T)ya[e];
-  for (int e = threadIdx.x; e < nk * n; e += kBlock)
-    B[e % n + (size_t)(e / n) * out.lb] = (T)yb[e];
+  for (int e = threadIdx.x; e < nk * m; e += kBlock) {
+    const int i = e % m, c = e / m, j = w.order[c];
+    w.y[e] = (i < K) ? w.g[i + (size_t)j * K] / sqrt(w.sig[j]) : 0.0;
+  }
+  __syncthreads();
+  apply_q(Ac, m, m, K, w.taua, w.y, m, nk);
+  for (int e = threadIdx.x; e < nk * m; e += kBlock) A[e % m + (size_t)(e / m) * out.la] = (T)w.y[e];
+  __syncthreads();
+  for (int e = threadIdx.x; e < nk * n; e += kBlock) {
+    const int i = e % n, c = e / n, j = w.order[c];
+    w.y[e] = (i < K) ? w.v[i + (size_t)j * K] * sqrt(w.sig[j]) : 0.0;
+  }
+  __syncthreads();
+  apply_q(Bc, n, n, K, w.taub, w.y, n, nk);
+  for (int e = threadIdx.x; e < nk * n; e += kBlock) B[e % n + (size_t)(e / n) * out.lb] = (T)w.y[e];
   __syncthreads();
   if (threadIdx.x == 0) *out.rank = nk;
+}
+
+// doubles of the QR path: panels, reusable y, core
+__device__ inline int64_t qr_panels(int m, int n, int K) { return (int64_t)(m + n) * K; }
+__device__ inline int64_t qr_y(int m, int n, int K) { return (int64_t)(m > n ? m : n) * K; }
+__device__ inline int64_t qr_core(int K) { return 2 * (int64_t)K * K + 4 * (int64_t)K; }
+
+__device__ inline QrScratch carve_qr(double* core, double* y, int K) {
+  QrScratch w;
+  w.taua = core;
+  w.taub = core + K;
+  w.g = core + 2 * K;
+  w.v = w.g + (size_t)K * K;
+  w.sig = w.v + (size_t)K * K;
+  w.order = (int*)(w.sig + K);
+  w.y = y;
+  return w;
 }
 
 // ---------------------------------------------------------------------------
@@ -345,6 +412,7 @@ __device__ void qr_truncate(double* Ac, int m, double* Bc, int n, int K, double*
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsbase, int* overflow) {
+  __shared__ double sh[kShDoubles];
   __shared__ double red[kWaves];
   __shared__ int flag;
   if (overflow[1]) return;
@@ -389,12 +457,14 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     if (threadIdx.x == 0) *tg.rank = k;
     return;
   }
-  double* ws = wsbase + t.ws;
+  double* gws = wsbase + t.ws;
   const int mn = m < n ? m : n;
   if (K >= mn) {
-    // dense path: eval everything, truncated SVD
-    double* D = ws;
-    ws += (size_t)m * n;
+    // dense path: eval everything (oriented so that the Jacobi works on min(m,n) columns)
+    const bool tr = m < n;
+    const int gr = tr ? n : m, gc = mn;
+    const int64_t need = (int64_t)m * n + (int64_t)gc * gc + 2 * gc;
+    double* G = need <= kShDoubles ? sh : gws;
     for (int e = threadIdx.x; e < m * n; e += kBlock) {
       const int i = e % m, j = e / m;
       double s = 0.0;
@@ -410,17 +480,34 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
         for (int l = 0; l < kq; ++l) sp += (double)PA[ii + (size_t)l * p.rk.la] * (double)PB[jj + (size_t)l * p.rk.lb];
         s += p.alpha * sp;
       }
-      D[e] = s;
+      if (tr)
+        G[j + (size_t)i * gr] = s;
+      else
+        G[i + (size_t)j * gr] = s;
     }
     __syncthreads();
-    dense_to_rk<T>(D, m, n, ws, fabs(t.eps), tg, overflow, &flag);
+    double* V = G + (size_t)m * n;
+    dense_to_rk<T>(G, gr, gc, tr, V, V + (size_t)gc * gc, (int*)(V + (size_t)gc * gc + gc),
+                   fabs(t.eps), tg, overflow, &flag);
     return;
   }
   // QR path: concatenate [target, alpha_i * parts] (zero padded)
-  double* Ac = ws;
-  ws += (size_t)m * K;
-  double* Bc = ws;
-  ws += (size_t)n * K;
+  const int64_t pan = qr_panels(m, n, K), yy = qr_y(m, n, K), core = qr_core(K);
+  double *Ac, *y, *cr;
+  if (pan + yy + core <= kShDoubles) {
+    Ac = sh;
+    y = sh + pan;
+    cr = y + yy;
+  } else if (core <= kShDoubles) {
+    Ac = gws;
+    y = gws + pan;
+    cr = sh;
+  } else {
+    Ac = gws;
+    y = gws + pan;
+    cr = y + yy;
+  }
+  double* Bc = Ac + (size_t)m * K;
   for (int e = threadIdx.x; e < m * kt; e += kBlock)
     Ac[e] = (double)TA[e % m + (size_t)(e / m) * tg.la];
   for (int e = threadIdx.x; e < n * kt; e += kBlock)
@@ -444,12 +531,13 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     off += kq;
   }
   __syncthreads();
-  qr_truncate<T>(Ac, m, Bc, n, K, ws, t.eps, tg, overflow, red, &flag);
+  qr_truncate<T>(Ac, m, Bc, n, K, carve_qr(cr, y, K), t.eps, tg, overflow, red, &flag);
 }
 
 // FADD_DENSE: RkMatrix::formattedAddParts with a full part (rk_matrix.cpp:984-1010)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fadd_dense(const FTask* tasks, double* wsbase, int* overflow) {
+  __shared__ double sh[kShDoubles];
   __shared__ int flag;
   if (overflow[1]) return;
   const FTask& t = tasks[blockIdx.x];
@@ -460,17 +548,25 @@ __global__ void __launch_bounds__(kBlock) k_fadd_dense(const FTask* tasks, doubl
   const T* P = (const T*)t.A;
   const int roff = t.parts[0].roff, coff = t.parts[0].coff;
   const int pm = t.m.s, pn = t.n.s;
-  double* D = wsbase + t.ws;
+  const bool tr = m < n;
+  const int mn = tr ? m : n, gr = tr ? n : m, gc = mn;
+  const int64_t need = (int64_t)m * n + (int64_t)gc * gc + 2 * gc;
+  double* G = need <= kShDoubles ? sh : wsbase + t.ws;
   for (int e = threadIdx.x; e < m * n; e += kBlock) {
     const int i = e % m, j = e / m;
     double s = 0.0;
     for (int l = 0; l < kt; ++l) s += (double)TA[i + (size_t)l * tg.la] * (double)TB[j + (size_t)l * tg.lb];
     const int ii = i - roff, jj = j - coff;
     if (ii >= 0 && ii < pm && jj >= 0 && jj < pn) s += t.alpha * (double)P[ii + (size_t)jj * t.lda];
-    D[e] = s;
+    if (tr)
+      G[j + (size_t)i * gr] = s;
+    else
+      G[i + (size_t)j * gr] = s;
   }
   __syncthreads();
-  dense_to_rk<T>(D, m, n, D + (size_t)m * n, t.eps, tg, overflow, &flag);
+  double* V = G + (size_t)m * n;
+  dense_to_rk<T>(G, gr, gc, tr, V, V + (size_t)gc * gc, (int*)(V + (size_t)gc * gc + gc), t.eps, tg,
+                 overflow, &flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -480,6 +576,7 @@ __global__ void __launch_bounds__(kBlock) k_fadd_dense(const FTask* tasks, doubl
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_rkrk(const FTask* tasks, double* wsbase, int* overflow) {
+  __shared__ double sh[kShDoubles];
   __shared__ int flag;
   if (overflow[1]) return;
   const FTask& t = tasks[blockIdx.x];
@@ -491,13 +588,14 @@ __global__ void __launch_bounds__(kBlock) k_rkrk(const FTask* tasks, double* wsb
     return;
   }
   const int p = r1.n;  // common dimension (rows of B1 == rows of A2)
-  double* ws = wsbase + t.ws;
   const bool tr = k1 < k2;
   const int gr = tr ? k2 : k1, gc = tr ? k1 : k2;
+  const int64_t need = (int64_t)gr * gc + (int64_t)gc * gc + 2 * gc;
+  double* ws = need <= kShDoubles ? sh : wsbase + t.ws;
   double* G = ws; ws += (size_t)gr * gc;
   double* V = ws; ws += (size_t)gc * gc;
   double* sig = ws; ws += gc;
-  int* order = (int*)ws; ws += gc;
+  int* order = (int*)ws;
   const T* A1 = (const T*)r1.a;
   const T* B1 = (const T*)r1.b;
   const T* A2 = (const T*)r2.a;
@@ -613,14 +711,16 @@ __global__ void __launch_bounds__(kBlock) k_acafull(const FTask* tasks, const in
 }  // namespace
 
 int64_t fadd_workspace(int m, int n, int K) {
+  // dense path only when K >= min(m, n); QR path otherwise (K actual < min)
   const int mn = m < n ? m : n;
-  int64_t dense = (int64_t)m * n * 2 + (int64_t)mn * mn + 2 * (int64_t)mn + 64;
-  int64_t qr = (int64_t)(m + n) * K * 2 + 2 * (int64_t)K * K + 4 * (int64_t)K + 64;
-  return (std::max(dense, qr) + 63) / 64 * 64;
+  int64_t dense = K >= mn ? (int64_t)m * n + (int64_t)mn * mn + 2 * (int64_t)mn : 0;
+  const int kq = K < mn ? K : mn - 1;
+  int64_t qr = (int64_t)(m + n) * kq + (int64_t)(m > n ? m : n) * kq + 2 * (int64_t)kq * kq + 4 * (int64_t)kq;
+  return (std::max(dense, qr) + 127) / 64 * 64;
 }
 int64_t fadd_dense_workspace(int m, int n) {
   const int mn = m < n ? m : n;
-  return ((int64_t)m * n * 2 + (int64_t)mn * mn + 2 * (int64_t)mn + 127) / 64 * 64;
+  return ((int64_t)m * n + (int64_t)mn * mn + 2 * (int64_t)mn + 127) / 64 * 64;
 }
 int64_t rkrk_workspace(int k1, int k2) {
   const int64_t a = k1 > k2 ? k1 : k2, b = k1 < k2 ? k1 : k2;
