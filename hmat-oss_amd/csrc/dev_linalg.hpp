@@ -11,11 +11,47 @@
 
 namespace hmx {
 
+constexpr int kQrChunk = 16;  // trailing columns reduced together
+
+// w[c] = sum_{i >= j} v_i A[i, c] for c in [c0, c1), v_j = 1, v_i = V[i] below:
+// every thread accumulates its rows for a chunk of columns, one wave_sum per
+// column and a cross-wave sum through LDS 'red' (kWaves * kQrChunk doubles).
+__device__ inline void house_dots(const double* v, int j, int m, const double* A, int lda, int c0,
+                                  int c1, double* w, double* red) {
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  for (int cb = c0; cb < c1; cb += kQrChunk) {
+    const int nc = c1 - cb < kQrChunk ? c1 - cb : kQrChunk;
+    double part[kQrChunk];
+#pragma unroll
+    for (int q = 0; q < kQrChunk; ++q) part[q] = 0.0;
+    for (int i = j + threadIdx.x; i < m; i += kBlock) {
+      const double vi = (i == j) ? 1.0 : v[i];
+#pragma unroll
+      for (int q = 0; q < kQrChunk; ++q)
+        if (q < nc) part[q] += vi * A[i + (size_t)(cb + q) * lda];
+    }
+#pragma unroll
+    for (int q = 0; q < kQrChunk; ++q) {
+      const double sum = wave_sum(part[q]);
+      if (lane == 0 && q < nc) red[wv * kQrChunk + q] = sum;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nc; q += kBlock) {
+      double sum = 0.0;
+#pragma unroll
+      for (int ww = 0; ww < kWaves; ++ww) sum += red[ww * kQrChunk + q];
+      w[cb + q] = sum;
+    }
+    __syncthreads();
+  }
+}
+
 // Householder QR, LAPACK dgeqrf conventions (dlarfg: beta = -sign(alpha)*norm,
 // tau = (beta - alpha) / beta, v = x / (alpha - beta), v[0] = 1 implicit).
 // On exit R is in the upper triangle, v_j below the diagonal, tau[j] set.
-__device__ inline void house_qr(double* A, int lda, int m, int k, double* tau, double* red) {
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+// 'w' is scratch of k doubles, 'red' of kWaves * kQrChunk doubles.
+__device__ inline void house_qr(double* A, int lda, int m, int k, double* tau, double* red,
+                                double* w) {
   const int steps = m < k ? m : k;
   for (int j = 0; j < steps; ++j) {
     double* col = A + (size_t)j * lda;
@@ -37,18 +73,16 @@ __device__ inline void house_qr(double* A, int lda, int m, int k, double* tau, d
       tau[j] = t;
     }
     __syncthreads();
-    if (t != 0.0) {
-      for (int c = j + 1 + w; c < k; c += kWaves) {
-        double* ac = A + (size_t)c * lda;
-        double d = 0.0;
-        for (int i = j + 1 + lane; i < m; i += kWave) d += col[i] * ac[i];
-        d = wave_sum(d) + ac[j];
-        const double f = t * d;
-        if (lane == 0) ac[j] -= f;
-        for (int i = j + 1 + lane; i < m; i += kWave) ac[i] -= f * col[i];
+    if (t != 0.0 && j + 1 < k) {
+      house_dots(col, j, m, A, lda, j + 1, k, w, red);
+      const int nc = k - j - 1, len = m - j;
+      for (int e = threadIdx.x; e < len * nc; e += kBlock) {
+        const int i = j + e % len, c = j + 1 + e / len;
+        const double vi = (i == j) ? 1.0 : col[i];
+        A[i + (size_t)c * lda] -= t * vi * w[c];
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
   if (threadIdx.x == 0)
     for (int j = steps; j < k; ++j) tau[j] = 0.0;
@@ -56,22 +90,19 @@ __device__ inline void house_qr(double* A, int lda, int m, int k, double* tau, d
 }
 
 // Y (m x r, ldy) <- Q Y with Q = H_0 ... H_{nref-1} from house_qr.
+// 'w' scratch of r doubles, 'red' of kWaves * kQrChunk doubles.
 __device__ inline void apply_q(const double* V, int ldv, int m, int nref, const double* tau,
-                               double* Y, int ldy, int r) {
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+                               double* Y, int ldy, int r, double* red, double* w) {
   for (int j = nref - 1; j >= 0; --j) {
     const double t = tau[j];
-    if (t != 0.0) {
-      const double* v = V + (size_t)j * ldv;
-      for (int c = w; c < r; c += kWaves) {
-        double* y = Y + (size_t)c * ldy;
-        double d = 0.0;
-        for (int i = j + 1 + lane; i < m; i += kWave) d += v[i] * y[i];
-        d = wave_sum(d) + y[j];
-        const double f = t * d;
-        if (lane == 0) y[j] -= f;
-        for (int i = j + 1 + lane; i < m; i += kWave) y[i] -= f * v[i];
-      }
+    if (t == 0.0) continue;
+    const double* v = V + (size_t)j * ldv;
+    house_dots(v, j, m, Y, ldy, 0, r, w, red);
+    const int len = m - j;
+    for (int e = threadIdx.x; e < len * r; e += kBlock) {
+      const int i = j + e % len, c = e / len;
+      const double vi = (i == j) ? 1.0 : v[i];
+      Y[i + (size_t)c * ldy] -= t * vi * w[c];
     }
     __syncthreads();
   }
@@ -105,6 +136,23 @@ __device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double
   const int np = (cols + 1) & ~1;  // players, padded to even
   const int rounds = np - 1, half = np / 2;
   const double tol = 1e-15;
+  // ||G||_F^2: columns below 1e-15 ||G|| carry no information for any
+  // truncation threshold used here; pairs of them are not rotated (a purely
+  // relative test never settles on such noise-level columns)
+  __shared__ double fro_s;
+  {
+    double f = 0.0;
+    for (int e = threadIdx.x; e < rows * cols; e += kBlock) {
+      const double x = G[e % rows + (size_t)(e / rows) * ldg];
+      f += x * x;
+    }
+    f = wave_sum(f);
+    if (threadIdx.x == 0) fro_s = 0.0;
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&fro_s, f);
+    __syncthreads();
+  }
+  const double floor2 = 1e-30 * fro_s;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     if (threadIdx.x == 0) *flag = 0;
     __syncthreads();
@@ -141,7 +189,7 @@ __device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double
         a = group_sum(a, g);
         b = group_sum(b, g);
         c2 = group_sum(c2, g);
-        if (!active || c2 == 0.0 || !(fabs(c2) > tol * sqrt(a * b))) continue;
+        if (!active || c2 == 0.0 || !(fabs(c2) > tol * sqrt(a * b)) || !(fabs(c2) > floor2)) continue;
         const double zeta = (b - a) / (2.0 * c2);
         const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;

@@ -13,6 +13,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1032,8 +1034,31 @@ class Exec {
     plan->ranks_used = fs_.rank_used;
     plan->ntasks = (long)n;
     plan->waves = W;
+    static const bool prof = getenv("HMX_PROF_FADD") != nullptr;
+    DevBuf pbuf;
+    if (prof) {
+      // task ids for the per-task phase clocks
+      for (size_t q = 0; q < n; ++q) sorted[q].pad = (int)q;
+      upload(plan->tasks, sorted.data(), n * sizeof(FTask), s);
+      pbuf.alloc(n * 8 * sizeof(double));
+      HMX_CHECK(hipMemsetAsync(pbuf.p, 0, n * 8 * sizeof(double), s));
+      g_fadd_prof = pbuf.as<double>();
+    }
     execute_plan<T>(*plan, fs_, s);
     HMX_CHECK(hipStreamSynchronize(s));
+    if (prof) {
+      g_fadd_prof = nullptr;
+      std::vector<double> h(n * 8);
+      HMX_CHECK(hipMemcpy(h.data(), pbuf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+      FILE* f = fopen(getenv("HMX_PROF_FADD"), "a");
+      if (f) {
+        for (size_t q = 0; q < n; ++q)
+          if (sorted[q].type == T_FADD)
+            fprintf(f, "%d %g %g %g %g %g %g %g %g\n", waves_[order[q]], h[8 * q], h[8 * q + 1], h[8 * q + 2],
+                    h[8 * q + 3], h[8 * q + 4], h[8 * q + 5], h[8 * q + 6], h[8 * q + 7]);
+        fclose(f);
+      }
+    }
     if (record_) record_->push_back(plan);
     M_.stats.tasks += (long)n;
     M_.stats.waves += W;

@@ -16,6 +16,10 @@ int64_t fadd_workspace(int m, int n, int kmax_concat);
 int64_t fadd_dense_workspace(int m, int n);
 int64_t rkrk_workspace(int k1, int k2);
 
+// When non-null, k_fadd records 8 doubles per task at g_fadd_prof[8 * task.pad]
+// (path, m, n, K, total, concat, qr, jacobi clocks) for performance analysis.
+extern double* g_fadd_prof;
+
 // One batched launch of 'count' tasks of 'type' (GEMM: 'ntiles' tiles).
 // flags[0] |= 1 on an Rk capacity overflow, flags[1] |= 1 on a non-positive
 // pivot in POTRF.

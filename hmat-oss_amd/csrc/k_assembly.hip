@@ -389,8 +389,8 @@ template <typename T>
 __device__ int truncate_panels(double* A, int m, double* B, int n, int k, const AcaWs& w,
                                double eps, T* outA, T* outB, int cap, int* overflow, double* red,
                                int* flag) {
-  house_qr(A, m, m, k, w.taua, red);
-  house_qr(B, n, n, k, w.taub, red);
+  house_qr(A, m, m, k, w.taua, red, w.sig);
+  house_qr(B, n, n, k, w.taub, red, w.sig);
   // g = Ra Rb^T (k x k), Ra, Rb upper triangular
   for (int e = threadIdx.x; e < k * k; e += kBlock) {
     const int i = e % k, j = e / k;
@@ -417,8 +417,8 @@ __device__ int truncate_panels(double* A, int m, double* B, int n, int k, const 
     w.yb[e] = (i < k) ? w.v[i + (size_t)j * k] * sqrt(w.sig[j]) : 0.0;
   }
   __syncthreads();
-  apply_q(A, m, m, k, w.taua, w.ya, m, nk);
-  apply_q(B, n, n, k, w.taub, w.yb, n, nk);
+  apply_q(A, m, m, k, w.taua, w.ya, m, nk, red, w.v);
+  apply_q(B, n, n, k, w.taub, w.yb, n, nk, red, w.v);
   for (int e = threadIdx.x; e < nk * m; e += kBlock) outA[e] = (T)w.ya[e];
   for (int e = threadIdx.x; e < nk * n; e += kBlock) outB[e] = (T)w.yb[e];
   __syncthreads();
@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(kBlock) k_aca(const LeafDesc* desc, const AcaT
                                                 T* arena, int* ranks, int* overflow, double* ws,
                                                 PointSet rp, PointSet cp, KernelParams kp,
                                                 double aca_eps, double trunc_eps, int partial) {
-  __shared__ double red[kWaves];
+  __shared__ double red[kWaves * kQrChunk];
   __shared__ double rv[kWaves];
   __shared__ int ri[kWaves];
   __shared__ int flag;

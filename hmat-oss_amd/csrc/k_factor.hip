@@ -344,16 +344,20 @@ __device__ int dense_to_rk(double* G, int gr, int gc, bool tr, double* V, double
 struct QrScratch {
   double *taua, *taub, *g, *v, *sig;
   int* order;
-  double* y;  // max(m, n) x K, reused for A then B
+  double* wq;  // K doubles: Householder dot products
+  double* y;   // max(m, n) x K, reused for A then B
 };
 
 // Panels Ac (m x K, ld m), Bc (n x K, ld n), K < min(m, n): QR both, SVD of
 // Ra Rb^T, cut at eps, write Qa U sqrt(S), Qb V sqrt(S) to out.
 template <typename T>
 __device__ void qr_truncate(double* Ac, int m, double* Bc, int n, int K, const QrScratch& w,
-                            double eps, RkRef out, int* overflow, double* red, int* flag) {
-  house_qr(Ac, m, m, K, w.taua, red);
-  house_qr(Bc, n, n, K, w.taub, red);
+                            double eps, RkRef out, int* overflow, double* red, int* flag,
+                            double* pr = nullptr) {
+  const long q0 = wall_clock64();
+  house_qr(Ac, m, m, K, w.taua, red, w.wq);
+  house_qr(Bc, n, n, K, w.taub, red, w.wq);
+  const long q1 = wall_clock64();
   for (int e = threadIdx.x; e < K * K; e += kBlock) {
     const int i = e % K, j = e / K;
     double s = 0.0;
@@ -363,6 +367,11 @@ __device__ void qr_truncate(double* Ac, int m, double* Bc, int n, int K, const Q
   }
   __syncthreads();
   jacobi_svd(w.g, K, K, K, w.v, K, flag);
+  const long q2 = wall_clock64();
+  if (pr && threadIdx.x == 0) {
+    pr[6] = (double)(q1 - q0);
+    pr[7] = (double)(q2 - q1);
+  }
   sorted_norms(w.g, K, K, K, w.sig, w.order);
   int nk = find_k(w.sig, w.order, K, eps);
   if (nk > out.cap) {
@@ -376,7 +385,7 @@ __device__ void qr_truncate(double* Ac, int m, double* Bc, int n, int K, const Q
     w.y[e] = (i < K) ? w.g[i + (size_t)j * K] / sqrt(w.sig[j]) : 0.0;
   }
   __syncthreads();
-  apply_q(Ac, m, m, K, w.taua, w.y, m, nk);
+  apply_q(Ac, m, m, K, w.taua, w.y, m, nk, red, w.wq);
   for (int e = threadIdx.x; e < nk * m; e += kBlock) A[e % m + (size_t)(e / m) * out.la] = (T)w.y[e];
   __syncthreads();
   for (int e = threadIdx.x; e < nk * n; e += kBlock) {
@@ -384,7 +393,7 @@ __device__ void qr_truncate(double* Ac, int m, double* Bc, int n, int K, const Q
     w.y[e] = (i < K) ? w.v[i + (size_t)j * K] * sqrt(w.sig[j]) : 0.0;
   }
   __syncthreads();
-  apply_q(Bc, n, n, K, w.taub, w.y, n, nk);
+  apply_q(Bc, n, n, K, w.taub, w.y, n, nk, red, w.wq);
   for (int e = threadIdx.x; e < nk * n; e += kBlock) B[e % n + (size_t)(e / n) * out.lb] = (T)w.y[e];
   __syncthreads();
   if (threadIdx.x == 0) *out.rank = nk;
@@ -393,7 +402,7 @@ __device__ void qr_truncate(double* Ac, int m, double* Bc, int n, int K, const Q
 // doubles of the QR path: panels, reusable y, core
 __device__ inline int64_t qr_panels(int m, int n, int K) { return (int64_t)(m + n) * K; }
 __device__ inline int64_t qr_y(int m, int n, int K) { return (int64_t)(m > n ? m : n) * K; }
-__device__ inline int64_t qr_core(int K) { return 2 * (int64_t)K * K + 4 * (int64_t)K; }
+__device__ inline int64_t qr_core(int K) { return 2 * (int64_t)K * K + 5 * (int64_t)K; }
 
 __device__ inline QrScratch carve_qr(double* core, double* y, int K) {
   QrScratch w;
@@ -403,6 +412,7 @@ __device__ inline QrScratch carve_qr(double* core, double* y, int K) {
   w.v = w.g + (size_t)K * K;
   w.sig = w.v + (size_t)K * K;
   w.order = (int*)(w.sig + K);
+  w.wq = w.sig + 2 * K;
   w.y = y;
   return w;
 }
@@ -411,12 +421,15 @@ __device__ inline QrScratch carve_qr(double* core, double* y, int K) {
 // FADD: RkMatrix::formattedAddParts (rk_matrix.cpp:846-981)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsbase, int* overflow) {
+__global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsbase, int* overflow,
+                                                 double* prof) {
   __shared__ double sh[kShDoubles];
-  __shared__ double red[kWaves];
+  __shared__ double red[kWaves * kQrChunk];
   __shared__ int flag;
   if (overflow[1]) return;
   const FTask& t = tasks[blockIdx.x];
+  const long t0 = wall_clock64();
+  double* pr = prof ? prof + (size_t)8 * t.pad : nullptr;
   const RkRef tg = t.tgt;
   const int m = tg.m, n = tg.n;
   const int kt = rank_of(tg);
@@ -455,6 +468,9 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     }
     __syncthreads();
     if (threadIdx.x == 0) *tg.rank = k;
+    if (pr && threadIdx.x == 0) {
+      pr[0] = 0; pr[1] = m; pr[2] = n; pr[3] = K; pr[4] = (double)(wall_clock64() - t0);
+    }
     return;
   }
   double* gws = wsbase + t.ws;
@@ -487,8 +503,13 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     }
     __syncthreads();
     double* V = G + (size_t)m * n;
+    const long t1 = wall_clock64();
     dense_to_rk<T>(G, gr, gc, tr, V, V + (size_t)gc * gc, (int*)(V + (size_t)gc * gc + gc),
                    fabs(t.eps), tg, overflow, &flag);
+    if (pr && threadIdx.x == 0) {
+      pr[0] = 1; pr[1] = m; pr[2] = n; pr[3] = K; pr[4] = (double)(wall_clock64() - t0);
+      pr[5] = (double)(t1 - t0); pr[6] = (G == sh);
+    }
     return;
   }
   // QR path: concatenate [target, alpha_i * parts] (zero padded)
@@ -531,7 +552,12 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     off += kq;
   }
   __syncthreads();
-  qr_truncate<T>(Ac, m, Bc, n, K, carve_qr(cr, y, K), t.eps, tg, overflow, red, &flag);
+  const long t1 = wall_clock64();
+  qr_truncate<T>(Ac, m, Bc, n, K, carve_qr(cr, y, K), t.eps, tg, overflow, red, &flag, pr);
+  if (pr && threadIdx.x == 0) {
+    pr[0] = 2 + (Ac == sh ? 0 : (cr == sh ? 1 : 2)); pr[1] = m; pr[2] = n; pr[3] = K;
+    pr[4] = (double)(wall_clock64() - t0); pr[5] = (double)(t1 - t0);
+  }
 }
 
 // FADD_DENSE: RkMatrix::formattedAddParts with a full part (rk_matrix.cpp:984-1010)
@@ -715,7 +741,7 @@ int64_t fadd_workspace(int m, int n, int K) {
   const int mn = m < n ? m : n;
   int64_t dense = K >= mn ? (int64_t)m * n + (int64_t)mn * mn + 2 * (int64_t)mn : 0;
   const int kq = K < mn ? K : mn - 1;
-  int64_t qr = (int64_t)(m + n) * kq + (int64_t)(m > n ? m : n) * kq + 2 * (int64_t)kq * kq + 4 * (int64_t)kq;
+  int64_t qr = (int64_t)(m + n) * kq + (int64_t)(m > n ? m : n) * kq + 2 * (int64_t)kq * kq + 5 * (int64_t)kq;
   return (std::max(dense, qr) + 127) / 64 * 64;
 }
 int64_t fadd_dense_workspace(int m, int n) {
@@ -726,6 +752,8 @@ int64_t rkrk_workspace(int k1, int k2) {
   const int64_t a = k1 > k2 ? k1 : k2, b = k1 < k2 ? k1 : k2;
   return (a * b + b * b + 2 * b + 127) / 64 * 64;
 }
+
+double* g_fadd_prof = nullptr;
 
 template <typename T>
 void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile* tiles,
@@ -742,7 +770,7 @@ void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile*
       hipLaunchKernelGGL(k_trsm<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
       break;
     case T_FADD:
-      hipLaunchKernelGGL(k_fadd<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags);
+      hipLaunchKernelGGL(k_fadd<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags, g_fadd_prof);
       break;
     case T_FADD_DENSE:
       hipLaunchKernelGGL(k_fadd_dense<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags);
