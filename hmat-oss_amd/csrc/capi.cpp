@@ -95,6 +95,9 @@ int assemble_generic(hmat_matrix_t* h, hmat_assemble_context_t* ctx) {
     spec.kind = k->kind;
     spec.length = k->length;
     spec.nugget = k->nugget;
+    spec.shift = k->kind == HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL ? k->shift : 0.0;
+    if (k->kind < 0 || k->kind > HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL)
+      throw std::runtime_error("hmat_mi355x_kernel_t: unknown kernel kind");
     spec.dim = k->dimension;
     spec.n = k->n;
     spec.points.assign(k->points, k->points + (size_t)k->n * k->dimension);
@@ -341,7 +344,8 @@ void hmat_mi355x_kernel_compute(void* ctx, int row, int col, void* result) {
   double d[3] = {0, 0, 0};
   for (int q = 0; q < k->dimension && q < 3; ++q)
     d[q] = k->points[(size_t)row * k->dimension + q] - k->points[(size_t)col * k->dimension + q];
-  hmx::KernelParams kp{k->kind, k->length, k->nugget};
+  hmx::KernelParams kp{k->kind, k->length, k->nugget,
+                      k->kind == HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL ? k->shift : 0.0};
   *(double*)result = hmx::kernel_value(kp, d[0], d[1], d[2], row == col);
 }
 

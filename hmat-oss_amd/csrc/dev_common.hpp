@@ -24,16 +24,18 @@ constexpr int kBlock = 256;  // threads per workgroup for the one-workgroup-per-
 constexpr int kWaves = kBlock / kWave;
 
 // Built-in interaction kernels (user numbering for the nugget).
-enum KernelKind : int { kExponential = 0, kGaussian = 1 };
+enum KernelKind : int { kExponential = 0, kGaussian = 1, kShiftedExponential = 2 };
 
 struct KernelParams {
   int kind;
   double length;
   double nugget;
+  double shift;  // kShiftedExponential: z offset of the source (non-symmetric kernel)
 };
 
 __host__ __device__ inline double kernel_value(const KernelParams& k, double dx, double dy,
                                                double dz, bool same) {
+  if (k.kind == kShiftedExponential) dz -= k.shift;
   const double r2 = dx * dx + dy * dy + dz * dz;
   double v;
   if (k.kind == kGaussian) {
@@ -46,10 +48,43 @@ __host__ __device__ inline double kernel_value(const KernelParams& k, double dx,
 }
 
 // ---- reductions -------------------------------------------------------------
+// Lane exchange through DPP (a VALU operand modifier, no LDS round trip).
+// kCtrl: 0xB1 swaps neighbours (quad_perm 1,0,3,2), 0x4E swaps pairs
+// (quad_perm 2,3,0,1), 0x141 mirrors each half-row of 8, 0x140 each row of 16.
+template <int kCtrl>
+__device__ inline double dpp_swap(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int kCtrl>
+__device__ inline float dpp_swap(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+
+// Sum over aligned groups of g lanes (g a power of two <= 64, uniform across
+// the wavefront; every lane of a group must be active).  Each step adds the
+// value of a partner lane, so every lane of a group ends with the same,
+// bitwise identical sum.
+template <typename T>
+__device__ inline T group_sum(T v, int g) {
+  if (g >= 2) v += dpp_swap<0xB1>(v);
+  if (g >= 4) v += dpp_swap<0x4E>(v);
+  if (g >= 8) v += dpp_swap<0x141>(v);
+  if (g >= 16) v += dpp_swap<0x140>(v);
+  if (g >= 32) v += __shfl_xor(v, 16, kWave);
+  if (g >= 64) v += __shfl_xor(v, 32, kWave);
+  return v;
+}
+
 template <typename T>
 __device__ inline T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  v += dpp_swap<0xB1>(v);
+  v += dpp_swap<0x4E>(v);
+  v += dpp_swap<0x141>(v);
+  v += dpp_swap<0x140>(v);
+  v += __shfl_xor(v, 16, kWave);
+  v += __shfl_xor(v, 32, kWave);
   return v;
 }
 

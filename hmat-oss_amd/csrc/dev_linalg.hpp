@@ -2,217 +2,320 @@
 // Householder QR of a tall panel, application of its Q, one-sided (Hestenes)
 // Jacobi SVD, and the singular-value ordering / truncation rule.
 //
-// All routines are executed cooperatively by a whole workgroup of kBlock
-// threads, work on column-major matrices that may live in LDS or in global
-// memory (generic pointers), and end with a __syncthreads().  Arithmetic is
-// FP64; callers convert at the boundary.
+// The routines work on column-major matrices that may live in LDS or in
+// global memory (generic pointers) and end with a synchronisation of the
+// threads that ran them.  Arithmetic is FP64; callers convert at the boundary.
 #pragma once
 #include "dev_common.hpp"
 
 namespace hmx {
 
-constexpr int kQrChunk = 16;  // trailing columns reduced together
+constexpr int kQrChunk = 16;  // size factor of the reduction scratch kept by callers
 
-// w[c] = sum_{i >= j} v_i A[i, c] for c in [c0, c1), v_j = 1, v_i = V[i] below:
-// every thread accumulates its rows for a chunk of columns, one wave_sum per
-// column and a cross-wave sum through LDS 'red' (kWaves * kQrChunk doubles).
-__device__ inline void house_dots(const double* v, int j, int m, const double* A, int lda, int c0,
-                                  int c1, double* w, double* red) {
-  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-  for (int cb = c0; cb < c1; cb += kQrChunk) {
-    const int nc = c1 - cb < kQrChunk ? c1 - cb : kQrChunk;
-    double part[kQrChunk];
-#pragma unroll
-    for (int q = 0; q < kQrChunk; ++q) part[q] = 0.0;
-    for (int i = j + threadIdx.x; i < m; i += kBlock) {
-      const double vi = (i == j) ? 1.0 : v[i];
-#pragma unroll
-      for (int q = 0; q < kQrChunk; ++q)
-        if (q < nc) part[q] += vi * A[i + (size_t)(cb + q) * lda];
-    }
-#pragma unroll
-    for (int q = 0; q < kQrChunk; ++q) {
-      const double sum = wave_sum(part[q]);
-      if (lane == 0 && q < nc) red[wv * kQrChunk + q] = sum;
-    }
+// The routines below are written for a "team" of kLanes threads: one
+// wavefront (kLanes == kWave, synchronised by wave_barrier: LDS accesses of
+// one wavefront complete in program order) or the whole workgroup
+// (kLanes == kBlock, __syncthreads).  Work items (columns, column pairs) are
+// spread over aligned groups of g lanes, g as large as the item count allows;
+// a group reduces its partial sums with group_sum, so a reduction costs
+// log2(g) register exchanges instead of a round trip through LDS.
+
+template <int kLanes>
+__device__ inline void team_sync() {
+  if constexpr (kLanes == kWave)
+    __builtin_amdgcn_wave_barrier();
+  else
     __syncthreads();
-    for (int q = threadIdx.x; q < nc; q += kBlock) {
-      double sum = 0.0;
-#pragma unroll
-      for (int ww = 0; ww < kWaves; ++ww) sum += red[ww * kQrChunk + q];
-      w[cb + q] = sum;
-    }
-    __syncthreads();
+}
+
+template <int kLanes>
+__device__ inline int team_tid() {
+  return kLanes == kWave ? (int)(threadIdx.x & (kWave - 1)) : (int)threadIdx.x;
+}
+
+// Largest power of two g <= kWave with g * items <= lanes (at least 1).
+__device__ inline int group_lanes(int items, int lanes) {
+  int g = kWave;
+  while (g > 1 && g * items > lanes) g >>= 1;
+  return g;
+}
+
+// sum_{i = i0, i0+g, ... < m} x[i] y[i]; four loads in flight per operand.
+__device__ inline double strided_dot(const double* x, const double* y, int i0, int m, int g) {
+  double s0 = 0.0, s1 = 0.0;
+  int i = i0;
+  for (; i + 3 * g < m; i += 4 * g) {
+    const double x0 = x[i], x1 = x[i + g], x2 = x[i + 2 * g], x3 = x[i + 3 * g];
+    const double y0 = y[i], y1 = y[i + g], y2 = y[i + 2 * g], y3 = y[i + 3 * g];
+    s0 += x0 * y0;
+    s1 += x1 * y1;
+    s0 += x2 * y2;
+    s1 += x3 * y3;
   }
+  for (; i < m; i += g) s0 += x[i] * y[i];
+  return s0 + s1;
+}
+
+// y[i] -= f x[i] for i = i0, i0+g, ... < m (x and y distinct columns).
+__device__ inline void strided_axpy(double* y, double f, const double* x, int i0, int m, int g) {
+  int i = i0;
+  for (; i + 3 * g < m; i += 4 * g) {
+    const double x0 = x[i], x1 = x[i + g], x2 = x[i + 2 * g], x3 = x[i + 3 * g];
+    const double y0 = y[i], y1 = y[i + g], y2 = y[i + 2 * g], y3 = y[i + 3 * g];
+    y[i] = y0 - f * x0;
+    y[i + g] = y1 - f * x1;
+    y[i + 2 * g] = y2 - f * x2;
+    y[i + 3 * g] = y3 - f * x3;
+  }
+  for (; i < m; i += g) y[i] -= f * x[i];
 }
 
 // Householder QR, LAPACK dgeqrf conventions (dlarfg: beta = -sign(alpha)*norm,
 // tau = (beta - alpha) / beta, v = x / (alpha - beta), v[0] = 1 implicit).
 // On exit R is in the upper triangle, v_j below the diagonal, tau[j] set.
-// 'w' is scratch of k doubles, 'red' of kWaves * kQrChunk doubles.
-__device__ inline void house_qr(double* A, int lda, int m, int k, double* tau, double* red,
-                                double* w) {
+// Column c belongs to lane group c mod (kLanes / g): that group forms the
+// reflector of column c and applies every earlier reflector to it.
+template <int kLanes>
+__device__ inline void team_house_qr(double* A, int lda, int m, int k, double* tau) {
+  const int tid = team_tid<kLanes>();
   const int steps = m < k ? m : k;
+  const int g = group_lanes(k, kLanes), ngrp = kLanes / g, grp = tid / g, sub = tid % g;
   for (int j = 0; j < steps; ++j) {
-    double* col = A + (size_t)j * lda;
-    double s = 0.0;
-    for (int i = j + 1 + threadIdx.x; i < m; i += kBlock) s += col[i] * col[i];
-    s = block_sum(s, red);
-    const double alpha = col[j];
-    double t = 0.0, scal = 0.0, beta = alpha;
-    if (s != 0.0) {
-      beta = -copysign(sqrt(alpha * alpha + s), alpha);
-      t = (beta - alpha) / beta;
-      scal = 1.0 / (alpha - beta);
-    }
-    __syncthreads();
-    if (s != 0.0)
-      for (int i = j + 1 + threadIdx.x; i < m; i += kBlock) col[i] *= scal;
-    if (threadIdx.x == 0) {
-      col[j] = beta;
-      tau[j] = t;
-    }
-    __syncthreads();
-    if (t != 0.0 && j + 1 < k) {
-      house_dots(col, j, m, A, lda, j + 1, k, w, red);
-      const int nc = k - j - 1, len = m - j;
-      for (int e = threadIdx.x; e < len * nc; e += kBlock) {
-        const int i = j + e % len, c = j + 1 + e / len;
-        const double vi = (i == j) ? 1.0 : col[i];
-        A[i + (size_t)c * lda] -= t * vi * w[c];
+    double* v = A + (size_t)j * lda;
+    if (grp == j % ngrp) {
+      double s = strided_dot(v, v, j + 1 + sub, m, g);
+      s = group_sum(s, g);
+      const double alpha = v[j];
+      double t = 0.0, beta = alpha, scal = 0.0;
+      if (s != 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + s), alpha);
+        t = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+        for (int i = j + 1 + sub; i < m; i += g) v[i] *= scal;
       }
-      __syncthreads();
+      if (sub == 0) {
+        tau[j] = t;
+        v[j] = beta;
+      }
     }
-  }
-  if (threadIdx.x == 0)
-    for (int j = steps; j < k; ++j) tau[j] = 0.0;
-  __syncthreads();
-}
-
-// Y (m x r, ldy) <- Q Y with Q = H_0 ... H_{nref-1} from house_qr.
-// 'w' scratch of r doubles, 'red' of kWaves * kQrChunk doubles.
-__device__ inline void apply_q(const double* V, int ldv, int m, int nref, const double* tau,
-                               double* Y, int ldy, int r, double* red, double* w) {
-  for (int j = nref - 1; j >= 0; --j) {
+    team_sync<kLanes>();
     const double t = tau[j];
-    if (t == 0.0) continue;
-    const double* v = V + (size_t)j * ldv;
-    house_dots(v, j, m, Y, ldy, 0, r, w, red);
-    const int len = m - j;
-    for (int e = threadIdx.x; e < len * r; e += kBlock) {
-      const int i = j + e % len, c = e / len;
-      const double vi = (i == j) ? 1.0 : v[i];
-      Y[i + (size_t)c * ldy] -= t * vi * w[c];
+    if (t != 0.0) {
+      int c = j + 1 + ((grp - (j + 1) % ngrp) + ngrp) % ngrp;
+      for (; c < k; c += ngrp) {
+        double* a = A + (size_t)c * lda;
+        double d = strided_dot(v, a, j + 1 + sub, m, g);
+        d = group_sum(d, g) + a[j];
+        const double f = t * d;
+        strided_axpy(a, f, v, j + 1 + sub, m, g);
+        if (sub == 0) a[j] -= f;
+      }
     }
-    __syncthreads();
+    team_sync<kLanes>();
   }
+  if (tid == 0)
+    for (int j = steps; j < k; ++j) tau[j] = 0.0;
+  team_sync<kLanes>();
 }
 
-// Sum over an aligned group of g lanes (g a power of two <= 64).
-template <typename T>
-__device__ inline T group_sum(T v, int g) {
-  for (int o = g >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+// Y (m x r, ldy) <- Q Y with Q = H_0 ... H_{nref-1} from team_house_qr;
+// column c of Y belongs to lane group c mod (kLanes / g).
+template <int kLanes>
+__device__ inline void team_apply_q(const double* V, int ldv, int m, int nref, const double* tau,
+                                    double* Y, int ldy, int r) {
+  const int tid = team_tid<kLanes>();
+  const int g = group_lanes(r, kLanes), ngrp = kLanes / g, grp = tid / g, sub = tid % g;
+  for (int c = grp; c < r; c += ngrp) {
+    double* y = Y + (size_t)c * ldy;
+    for (int j = nref - 1; j >= 0; --j) {
+      const double t = tau[j];
+      if (t == 0.0) continue;
+      const double* v = V + (size_t)j * ldv;
+      double d = strided_dot(v, y, j + 1 + sub, m, g);
+      d = group_sum(d, g) + y[j];
+      const double f = t * d;
+      strided_axpy(y, f, v, j + 1 + sub, m, g);
+      if (sub == 0) y[j] -= f;
+      if (g > 1) __builtin_amdgcn_wave_barrier();
+    }
+  }
+  team_sync<kLanes>();
 }
 
-// One-sided Jacobi SVD of G (rows x cols, ldg): on exit G = U * Sigma
+// Triangular factor T (k x k, ld k, upper) of the block reflector
+// H_0 H_1 ... H_{k-1} = I - V T V^T of team_house_qr (LAPACK dlarft, forward,
+// columnwise): S = V^T V (strict upper part, into S, k x k), then
+// T[l, j] = -tau_j sum_{r=l}^{j-1} T[l, r] S[r, j], T[j, j] = tau_j.
+// The recurrence runs on the team's first wavefront; row l of T is written
+// by lane l mod 64 only, so its steps need no cross-lane exchange.
+template <int kLanes>
+__device__ inline void team_larft(const double* V, int ldv, int m, int k, const double* tau,
+                                  double* S, double* T) {
+  const int tid = team_tid<kLanes>();
+  const int npairs = k * (k - 1) / 2;
+  if (npairs > 0) {
+    const int g = group_lanes(npairs, kLanes), ngrp = kLanes / g, grp = tid / g, sub = tid % g;
+    for (int p = grp; p < npairs; p += ngrp) {
+      int j = (int)((1.0 + sqrt(1.0 + 8.0 * p)) * 0.5);
+      while (j * (j - 1) / 2 > p) --j;
+      while ((j + 1) * j / 2 <= p) ++j;
+      const int l = p - j * (j - 1) / 2;
+      const double* vl = V + (size_t)l * ldv;
+      const double* vj = V + (size_t)j * ldv;
+      double d = strided_dot(vl, vj, j + 1 + sub, m, g);
+      d = group_sum(d, g) + vl[j];
+      if (sub == 0) S[l + (size_t)j * k] = d;
+    }
+  }
+  team_sync<kLanes>();
+  if (tid < kWave) {
+    for (int j = 0; j < k; ++j) {
+      const double tj = tau[j];
+      for (int l = tid; l < j; l += kWave) {
+        double a = 0.0;
+        for (int r = l; r < j; ++r) a += T[l + (size_t)r * k] * S[r + (size_t)j * k];
+        T[l + (size_t)j * k] = -tj * a;
+      }
+      if (tid == (j & (kWave - 1))) T[j + (size_t)j * k] = tj;
+    }
+  }
+  team_sync<kLanes>();
+}
+
+// Workgroup-wide wrappers (the workgroup's own reduction scratch is unused).
+__device__ inline void house_qr(double* A, int lda, int m, int k, double* tau, double* = nullptr,
+                                double* = nullptr) {
+  team_house_qr<kBlock>(A, lda, m, k, tau);
+}
+__device__ inline void apply_q(const double* V, int ldv, int m, int nref, const double* tau,
+                               double* Y, int ldy, int r, double* = nullptr, double* = nullptr) {
+  team_apply_q<kBlock>(V, ldv, m, nref, tau, Y, ldy, r);
+}
+// Single-wavefront versions (operands in LDS).
+__device__ inline void wave_house_qr(double* A, int lda, int m, int k, double* tau) {
+  team_house_qr<kWave>(A, lda, m, k, tau);
+}
+__device__ inline void wave_apply_q(const double* V, int ldv, int m, int nref, const double* tau,
+                                    double* Y, int ldy, int r) {
+  team_apply_q<kWave>(V, ldv, m, nref, tau, Y, ldy, r);
+}
+
+// One-sided (Hestenes) Jacobi SVD of G (rows x cols, ldg): on exit G = U Sigma
 // (columns mutually orthogonal, unsorted) and V (cols x cols, ldv) holds the
-// accumulated rotations so that G_in = (U Sigma) V^T.  'flag' is a shared int.
-// Column pairs of one round (round-robin ordering) are independent; each is
-// handled by a group of g lanes sized to the column length, so that a
-// wavefront works on 64 / g pairs at once.
-__device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
-                                  int* flag, int max_sweeps = 40) {
-  for (int e = threadIdx.x; e < cols * cols; e += kBlock) {
+// accumulated rotations so that G_in = (U Sigma) V^T.  Round-robin ordering:
+// the cols/2 pairs of a round are independent and each belongs to one lane
+// group.  A pair is rotated while |g| > 1e-13 sqrt(a b) and
+// |g| > 1e-30 ||G||_F^2 (columns at rounding level of ||G|| carry nothing for
+// the truncation thresholds used here).  Returns the number of sweeps.
+// 'sh2' is 2 shared doubles of scratch (workgroup team only).
+template <int kLanes>
+__device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double* V, int ldv,
+                                  double* sh2, int max_sweeps) {
+  const int tid = team_tid<kLanes>();
+  for (int e = tid; e < cols * cols; e += kLanes) {
     const int i = e % cols, j = e / cols;
     V[i + (size_t)j * ldv] = (i == j) ? 1.0 : 0.0;
   }
-  __syncthreads();
-  if (cols < 2) return;
-  const int longest = rows > cols ? rows : cols;
-  int g = 8;
-  while (g < 64 && g * 4 < longest) g <<= 1;
-  const int gl = threadIdx.x % g, grp = threadIdx.x / g, ngrp = kBlock / g;
-  const int np = (cols + 1) & ~1;  // players, padded to even
-  const int rounds = np - 1, half = np / 2;
-  const double tol = 1e-15;
-  // ||G||_F^2: columns below 1e-15 ||G|| carry no information for any
-  // truncation threshold used here; pairs of them are not rotated (a purely
-  // relative test never settles on such noise-level columns)
-  __shared__ double fro_s;
-  {
-    double f = 0.0;
-    for (int e = threadIdx.x; e < rows * cols; e += kBlock) {
-      const double x = G[e % rows + (size_t)(e / rows) * ldg];
-      f += x * x;
-    }
-    f = wave_sum(f);
-    if (threadIdx.x == 0) fro_s = 0.0;
-    __syncthreads();
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&fro_s, f);
-    __syncthreads();
+  double f = 0.0;
+  for (int e = tid; e < rows * cols; e += kLanes) {
+    const double x = G[e % rows + (size_t)(e / rows) * ldg];
+    f += x * x;
   }
-  const double floor2 = 1e-30 * fro_s;
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    if (threadIdx.x == 0) *flag = 0;
+  f = wave_sum(f);
+  if constexpr (kLanes != kWave) {
+    if (threadIdx.x == 0) sh2[0] = 0.0;
     __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&sh2[0], f);
+    __syncthreads();
+    f = sh2[0];
+  }
+  team_sync<kLanes>();
+  if (cols < 2) return 0;
+  const double floor2 = 1e-30 * f, tol = 1e-13;
+  const int np = (cols + 1) & ~1, rounds = np - 1, half = np / 2;
+  const int g = group_lanes(half, kLanes), ngrp = kLanes / g, grp = tid / g, sub = tid % g;
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    int rotated = 0;
     for (int rd = 0; rd < rounds; ++rd) {
-      for (int pk0 = 0; pk0 < half; pk0 += ngrp) {
-        const int pk = pk0 + grp;
-        int p = -1, q = -1;
-        if (pk < half) {
-          if (pk == 0) {
-            p = np - 1;
-            q = rd;
-          } else {
-            p = (rd + pk) % (np - 1);
-            q = (rd - pk + (np - 1)) % (np - 1);
-          }
-          if (p > q) {
-            const int tmp = p;
-            p = q;
-            q = tmp;
-          }
+      for (int pk = grp; pk < half; pk += ngrp) {
+        int p, q;
+        if (pk == 0) {
+          p = np - 1;
+          q = rd;
+        } else {
+          p = (rd + pk) % (np - 1);
+          q = (rd - pk + (np - 1)) % (np - 1);
         }
-        const bool active = pk < half && q < cols;
+        if (p > q) {
+          const int tmp = p;
+          p = q;
+          q = tmp;
+        }
+        if (q >= cols) continue;
+        double* gp = G + (size_t)p * ldg;
+        double* gq = G + (size_t)q * ldg;
         double a = 0.0, b = 0.0, c2 = 0.0;
-        double* gp = G + (size_t)(active ? p : 0) * ldg;
-        double* gq = G + (size_t)(active ? q : 0) * ldg;
-        if (active)
-          for (int i = gl; i < rows; i += g) {
-            const double x = gp[i], y = gq[i];
-            a += x * x;
-            b += y * y;
-            c2 += x * y;
-          }
-        // every lane of the wavefront takes part in the shuffles
+        for (int i = sub; i < rows; i += g) {
+          const double x = gp[i], y = gq[i];
+          a += x * x;
+          b += y * y;
+          c2 += x * y;
+        }
         a = group_sum(a, g);
         b = group_sum(b, g);
         c2 = group_sum(c2, g);
-        if (!active || c2 == 0.0 || !(fabs(c2) > tol * sqrt(a * b)) || !(fabs(c2) > floor2)) continue;
+        if (!(fabs(c2) > tol * sqrt(a * b)) || !(fabs(c2) > floor2)) continue;
         const double zeta = (b - a) / (2.0 * c2);
         const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-        for (int i = gl; i < rows; i += g) {
+        for (int i = sub; i < rows; i += g) {
           const double x = gp[i], y = gq[i];
           gp[i] = c * x - s * y;
           gq[i] = s * x + c * y;
         }
         double* vp = V + (size_t)p * ldv;
         double* vq = V + (size_t)q * ldv;
-        for (int i = gl; i < cols; i += g) {
+        for (int i = sub; i < cols; i += g) {
           const double x = vp[i], y = vq[i];
           vp[i] = c * x - s * y;
           vq[i] = s * x + c * y;
         }
-        if (gl == 0) *flag = 1;
+        rotated = 1;
       }
+      team_sync<kLanes>();
+    }
+    int any;
+    if constexpr (kLanes == kWave) {
+      any = __ballot(rotated) != 0;
+    } else {
+      if (threadIdx.x == 0) sh2[1] = 0.0;
+      __syncthreads();
+      if (rotated) sh2[1] = 1.0;
+      __syncthreads();
+      any = sh2[1] != 0.0;
       __syncthreads();
     }
-    if (*flag == 0) break;
-    __syncthreads();
+    if (!any) {
+      ++sweep;
+      break;
+    }
   }
-  __syncthreads();
+  team_sync<kLanes>();
+  return sweep;
+}
+
+// Workgroup version used by the existing call sites.
+__device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
+                                  int* /*flag*/, int max_sweeps = 40) {
+  __shared__ double sh2[2];
+  team_jacobi<kBlock>(G, ldg, rows, cols, V, ldv, sh2, max_sweeps);
+}
+
+// Single-wavefront version (G, V in LDS).
+__device__ inline int wave_jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
+                                      int max_sweeps = 40) {
+  return team_jacobi<kWave>(G, ldg, rows, cols, V, ldv, nullptr, max_sweeps);
 }
 
 // Column norms of G into sig[0..cols), then order[] = permutation sorting

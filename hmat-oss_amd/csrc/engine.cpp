@@ -28,7 +28,11 @@ void DevBuf::ensure(size_t nbytes) {
 
 void upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
   b.ensure(bytes);
-  if (bytes) HMX_CHECK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+  if (!bytes) return;
+  // the source is ordinary (pageable) host memory that callers reuse or free
+  // right after: wait until the copy has read it
+  HMX_CHECK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+  HMX_CHECK(hipStreamSynchronize(s));
 }
 
 template <>
@@ -138,7 +142,7 @@ void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
   if (cols != rows) upload_points(k, *cols, pts_cols, user_cols, stream);
   PointSet rp{pts_rows.as<double>(), user_rows.as<int>(), rows->n};
   PointSet cp = (cols == rows) ? rp : PointSet{pts_cols.as<double>(), user_cols.as<int>(), cols->n};
-  KernelParams kp{k.kind, k.length, k.nugget};
+  KernelParams kp{k.kind, k.length, k.nugget, k.shift};
   init_store(s64);
   HMX_CHECK(hipMemsetAsync(overflow_.p, 0, sizeof(int), stream));
 

@@ -55,6 +55,7 @@ struct KernelSpec {
   int kind = kExponential;
   double length = 1.0;
   double nugget = 0.0;
+  double shift = 0.0;
   int dim = 3;
   int n = 0;
   std::vector<double> points;  // user numbering
@@ -97,6 +98,7 @@ class Matrix {
   bool assembled = false;
   int factorization = -1;    // hmat_factorization_t
   bool factor32 = false;      // factor stored in s32 (mixed precision)
+  DevBuf pivots;              // LU: getrf interchanges of each diagonal leaf, at its rows
   int overflow = 0;
   Stats stats;
 

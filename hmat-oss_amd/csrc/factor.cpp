@@ -100,19 +100,67 @@ struct FactorState {
   int64_t next_temp = 0;
   // cached solve plans, keyed by (precision, nrhs), and their right-hand side
   std::map<std::pair<int, int>, PlanList> solve_plans;
-  DevBuf solve_rhs, ir_x, ir_r, ir_norms;
+  // replays of a recorded solve plan captured as one HIP graph; the captured
+  // kernel arguments hold the buffer addresses below, so a graph is rebuilt
+  // when any of them has moved
+  struct SolveGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *ws = nullptr, *rhs = nullptr, *ranks = nullptr, *flags = nullptr;
+    void release() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+    }
+  };
+  std::map<std::pair<int, int>, SolveGraph> solve_graphs;
+  // right-hand sides of each recorded solve plan: the plan's tasks hold this
+  // buffer's address, so every (precision, nrhs) keeps its own
+  std::map<std::pair<int, int>, DevBuf> solve_rhs;
+  DevBuf ir_x, ir_r, ir_norms;
+  void drop_graphs() {
+    for (auto& g : solve_graphs) g.second.release();
+    solve_graphs.clear();
+  }
+  ~FactorState() { drop_graphs(); }
 };
 
 namespace {
 template <typename T>
-void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s) {
+void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool capturing = false) {
   fs.ws.ensure((size_t)std::max<int64_t>(P.ws, 1) * sizeof(double));
-  if (P.ranks_used) HMX_CHECK(hipMemsetAsync(fs.ranks.p, 0, P.ranks_used * sizeof(int), s));
+  // a fill kernel rather than a memset node, so that the plan can be captured
+  if (P.ranks_used) launch_fill_int(fs.ranks.as<int>(), (size_t)P.ranks_used, 0, s);
   const FTask* dt = P.tasks.as<FTask>();
   const GemmTile* dtl = P.tiles.as<GemmTile>();
-  for (const SegLaunch& L : P.launches)
+  // HMX_TRACE_LAUNCHES=<seconds>: time every launch with events and report
+  // the launches slower than that on stderr (diagnostics; not for capture)
+  static const double trace_env = getenv("HMX_TRACE_LAUNCHES") ? atof(getenv("HMX_TRACE_LAUNCHES")) : -1.0;
+  const double trace = capturing ? -1.0 : trace_env;
+  std::vector<hipEvent_t> ev;
+  if (trace >= 0) {
+    ev.resize(P.launches.size() + 1);
+    for (auto& e : ev) HMX_CHECK(hipEventCreate(&e));
+    HMX_CHECK(hipEventRecord(ev[0], s));
+  }
+  for (size_t q = 0; q < P.launches.size(); ++q) {
+    const SegLaunch& L = P.launches[q];
     launch_task_kernel<T>(L.type, dt + L.first, L.count, dtl + L.tfirst, L.tcount,
                           fs.ws.as<double>(), fs.flags.as<int>(), s);
+    if (trace >= 0) HMX_CHECK(hipEventRecord(ev[q + 1], s));
+  }
+  if (trace >= 0) {
+    HMX_CHECK(hipStreamSynchronize(s));
+    for (size_t q = 0; q < P.launches.size(); ++q) {
+      float ms = 0.f;
+      HMX_CHECK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+      const SegLaunch& L = P.launches[q];
+      if (ms * 1e-3 > trace)
+        fprintf(stderr, "launch %zu type %d tasks %d tiles %d: %.4f s\n", q, L.type, L.count, L.tcount, ms * 1e-3);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
 }
 }  // namespace
 
@@ -196,7 +244,9 @@ class Exec {
                                                              : fs_.chunks.back()->bytes * 2);
       if (fs_.temp_total + want <= fs_.temp_budget || fs_.chunks.empty()) {
         auto b = std::make_unique<DevBuf>();
+        const double ta0 = now_s();
         b->alloc(want);
+        t_alloc_ += now_s() - ta0;
         fs_.temp_total += want;
         fs_.chunks.push_back(std::move(b));
         continue;
@@ -204,7 +254,9 @@ class Exec {
       flush(true);  // throws if temporaries are live
       if (bytes > fs_.chunks.back()->bytes) {
         auto b = std::make_unique<DevBuf>();
+        const double ta1 = now_s();
         b->alloc(bytes);
+        t_alloc_ += now_s() - ta1;
         fs_.temp_total += bytes;
         fs_.chunks.push_back(std::move(b));
       }
@@ -250,11 +302,17 @@ class Exec {
       out.push_back((uint64_t)buf << 20 | 0xFFFFF);
       return;
     }
+    if (m <= 0) return;
+    if (row0 + m > (int)row2leaf_.size())
+      throw std::runtime_error("dependency keys: rows [" + std::to_string(row0) + ", " +
+                               std::to_string(row0 + m) + ") outside the matrix");
     const int q0 = row2leaf_[row0], q1 = row2leaf_[row0 + m - 1];
     for (int q = q0; q <= q1; ++q) out.push_back((uint64_t)buf << 20 | (uint64_t)q);
   }
-  void rd(const DV& d) { keys(d.buf, d.row0, d.trans ? d.n : d.m, r_); }
-  void wr(const DV& d) { keys(d.buf, d.row0, d.trans ? d.n : d.m, w_); }
+  // row0 numbers the storage rows of a temporary (a transposed view selects
+  // storage columns and keeps row0), so the keys always span the storage rows
+  void rd(const DV& d) { keys(d.buf, d.row0, d.m, r_); }
+  void wr(const DV& d) { keys(d.buf, d.row0, d.m, w_); }
   void rd(const RV& r) {
     keys(r.bufa, r.row0, r.m, r_);
     keys(r.bufb, r.col0, r.n, r_);
@@ -265,7 +323,9 @@ class Exec {
   }
 
   void emit(FTask& t, int64_t ws_doubles = 0) {
-    int wave = 1;
+    // HMX_SERIAL_TASKS: one task per wave (diagnostics: rules out scheduling races)
+    static const bool serial = getenv("HMX_SERIAL_TASKS") != nullptr;
+    int wave = serial ? max_wave_ + 1 : 1;
     for (uint64_t k : r_) {
       auto it = units_.find(k);
       if (it != units_.end()) wave = std::max(wave, it->second.lw + 1);
@@ -343,7 +403,28 @@ class Exec {
     emit(t);
   }
 
-  // mode 0: X <- X L^{-T}; mode 1: X <- L^{-1} X; mode 2: X <- L^{-T} X.
+  // HMatrix::luDecomposition at a leaf (full_matrix.cpp:230-237): getrf in
+  // place, row interchanges into the matrix's pivot array at the leaf's rows.
+  void getrf(const Block* b) {
+    DV d = full_of(b);
+    FTask t{};
+    t.type = T_GETRF;
+    t.m = dim(d.m);
+    t.A = d.p;
+    t.lda = d.ld;
+    t.B = piv_of(b);
+    rd(d);
+    wr(d);
+    emit(t);
+  }
+
+  int* piv_of(const Block* b) const {
+    if (!M_.pivots.p) throw std::runtime_error("LU pivots not allocated");
+    return M_.pivots.as<int>() + b->rows->offset;
+  }
+
+  // mode 0: X <- X L^{-T}; mode 1: X <- L^{-1} X; mode 2: X <- L^{-T} X, with
+  // the kTrsm* flags of ftask.hpp for the LU leaves.
   // 'other' is the free dimension of X (rows for mode 0, columns otherwise).
   void trsm(int mode, const Block* Lb, const DV& X, Dim other) {
     DV L = full_of(Lb);
@@ -354,6 +435,7 @@ class Exec {
     t.m = other;
     t.A = L.p;
     t.lda = L.ld;
+    if (mode & kTrsmPiv) t.B = piv_of(Lb);
     t.C = X.p;
     t.ldc = X.ld;
     rd(L);
@@ -941,6 +1023,183 @@ class Exec {
     }
   }
 
+  // ---- H-LU (non-symmetric storage) ------------------------------------------------
+  // HMatrix::solveLowerTriangularLeft(ScalarArray) h_matrix.cpp:2338-2372 with
+  // LU, UNIT, LOWER: leaves apply their interchanges then a unit lower solve.
+  void lu_solve_lower_left(const Block* L, const DV& X, Dim ncol) {
+    if (L->is_leaf()) {
+      trsm(1 | kTrsmUnit | kTrsmPiv, L, X, ncol);
+      return;
+    }
+    int off = 0;
+    std::vector<std::pair<int, int>> subs;
+    for (int i = 0; i < L->nrow; ++i) {
+      const Block* d = L->get(i, i);
+      subs.push_back({off, d->n()});
+      for (int j = 0; j < i; ++j) {
+        const Block* lij = L->get(i, j);
+        if (lij)
+          hgemv(lij, 'N', -1.0, rows_of(X, subs[j].first, subs[j].second),
+                rows_of(X, subs[i].first, subs[i].second), ncol);
+      }
+      lu_solve_lower_left(d, rows_of(X, off, d->n()), ncol);
+      off += d->n();
+    }
+  }
+
+  // HMatrix::solveUpperTriangularLeft(ScalarArray) h_matrix.cpp:2504-2540
+  // with Uplo UPPER: X <- U^{-1} X (u_ji = get(j, i), applied 'N').
+  void lu_solve_upper_left(const Block* U, const DV& X, Dim ncol) {
+    if (U->is_leaf()) {
+      trsm(2 | kTrsmTransA, U, X, ncol);
+      return;
+    }
+    std::vector<std::pair<int, int>> subs;
+    int off = 0;
+    for (int i = 0; i < U->nrow; ++i) {
+      subs.push_back({off, U->get(i, i)->n()});
+      off += U->get(i, i)->n();
+    }
+    for (int i = U->nrow - 1; i >= 0; --i) {
+      lu_solve_upper_left(U->get(i, i), rows_of(X, subs[i].first, subs[i].second), ncol);
+      for (int j = 0; j < i; ++j) {
+        const Block* uji = U->get(j, i);
+        if (uji)
+          hgemv(uji, 'N', -1.0, rows_of(X, subs[i].first, subs[i].second),
+                rows_of(X, subs[j].first, subs[j].second), ncol);
+      }
+    }
+  }
+
+  // solveLowerTriangularLeft(ScalarArray) with Uplo UPPER (no interchanges):
+  // X <- U^{-T} X (u_ji = get(j, i), applied 'T').
+  void lu_solve_upper_t_left(const Block* U, const DV& X, Dim ncol) {
+    if (U->is_leaf()) {
+      trsm(1 | kTrsmTransA, U, X, ncol);
+      return;
+    }
+    int off = 0;
+    std::vector<std::pair<int, int>> subs;
+    for (int i = 0; i < U->nrow; ++i) {
+      const Block* d = U->get(i, i);
+      subs.push_back({off, d->n()});
+      for (int j = 0; j < i; ++j) {
+        const Block* uji = U->get(j, i);
+        if (uji)
+          hgemv(uji, 'T', -1.0, rows_of(X, subs[j].first, subs[j].second),
+                rows_of(X, subs[i].first, subs[i].second), ncol);
+      }
+      lu_solve_upper_t_left(d, rows_of(X, off, d->n()), ncol);
+      off += d->n();
+    }
+  }
+
+  // HMatrix::solveLowerTriangularLeft(HMatrix) h_matrix.cpp:2297-2336 +
+  // recursiveSolveLowerTriangularLeft recursion.cpp:191-233 (LU, UNIT, LOWER)
+  void lu_solve_lower_left_h(const Block* L, const Block* b) {
+    if (!b) return;
+    if (!L->is_leaf() && !b->is_leaf()) {
+      if (L->ncol == b->nrow) {
+        for (int k = 0; k < b->ncol; ++k)
+          for (int i = 0; i < L->nrow; ++i) {
+            const Block* bik = b->get(i, k);
+            if (!bik) continue;
+            for (int j = 0; j < i; ++j) {
+              const Block* lij = L->get(i, j);
+              const Block* bjk = b->get(j, k);
+              if (lij && bjk) hgemm(bik, 'N', 'N', -1.0, lij, bjk);
+            }
+            lu_solve_lower_left_h(L->get(i, i), bik);
+          }
+      } else if (L->ncol > 1 && b->nrow == 1 && b->ncol > 1) {
+        for (int k = 0; k < b->ncol; ++k) lu_solve_lower_left_h(L, b->get(0, k));
+      } else {
+        throw std::runtime_error("solveLowerTriangularLeft: case not handled");
+      }
+      return;
+    }
+    if (!b->is_leaf()) {
+      if (b->nrow != 1) throw std::runtime_error("solveLowerTriangularLeft: H right-hand side with a leaf diagonal");
+      for (int k = 0; k < b->ncol; ++k) lu_solve_lower_left_h(L, b->get(0, k));
+      return;
+    }
+    if (b->kind == Kind::Full) {
+      lu_solve_lower_left(L, full_of(b), dim(b->n()));
+    } else {
+      RV r = rk_of(b);
+      lu_solve_lower_left(L, dv_panel(r.a, r.la, r.m, r.cap, r.bufa, r.row0), dim(r.cap, r.rank));
+    }
+  }
+
+  // HMatrix::solveUpperTriangularRight(HMatrix) h_matrix.cpp:2380-2415 +
+  // recursiveSolveUpperTriangularRight recursion.cpp:86-134 (LU, NONUNIT, UPPER)
+  void lu_solve_upper_right_h(const Block* U, const Block* b) {
+    if (!b) return;
+    if (!U->is_leaf() && !b->is_leaf()) {
+      if (U->nrow == b->ncol) {
+        for (int k = 0; k < b->nrow; ++k)
+          for (int i = 0; i < U->nrow; ++i) {
+            const Block* bki = b->get(k, i);
+            if (!bki) continue;
+            for (int j = 0; j < i; ++j) {
+              const Block* uji = U->get(j, i);
+              const Block* bkj = b->get(k, j);
+              if (bkj && uji) hgemm(bki, 'N', 'N', -1.0, bkj, uji);
+            }
+            lu_solve_upper_right_h(U->get(i, i), bki);
+          }
+      } else if (U->nrow > 1 && b->ncol == 1 && b->nrow > 1) {
+        for (int k = 0; k < b->nrow; ++k) lu_solve_upper_right_h(U, b->get(k, 0));
+      } else {
+        throw std::runtime_error("solveUpperTriangularRight: case not handled");
+      }
+      return;
+    }
+    if (!b->is_leaf()) {
+      if (b->ncol != 1) throw std::runtime_error("solveUpperTriangularRight: H right-hand side with a leaf diagonal");
+      for (int k = 0; k < b->nrow; ++k) lu_solve_upper_right_h(U, b->get(k, 0));
+      return;
+    }
+    if (b->kind == Kind::Full) {
+      if (!U->is_leaf()) throw std::runtime_error("solveUpperTriangularRight: full block with an H diagonal");
+      trsm(0 | kTrsmTransA, U, full_of(b), dim(b->m()));
+    } else {
+      // Xb^T U = Bb^T  <=>  U^T Xb = Bb (solve on the B panel only)
+      RV r = rk_of(b);
+      lu_solve_upper_t_left(U, dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0), dim(r.cap, r.rank));
+    }
+  }
+
+  // HMatrix::luDecomposition h_matrix.cpp:2567 + recursiveLuDecomposition
+  // recursion.cpp:238-287
+  void lu(const Block* h) {
+    if (h->is_leaf()) {
+      if (h->kind != Kind::Full) throw std::runtime_error("lu: Rk diagonal block");
+      getrf(h);
+      return;
+    }
+    if (h->nrow != h->ncol) throw std::runtime_error("lu: non-square block grid");
+    const int n = h->nrow;
+    for (int k = 0; k < n; ++k) {
+      const Block* hkk = h->get(k, k);
+      if (!hkk) continue;
+      lu(hkk);
+      for (int i = k + 1; i < n; ++i)
+        if (h->get(k, i)) lu_solve_lower_left_h(hkk, h->get(k, i));
+      for (int i = k + 1; i < n; ++i)
+        if (h->get(i, k)) lu_solve_upper_right_h(hkk, h->get(i, k));
+      maybe_flush();
+      for (int i = k + 1; i < n; ++i) {
+        if (!h->get(i, k)) continue;
+        for (int j = k + 1; j < n; ++j)
+          if (h->get(i, j) && h->get(k, j)) {
+            hgemm(h->get(i, j), 'N', 'N', -1.0, h->get(i, k), h->get(k, j));
+            maybe_flush();
+          }
+      }
+    }
+  }
+
   // HMatrix::lltDecomposition h_matrix.cpp:2547-2564 + recursiveLltDecomposition
   void llt(const Block* h) {
     if (h->is_leaf()) {
@@ -987,6 +1246,12 @@ class Exec {
   }
 
   void run_segment() {
+    const double tseg0 = now_s();
+    struct SegTimer {
+      double& acc;
+      double t0;
+      ~SegTimer() { acc += now_s() - t0; }
+    } seg_timer{t_seg_, tseg0};
     hipStream_t s = M_.stream;
     const int W = max_wave_;
     auto plan = std::make_shared<SegmentPlan>();
@@ -1028,8 +1293,12 @@ class Exec {
       launches.push_back(L);
     }
     upload(plan->tasks, sorted.data(), n * sizeof(FTask), s);
-    upload(plan->tiles, tiles.data(), std::max<size_t>(tiles.size(), 1) * sizeof(GemmTile), s);
-    if (!tiles.size()) HMX_CHECK(hipMemsetAsync(plan->tiles.p, 0, sizeof(GemmTile), s));
+    if (!tiles.empty()) {
+      upload(plan->tiles, tiles.data(), tiles.size() * sizeof(GemmTile), s);
+    } else {
+      plan->tiles.ensure(sizeof(GemmTile));
+      HMX_CHECK(hipMemsetAsync(plan->tiles.p, 0, sizeof(GemmTile), s));
+    }
     plan->ws = ws_max;
     plan->ranks_used = fs_.rank_used;
     plan->ntasks = (long)n;
@@ -1044,8 +1313,13 @@ class Exec {
       HMX_CHECK(hipMemsetAsync(pbuf.p, 0, n * 8 * sizeof(double), s));
       g_fadd_prof = pbuf.as<double>();
     }
+    static const bool trace_seg = getenv("HMX_TRACE_LAUNCHES") != nullptr;
+    const double te0 = trace_seg ? now_s() : 0.0;
     execute_plan<T>(*plan, fs_, s);
     HMX_CHECK(hipStreamSynchronize(s));
+    if (trace_seg)
+      fprintf(stderr, "segment: %zu tasks, %zu launches, execute %.4f s\n", n, plan->launches.size(),
+              now_s() - te0);
     if (prof) {
       g_fadd_prof = nullptr;
       std::vector<double> h(n * 8);
@@ -1054,7 +1328,7 @@ class Exec {
       if (f) {
         for (size_t q = 0; q < n; ++q)
           if (sorted[q].type == T_FADD)
-            fprintf(f, "%d %g %g %g %g %g %g %g %g\n", waves_[order[q]], h[8 * q], h[8 * q + 1], h[8 * q + 2],
+            fprintf(f, "%d %.12g %.12g %.12g %.12g %.12g %.12g %.12g %.12g\n", waves_[order[q]], h[8 * q], h[8 * q + 1], h[8 * q + 2],
                     h[8 * q + 3], h[8 * q + 4], h[8 * q + 5], h[8 * q + 6], h[8 * q + 7]);
         fclose(f);
       }
@@ -1070,12 +1344,19 @@ class Exec {
   void check_flags() {
     int f[2] = {0, 0};
     HMX_CHECK(hipMemcpy(f, fs_.flags.p, sizeof(f), hipMemcpyDeviceToHost));
+    if (f[1] && lu_mode_) throw std::runtime_error("luDecomposition: getrf found a zero pivot (singular matrix)");
     if (f[1]) throw std::runtime_error("lltDecomposition: matrix is not positive definite");
     if (f[0]) throw std::runtime_error("an Rk rank exceeded the panel capacity during the factorisation");
   }
 
   int live() const { return live_; }
   void record_into(PlanList* p) { record_ = p; }
+  void set_lu(bool v) { lu_mode_ = v; }
+  void report(double total) const {
+    if (getenv("HMX_TRACE_LAUNCHES"))
+      fprintf(stderr, "factorisation: total %.4f s, segments %.4f s, temp allocations %.4f s, %zu units\n",
+              total, t_seg_, t_alloc_, units_.size());
+  }
 
  private:
   static constexpr size_t kMaxTasks = 3000000;
@@ -1092,6 +1373,8 @@ class Exec {
   std::vector<uint64_t> r_, w_;
   int max_wave_ = 0;
   PlanList* record_ = nullptr;
+  bool lu_mode_ = false;
+  double t_alloc_ = 0.0, t_seg_ = 0.0;
 };
 
 // ---------------------------------------------------------------------------
@@ -1100,12 +1383,19 @@ class Exec {
 void Matrix::factorize(int algo, bool fp32) {
   if (!assembled) throw std::runtime_error("factorize: matrix not assembled");
   if (factorization != -1) throw std::runtime_error("factorize: matrix already factorised");
-  if (algo != hmat_factorization_llt) throw std::runtime_error("factorize: only hmat_factorization_llt is provided");
-  if (!lower_sym) throw std::runtime_error("factorize: LLt needs lower-symmetric storage");
+  if (algo != hmat_factorization_llt && algo != hmat_factorization_lu)
+    throw std::runtime_error("factorize: only hmat_factorization_llt and hmat_factorization_lu are provided");
+  if (algo == hmat_factorization_llt && !lower_sym)
+    throw std::runtime_error("factorize: LLt needs lower-symmetric storage");
+  if (algo == hmat_factorization_lu && lower_sym)
+    throw std::runtime_error("factorize: LU needs non-symmetric storage");
+  if (algo == hmat_factorization_lu && rows->n != cols->n)
+    throw std::runtime_error("factorize: LU needs a square matrix");
   device_init();
   const double t0 = now_s();
   if (!fstate_) fstate_ = std::make_shared<FactorState>();
   fstate_->solve_plans.clear();
+  fstate_->drop_graphs();
   stats.tasks = stats.waves = stats.launches = stats.segments = 0;
   if (fp32 || value_type == HMAT_SIMPLE_PRECISION) {
     if (value_type == HMAT_DOUBLE_PRECISION) {
@@ -1116,15 +1406,29 @@ void Matrix::factorize(int algo, bool fp32) {
       s32.valid = true;
     }
     Exec<float> ex(*this, s32, *fstate_);
-    ex.llt(bt->root);
+    ex.set_lu(algo == hmat_factorization_lu);
+    if (algo == hmat_factorization_lu) {
+      pivots.alloc(sizeof(int) * (size_t)std::max(rows->n, 1));
+      ex.lu(bt->root);
+    } else {
+      ex.llt(bt->root);
+    }
     ex.flush(false);
     ex.check_flags();
+    ex.report(now_s() - t0);
     factor32 = true;
   } else {
     Exec<double> ex(*this, s64, *fstate_);
-    ex.llt(bt->root);
+    ex.set_lu(algo == hmat_factorization_lu);
+    if (algo == hmat_factorization_lu) {
+      pivots.alloc(sizeof(int) * (size_t)std::max(rows->n, 1));
+      ex.lu(bt->root);
+    } else {
+      ex.llt(bt->root);
+    }
     ex.flush(false);
     ex.check_flags();
+    ex.report(now_s() - t0);
     factor32 = false;
   }
   factorization = algo;
@@ -1144,31 +1448,65 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st) {
   const auto key = std::make_pair((int)sizeof(T), nrhs);
   const size_t bytes = (size_t)n * nrhs * sizeof(T);
   auto it = fs.solve_plans.find(key);
+  if (it == fs.solve_plans.end()) fs.solve_rhs.erase(key);
+  DevBuf& rhs = fs.solve_rhs[key];
   if (it == fs.solve_plans.end()) {
     // record the forward + backward substitution once for this nrhs
-    fs.solve_rhs.ensure(bytes * 2);
+    fs.solve_rhs[key].ensure(bytes);
     PlanList plans;
     Exec<T> ex(*this, const_cast<Store<T>&>(st), fs);
     ex.record_into(&plans);
     DV X;
-    X.p = fs.solve_rhs.p;
+    X.p = fs.solve_rhs[key].p;
     X.ld = n;
     X.m = n;
     X.n = nrhs;
     X.buf = kTempBase + (int64_t(1) << 30);  // tracked per leaf cluster of rows
     X.row0 = 0;
-    HMX_CHECK(hipMemcpy2DAsync(fs.solve_rhs.p, (size_t)n * sizeof(T), b, (size_t)ldb * sizeof(T),
+    HMX_CHECK(hipMemcpy2DAsync(rhs.p, (size_t)n * sizeof(T), b, (size_t)ldb * sizeof(T),
                                (size_t)n * sizeof(T), nrhs, hipMemcpyDeviceToDevice, stream));
-    ex.solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
-    ex.solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
+    if (factorization == hmat_factorization_lu) {
+      // HMatrix::solve (LU) h_matrix.cpp:2772-2779
+      ex.lu_solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
+      ex.lu_solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
+    } else {
+      // HMatrix::solveLlt h_matrix.cpp:3010-3018
+      ex.solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
+      ex.solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
+    }
     ex.flush(false);
     it = fs.solve_plans.emplace(key, std::move(plans)).first;
   } else {
-    HMX_CHECK(hipMemcpy2DAsync(fs.solve_rhs.p, (size_t)n * sizeof(T), b, (size_t)ldb * sizeof(T),
+    HMX_CHECK(hipMemcpy2DAsync(rhs.p, (size_t)n * sizeof(T), b, (size_t)ldb * sizeof(T),
                                (size_t)n * sizeof(T), nrhs, hipMemcpyDeviceToDevice, stream));
-    for (const auto& p : it->second) execute_plan<T>(*p, fs, stream);
+    // replays run as one HIP graph: the substitution is a long chain of small
+    // batched launches whose cost is mostly launch overhead
+    int64_t wsmax = 1;
+    for (const auto& p : it->second) wsmax = std::max(wsmax, p->ws);
+    fs.ws.ensure((size_t)wsmax * sizeof(double));
+    FactorState::SolveGraph& g = fs.solve_graphs[key];
+    if (!g.exec || g.ws != fs.ws.p || g.rhs != rhs.p || g.ranks != fs.ranks.p ||
+        g.flags != fs.flags.p) {
+      g.release();
+      HMX_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        for (const auto& p : it->second) execute_plan<T>(*p, fs, stream, true);
+      } catch (...) {
+        hipGraph_t broken = nullptr;
+        (void)hipStreamEndCapture(stream, &broken);
+        if (broken) (void)hipGraphDestroy(broken);
+        throw;
+      }
+      HMX_CHECK(hipStreamEndCapture(stream, &g.graph));
+      HMX_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+      g.ws = fs.ws.p;
+      g.rhs = rhs.p;
+      g.ranks = fs.ranks.p;
+      g.flags = fs.flags.p;
+    }
+    HMX_CHECK(hipGraphLaunch(g.exec, stream));
   }
-  HMX_CHECK(hipMemcpy2DAsync(b, (size_t)ldb * sizeof(T), fs.solve_rhs.p, (size_t)n * sizeof(T),
+  HMX_CHECK(hipMemcpy2DAsync(b, (size_t)ldb * sizeof(T), rhs.p, (size_t)n * sizeof(T),
                              (size_t)n * sizeof(T), nrhs, hipMemcpyDeviceToDevice, stream));
   if (!user_stream) HMX_CHECK(hipStreamSynchronize(stream));
   stats.t_solve = now_s() - t0;
@@ -1213,7 +1551,8 @@ void Matrix::solve_host(void* b, int nrhs, bool permute) {
 
 double Matrix::logdet() {
   // HMatrix::logdet h_matrix.cpp:2925-2936: sum of log of the diagonal of L
-  if (factorization == -1) throw std::runtime_error("logdet is only supported for LLt factorized matrices.");
+  if (factorization != hmat_factorization_llt)
+    throw std::runtime_error("logdet is only supported for LLt factorized matrices.");
   double r = 0.0;
   for (const Block* b : bt->leaves) {
     if (b->kind != Kind::Full || b->rows != b->cols) continue;

@@ -17,8 +17,12 @@ int64_t fadd_dense_workspace(int m, int n);
 int64_t rkrk_workspace(int k1, int k2);
 
 // When non-null, k_fadd records 8 doubles per task at g_fadd_prof[8 * task.pad]
-// (path, m, n, K, total, concat, qr, jacobi clocks) for performance analysis.
+// (path, m, n, K, total clocks, set-up, QR, SVD + output; the SVD field
+// carries 1e-3 x the Jacobi sweep count) for performance analysis.
 extern double* g_fadd_prof;
+
+// p[0..n) = v on stream s (a kernel, so it may be captured in a graph).
+void launch_fill_int(int* p, size_t n, int v, hipStream_t s);
 
 // One batched launch of 'count' tasks of 'type' (GEMM: 'ntiles' tiles).
 // flags[0] |= 1 on an Rk capacity overflow, flags[1] |= 1 on a non-positive

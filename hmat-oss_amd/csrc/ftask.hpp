@@ -21,8 +21,19 @@ enum TaskType : int {
   T_ACAFULL,      // dense -> Rk by full-pivot ACA
   T_ZERO,         // C = 0
   T_AXPY,         // C += alpha op(A)
+  T_GETRF,        // A <- P L U in place (L unit lower), row interchanges into B (int*)
   T_NTYPES
 };
+
+// T_TRSM mode: a base operation on a lower-triangular operand L
+//   0: X <- X L^{-T}   1: X <- L^{-1} X   2: X <- L^{-T} X
+// plus flags: kTrsmTransA takes L = A^T (A upper triangular, as U of an LU
+// leaf), kTrsmUnit takes a unit diagonal, kTrsmPiv first applies the row
+// interchanges B (int*, getrf order) to X (left modes).
+constexpr int kTrsmBase = 3;
+constexpr int kTrsmTransA = 4;
+constexpr int kTrsmUnit = 8;
+constexpr int kTrsmPiv = 16;
 
 struct Dim {
   int s;             // static size / capacity
@@ -52,7 +63,7 @@ struct FTask {
   double alpha;
   double eps;
   void* A;
-  void* B;
+  void* B;           // GETRF / TRSM with kTrsmPiv: row interchanges (int*)
   void* C;
   int lda, ldb, ldc;
   int nparts;

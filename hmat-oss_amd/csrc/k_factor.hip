@@ -162,10 +162,94 @@ __global__ void __launch_bounds__(kBlock) k_potrf(const FTask* tasks, int* info)
 }
 
 // ---------------------------------------------------------------------------
+// GETRF of a diagonal leaf (ScalarArray::luDecomposition, scalar_array.cpp:
+// 694-705): A = P L U in place, L unit lower, U upper, partial pivoting as
+// LAPACK getrf (pivot = first row of largest |a_ij| in the column), 0-based
+// interchanges into piv[0..n).  A zero or non-finite pivot sets info (the
+// reference throws on getrf's info > 0).  Right-looking, one column per step.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ void getrf_body(T* P, int ld, int n, int* piv, int* info) {
+  __shared__ int pv;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int j = 0; j < n; ++j) {
+    if (threadIdx.x < kWave) {
+      double best = -1.0;
+      int bi = n;
+      for (int i = j + lane; i < n; i += kWave) {
+        const double v = fabs((double)P[i + (size_t)j * ld]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+      for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, kWave);
+        const int oi = __shfl_xor(bi, o, kWave);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (bi >= n) bi = j;  // column of NaNs: keep the row, the pivot test below fails
+      if (lane == 0) {
+        pv = bi;
+        piv[j] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = pv;
+    if (p != j)
+      for (int c = threadIdx.x; c < n; c += kBlock) {
+        const T a = P[j + (size_t)c * ld];
+        P[j + (size_t)c * ld] = P[p + (size_t)c * ld];
+        P[p + (size_t)c * ld] = a;
+      }
+    __syncthreads();
+    const double d = (double)P[j + (size_t)j * ld];
+    if (d == 0.0 || !isfinite(d)) {
+      if (threadIdx.x == 0) atomicOr(info, 1);
+      return;
+    }
+    for (int i = j + 1 + threadIdx.x; i < n; i += kBlock) P[i + (size_t)j * ld] = (T)((double)P[i + (size_t)j * ld] / d);
+    __syncthreads();
+    const int r = n - j - 1;
+    for (int e = threadIdx.x; e < r * r; e += kBlock) {
+      const int i = j + 1 + e % r, c = j + 1 + e / r;
+      P[i + (size_t)c * ld] -= P[i + (size_t)j * ld] * P[j + (size_t)c * ld];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_getrf(const FTask* tasks, int* info) {
+  __shared__ T lds[kLdsDense * kLdsDense];
+  const FTask& t = tasks[blockIdx.x];
+  const int n = t.m.s;
+  T* A = (T*)t.A;
+  const int lda = t.lda;
+  int* piv = (int*)t.B;
+  if (n <= kLdsDense) {
+    for (int e = threadIdx.x; e < n * n; e += kBlock) lds[e] = A[e % n + (size_t)(e / n) * lda];
+    __syncthreads();
+    getrf_body(lds, n, n, piv, info);
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += kBlock) A[e % n + (size_t)(e / n) * lda] = lds[e];
+  } else {
+    getrf_body(A, lda, n, piv, info);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // TRSM with a lower-triangular leaf L (n x n), L staged in LDS:
 //   mode 0: X (m x n) <- X L^{-T}     (solveUpperTriangularRight, LLt)
 //   mode 1: X (n x k) <- L^{-1} X     (solveLowerTriangularLeft)
 //   mode 2: X (n x k) <- L^{-T} X     (solveUpperTriangularLeft, Uplo LOWER)
+// Flags (ftask.hpp): kTrsmTransA reads L as the transpose of an upper
+// triangle U (so 0 is X U^{-1}, 1 is U^{-T} X, 2 is U^{-1} X: the LU cases of
+// scalar_array.cpp:852-874), kTrsmUnit assumes a unit diagonal, kTrsmPiv
+// applies getrf's row interchanges to X first (laswp, scalar_array.cpp:855).
 // Rows (mode 0) / columns (modes 1, 2) of X are independent: one thread runs
 // the substitution of one of them, reading L from LDS (broadcast) and its own
 // row / column from LDS when it fits, so no barrier is needed inside.
@@ -184,12 +268,17 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
   const int ldl = t.lda;
   T* X = (T*)t.C;
   const int ldx = t.ldc;
+  const int mode = t.mode & kTrsmBase;
+  const bool transa = (t.mode & kTrsmTransA) != 0, unit = (t.mode & kTrsmUnit) != 0;
   const bool lds_l = n * n <= kShDoubles;
   double* Ls = sh;
   if (lds_l)
-    for (int e = threadIdx.x; e < n * n; e += kBlock) Ls[e] = (double)Lg[e % n + (size_t)(e / n) * ldl];
+    for (int e = threadIdx.x; e < n * n; e += kBlock) {
+      const int i = e % n, j = e / n;
+      Ls[e] = transa ? (double)Lg[j + (size_t)i * ldl] : (double)Lg[i + (size_t)j * ldl];
+    }
   // X block staged in LDS behind L when it fits: rows = n for modes 1/2
-  const int xr = t.mode == 0 ? other : n, xc = t.mode == 0 ? n : other;
+  const int xr = mode == 0 ? other : n, xc = mode == 0 ? n : other;
   const int xl = xr | 1;  // odd leading dimension: fewer bank conflicts for column walks
   const bool lds_x = lds_l && n * n + xl * xc <= kShDoubles;
   double* Xs = sh + n * n;
@@ -199,7 +288,11 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
       Xs[i + j * xl] = (double)X[i + (size_t)j * ldx];
     }
   __syncthreads();
-  auto Lat = [&](int i, int j) -> double { return lds_l ? Ls[i + j * n] : (double)Lg[i + (size_t)j * ldl]; };
+  auto Lat = [&](int i, int j) -> double {
+    if (lds_l) return Ls[i + j * n];
+    return transa ? (double)Lg[j + (size_t)i * ldl] : (double)Lg[i + (size_t)j * ldl];
+  };
+  auto Ldiv = [&](double s, int j) -> double { return unit ? s : s / Lat(j, j); };
   auto Xg = [&](int i, int j) -> double { return lds_x ? Xs[i + j * xl] : (double)X[i + (size_t)j * ldx]; };
   auto Xp = [&](int i, int j, double v) {
     if (lds_x)
@@ -207,16 +300,31 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
     else
       X[i + (size_t)j * ldx] = (T)v;
   };
+  if ((t.mode & kTrsmPiv) && mode != 0) {
+    // each thread replays the interchanges on its own columns
+    const int* piv = (const int*)t.B;
+    for (int c = threadIdx.x; c < other; c += kBlock)
+      for (int i = 0; i < n; ++i) {
+        int p = piv[i];
+        p = p < 0 ? 0 : (p >= n ? n - 1 : p);
+        if (p != i) {
+          const double a = Xg(i, c), b = Xg(p, c);
+          Xp(i, c, b);
+          Xp(p, c, a);
+        }
+      }
+    __syncthreads();
+  }
   // Blocked substitution: a 16-wide diagonal block is solved with one thread per
   // independent row / column, then the rest gets a rank-16 update from all threads.
-  if (t.mode == 0) {
+  if (mode == 0) {
     for (int jb = 0; jb < n; jb += kNb) {
       const int je = jb + kNb < n ? jb + kNb : n;
       for (int r = threadIdx.x; r < other; r += kBlock)
         for (int c = jb; c < je; ++c) {
           double s = Xg(r, c);
           for (int l = jb; l < c; ++l) s -= Lat(c, l) * Xg(r, l);
-          Xp(r, c, s / Lat(c, c));
+          Xp(r, c, Ldiv(s, c));
         }
       __syncthreads();
       const int nc = n - je;
@@ -228,14 +336,14 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
       }
       __syncthreads();
     }
-  } else if (t.mode == 1) {
+  } else if (mode == 1) {
     for (int jb = 0; jb < n; jb += kNb) {
       const int je = jb + kNb < n ? jb + kNb : n;
       for (int c = threadIdx.x; c < other; c += kBlock)
         for (int j = jb; j < je; ++j) {
           double s = Xg(j, c);
           for (int l = jb; l < j; ++l) s -= Lat(j, l) * Xg(l, c);
-          Xp(j, c, s / Lat(j, j));
+          Xp(j, c, Ldiv(s, j));
         }
       __syncthreads();
       const int nr = n - je;
@@ -254,7 +362,7 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
         for (int j = je - 1; j >= jb; --j) {
           double s = Xg(j, c);
           for (int l = j + 1; l < je; ++l) s -= Lat(l, j) * Xg(l, c);
-          Xp(j, c, s / Lat(j, j));
+          Xp(j, c, Ldiv(s, j));
         }
       __syncthreads();
       for (int e = threadIdx.x; e < jb * other; e += kBlock) {
@@ -341,80 +449,144 @@ __device__ int dense_to_rk(double* G, int gr, int gc, bool tr, double* V, double
   return k;
 }
 
+// Scratch of the QR recompression (all K x K blocks ld K).
 struct QrScratch {
-  double *taua, *taub, *g, *v, *sig;
-  int* order;
-  double* wq;  // K doubles: Householder dot products
-  double* y;   // max(m, n) x K, reused for A then B
+  double *taua, *taub;  // K each
+  double *g, *v;        // core SVD: G = Ra Rb^T -> U Sigma, and its V
+  double *ta, *tb;      // block-reflector factors of the two panels
+  double *x, *w;        // K x nk work blocks of the back-transformation
+  double *sig, *scale;  // K each
+  int* order;           // K
 };
 
-// Panels Ac (m x K, ld m), Bc (n x K, ld n), K < min(m, n): QR both, SVD of
-// Ra Rb^T, cut at eps, write Qa U sqrt(S), Qb V sqrt(S) to out.
+// doubles of the scratch above
+__device__ inline int64_t qr_core(int K) { return 6 * (int64_t)K * K + 5 * (int64_t)K; }
+
+__device__ inline QrScratch carve_qr(double* core, int K) {
+  const size_t K2 = (size_t)K * K;
+  QrScratch w;
+  w.taua = core;
+  w.taub = core + K;
+  w.sig = core + 2 * K;
+  w.scale = core + 3 * K;
+  w.order = (int*)(core + 4 * K);
+  w.g = core + 5 * K;
+  w.v = w.g + K2;
+  w.ta = w.v + K2;
+  w.tb = w.ta + K2;
+  w.x = w.tb + K2;
+  w.w = w.x + K2;
+  return w;
+}
+
+// Rows of one side of the formatted sum, concatenated: the target's panel
+// followed by every live part's panel placed at its row offset (zero
+// elsewhere); alpha_i scales the A side.  dst is rows x K, ld ld.
 template <typename T>
-__device__ void qr_truncate(double* Ac, int m, double* Bc, int n, int K, const QrScratch& w,
-                            double eps, RkRef out, int* overflow, double* red, int* flag,
-                            double* pr = nullptr) {
-  const long q0 = wall_clock64();
-  house_qr(Ac, m, m, K, w.taua, red, w.wq);
-  house_qr(Bc, n, n, K, w.taub, red, w.wq);
-  const long q1 = wall_clock64();
+__device__ void concat_side(double* dst, int ld, int rows, bool bside, const FTask& t, int kt) {
+  const RkRef& tg = t.tgt;
+  const T* src = (const T*)(bside ? tg.b : tg.a);
+  const int lds = bside ? tg.lb : tg.la;
+  for (int e = threadIdx.x; e < rows * kt; e += kBlock) {
+    const int i = e % rows, c = e / rows;
+    dst[i + (size_t)c * ld] = (double)src[i + (size_t)c * lds];
+  }
+  int off = kt;
+  for (int q = 0; q < t.nparts; ++q) {
+    const Part& p = t.parts[q];
+    const int kq = rank_of(p.rk);
+    if (!(kq > 0 && p.alpha != 0.0 && p.rk.m > 0 && p.rk.n > 0)) continue;
+    const T* P = (const T*)(bside ? p.rk.b : p.rk.a);
+    const int ldp = bside ? p.rk.lb : p.rk.la;
+    const int o = bside ? p.coff : p.roff, pr = bside ? p.rk.n : p.rk.m;
+    const double f = bside ? 1.0 : p.alpha;
+    for (int e = threadIdx.x; e < rows * kq; e += kBlock) {
+      const int i = e % rows, c = e / rows, ii = i - o;
+      dst[i + (size_t)(off + c) * ld] = (ii >= 0 && ii < pr) ? f * (double)P[ii + (size_t)c * ldp] : 0.0;
+    }
+    off += kq;
+  }
+}
+
+// G (K x K) = Ra Rb^T with Ra, Rb the upper triangles of two panels.
+__device__ inline void core_product(const double* Ra, int lra, const double* Rb, int lrb, int K,
+                                    double* G) {
   for (int e = threadIdx.x; e < K * K; e += kBlock) {
     const int i = e % K, j = e / K;
     double s = 0.0;
     const int lo = i > j ? i : j;
-    for (int l = lo; l < K; ++l) s += Ac[i + (size_t)l * m] * Bc[j + (size_t)l * n];
-    w.g[i + (size_t)j * K] = s;
+    for (int l = lo; l < K; ++l) s += Ra[i + (size_t)l * lra] * Rb[j + (size_t)l * lrb];
+    G[i + (size_t)j * K] = s;
+  }
+}
+
+// out (rows x nk, ld ldo) = Q [U; 0] with Q = I - V Tq V^T (rows x K panel of
+// reflectors V, ld ldv) and U[i, c] = src[i, order[c]] * scale[c] (K x nk,
+// ld K): X = V1^T U, W = Tq X, out = [U; 0] - V W.  Every output element is
+// an independent K-term sum.
+template <typename T>
+__device__ void wy_apply_out(const double* V, int ldv, int rows, int K, const double* Tq,
+                             const double* src, const QrScratch& w, int nk, T* out, int ldo) {
+  for (int e = threadIdx.x; e < K * nk; e += kBlock) {
+    const int l = e % K, c = e / K;
+    const double* u = src + (size_t)w.order[c] * K;
+    double s = u[l];
+    for (int i = l + 1; i < K; ++i) s += V[i + (size_t)l * ldv] * u[i];
+    w.x[l + (size_t)c * K] = s * w.scale[c];
   }
   __syncthreads();
-  jacobi_svd(w.g, K, K, K, w.v, K, flag);
-  const long q2 = wall_clock64();
-  if (pr && threadIdx.x == 0) {
-    pr[6] = (double)(q1 - q0);
-    pr[7] = (double)(q2 - q1);
+  for (int e = threadIdx.x; e < K * nk; e += kBlock) {
+    const int l = e % K, c = e / K;
+    double s = 0.0;
+    for (int r = l; r < K; ++r) s += Tq[l + (size_t)r * K] * w.x[r + (size_t)c * K];
+    w.w[l + (size_t)c * K] = s;
   }
+  __syncthreads();
+  for (int e = threadIdx.x; e < rows * nk; e += kBlock) {
+    const int i = e % rows, c = e / rows;
+    const double* wc = w.w + (size_t)c * K;
+    double s = 0.0;
+    const int lmax = i < K ? i : K;
+    for (int l = 0; l < lmax; ++l) s += V[i + (size_t)l * ldv] * wc[l];
+    double y = -s;
+    if (i < K) y += src[i + (size_t)w.order[c] * K] * w.scale[c] - wc[i];
+    out[i + (size_t)c * ldo] = (T)y;
+  }
+  __syncthreads();
+}
+
+// After QR of both panels (reflectors Va, Vb, factors ta, tb) and G = Ra Rb^T
+// in w.g: SVD of G, cut at eps, write Qa U sqrt(S) and Qb V sqrt(S) to out.
+template <typename T>
+__device__ void qr_finish(const double* Va, int lda, int m, const double* Vb, int ldb, int n, int K,
+                          const QrScratch& w, bool wave_core, double eps, RkRef out, int* overflow,
+                          int* flag, int* sweeps_out) {
+  __shared__ int sweeps;
+  if (wave_core) {
+    if (threadIdx.x < kWave) {
+      const int sw = wave_jacobi_svd(w.g, K, K, K, w.v, K);
+      if (threadIdx.x == 0) sweeps = sw;
+    }
+    __syncthreads();
+  } else {
+    jacobi_svd(w.g, K, K, K, w.v, K, flag);
+    if (threadIdx.x == 0) sweeps = 0;
+    __syncthreads();
+  }
+  if (sweeps_out) *sweeps_out = sweeps;
   sorted_norms(w.g, K, K, K, w.sig, w.order);
   int nk = find_k(w.sig, w.order, K, eps);
   if (nk > out.cap) {
     if (threadIdx.x == 0) atomicOr(overflow, 1);
     nk = out.cap;
   }
-  T* A = (T*)out.a;
-  T* B = (T*)out.b;
-  for (int e = threadIdx.x; e < nk * m; e += kBlock) {
-    const int i = e % m, c = e / m, j = w.order[c];
-    w.y[e] = (i < K) ? w.g[i + (size_t)j * K] / sqrt(w.sig[j]) : 0.0;
-  }
+  for (int c = threadIdx.x; c < nk; c += kBlock) w.scale[c] = 1.0 / sqrt(w.sig[w.order[c]]);
   __syncthreads();
-  apply_q(Ac, m, m, K, w.taua, w.y, m, nk, red, w.wq);
-  for (int e = threadIdx.x; e < nk * m; e += kBlock) A[e % m + (size_t)(e / m) * out.la] = (T)w.y[e];
+  wy_apply_out<T>(Va, lda, m, K, w.ta, w.g, w, nk, (T*)out.a, out.la);
+  for (int c = threadIdx.x; c < nk; c += kBlock) w.scale[c] = sqrt(w.sig[w.order[c]]);
   __syncthreads();
-  for (int e = threadIdx.x; e < nk * n; e += kBlock) {
-    const int i = e % n, c = e / n, j = w.order[c];
-    w.y[e] = (i < K) ? w.v[i + (size_t)j * K] * sqrt(w.sig[j]) : 0.0;
-  }
-  __syncthreads();
-  apply_q(Bc, n, n, K, w.taub, w.y, n, nk, red, w.wq);
-  for (int e = threadIdx.x; e < nk * n; e += kBlock) B[e % n + (size_t)(e / n) * out.lb] = (T)w.y[e];
-  __syncthreads();
+  wy_apply_out<T>(Vb, ldb, n, K, w.tb, w.v, w, nk, (T*)out.b, out.lb);
   if (threadIdx.x == 0) *out.rank = nk;
-}
-
-// doubles of the QR path: panels, reusable y, core
-__device__ inline int64_t qr_panels(int m, int n, int K) { return (int64_t)(m + n) * K; }
-__device__ inline int64_t qr_y(int m, int n, int K) { return (int64_t)(m > n ? m : n) * K; }
-__device__ inline int64_t qr_core(int K) { return 2 * (int64_t)K * K + 5 * (int64_t)K; }
-
-__device__ inline QrScratch carve_qr(double* core, double* y, int K) {
-  QrScratch w;
-  w.taua = core;
-  w.taub = core + K;
-  w.g = core + 2 * K;
-  w.v = w.g + (size_t)K * K;
-  w.sig = w.v + (size_t)K * K;
-  w.order = (int*)(w.sig + K);
-  w.wq = w.sig + 2 * K;
-  w.y = y;
-  return w;
 }
 
 // ---------------------------------------------------------------------------
@@ -424,7 +596,6 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsbase, int* overflow,
                                                  double* prof) {
   __shared__ double sh[kShDoubles];
-  __shared__ double red[kWaves * kQrChunk];
   __shared__ int flag;
   if (overflow[1]) return;
   const FTask& t = tasks[blockIdx.x];
@@ -512,51 +683,98 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     }
     return;
   }
-  // QR path: concatenate [target, alpha_i * parts] (zero padded)
-  const int64_t pan = qr_panels(m, n, K), yy = qr_y(m, n, K), core = qr_core(K);
-  double *Ac, *y, *cr;
-  if (pan + yy + core <= kShDoubles) {
-    Ac = sh;
-    y = sh + pan;
-    cr = y + yy;
-  } else if (core <= kShDoubles) {
-    Ac = gws;
-    y = gws + pan;
-    cr = sh;
+  // QR path (truncate, rk_matrix.cpp:490-585) on the concatenated panels
+  // [target, alpha_i parts].  Three placements:
+  //  fast   both panels and the scratch in LDS; wavefront 0 reduces A while
+  //         wavefront 1 reduces B;
+  //  staged one panel at a time in LDS (workgroup QR), reflectors parked in
+  //         the workspace;
+  //  global panels in the workspace, scratch in LDS when it fits.
+  // Odd leading dimensions in LDS keep the strided lane accesses spread over
+  // the banks.
+  const int64_t core = qr_core(K);
+  const int ldAf = m | 1, ldBf = n | 1, ldM = (m > n ? m : n) | 1;
+  int mode;
+  double *Va, *Vb, *cr;
+  int lda, ldb;
+  if ((int64_t)(ldAf + ldBf) * K + core <= kShDoubles) {
+    mode = 5;
+    Va = sh;
+    lda = ldAf;
+    Vb = sh + (size_t)ldAf * K;
+    ldb = ldBf;
+    cr = Vb + (size_t)ldBf * K;
+  } else if ((int64_t)ldM * K + core <= kShDoubles) {
+    mode = 6;
+    Va = gws;
+    lda = m;
+    Vb = gws + (size_t)m * K;
+    ldb = n;
+    cr = sh + (size_t)ldM * K;
   } else {
-    Ac = gws;
-    y = gws + pan;
-    cr = y + yy;
+    Va = gws;
+    lda = m;
+    Vb = gws + (size_t)m * K;
+    ldb = n;
+    cr = core <= kShDoubles ? sh : gws + (size_t)(m + n) * K;
+    mode = cr == sh ? 3 : 4;
   }
-  double* Bc = Ac + (size_t)m * K;
-  for (int e = threadIdx.x; e < m * kt; e += kBlock)
-    Ac[e] = (double)TA[e % m + (size_t)(e / m) * tg.la];
-  for (int e = threadIdx.x; e < n * kt; e += kBlock)
-    Bc[e] = (double)TB[e % n + (size_t)(e / n) * tg.lb];
-  int off = kt;
-  for (int q = 0; q < t.nparts; ++q) {
-    const Part& p = t.parts[q];
-    const int kq = rank_of(p.rk);
-    if (!(kq > 0 && p.alpha != 0.0 && p.rk.m > 0 && p.rk.n > 0)) continue;
-    const T* PA = (const T*)p.rk.a;
-    const T* PB = (const T*)p.rk.b;
-    for (int e = threadIdx.x; e < m * kq; e += kBlock) {
-      const int i = e % m, c = e / m, ii = i - p.roff;
-      Ac[i + (size_t)(off + c) * m] =
-          (ii >= 0 && ii < p.rk.m) ? p.alpha * (double)PA[ii + (size_t)c * p.rk.la] : 0.0;
+  const QrScratch w = carve_qr(cr, K);
+  const long t1 = wall_clock64();
+  if (mode == 5) {
+    concat_side<T>(Va, lda, m, false, t, kt);
+    concat_side<T>(Vb, ldb, n, true, t, kt);
+    __syncthreads();
+    const int wid = threadIdx.x / kWave;
+    if (wid == 0) {
+      wave_house_qr(Va, lda, m, K, w.taua);
+      team_larft<kWave>(Va, lda, m, K, w.taua, w.x, w.ta);
+    } else if (wid == 1) {
+      wave_house_qr(Vb, ldb, n, K, w.taub);
+      team_larft<kWave>(Vb, ldb, n, K, w.taub, w.w, w.tb);
     }
-    for (int e = threadIdx.x; e < n * kq; e += kBlock) {
-      const int i = e % n, c = e / n, ii = i - p.coff;
-      Bc[i + (size_t)(off + c) * n] = (ii >= 0 && ii < p.rk.n) ? (double)PB[ii + (size_t)c * p.rk.lb] : 0.0;
+    __syncthreads();
+    core_product(Va, lda, Vb, ldb, K, w.g);
+  } else if (mode == 6) {
+    double* P = sh;
+    concat_side<T>(P, ldM, m, false, t, kt);
+    __syncthreads();
+    house_qr(P, ldM, m, K, w.taua);
+    team_larft<kBlock>(P, ldM, m, K, w.taua, w.x, w.ta);
+    for (int e = threadIdx.x; e < m * K; e += kBlock) {
+      const int i = e % m, c = e / m;
+      Va[i + (size_t)c * lda] = P[i + (size_t)c * ldM];
+      if (i < K) w.w[i + (size_t)c * K] = P[i + (size_t)c * ldM];  // Ra
     }
-    off += kq;
+    __syncthreads();
+    concat_side<T>(P, ldM, n, true, t, kt);
+    __syncthreads();
+    house_qr(P, ldM, n, K, w.taub);
+    team_larft<kBlock>(P, ldM, n, K, w.taub, w.x, w.tb);
+    for (int e = threadIdx.x; e < n * K; e += kBlock) {
+      const int i = e % n, c = e / n;
+      Vb[i + (size_t)c * ldb] = P[i + (size_t)c * ldM];
+    }
+    core_product(w.w, K, P, ldM, K, w.g);
+  } else {
+    concat_side<T>(Va, lda, m, false, t, kt);
+    concat_side<T>(Vb, ldb, n, true, t, kt);
+    __syncthreads();
+    house_qr(Va, lda, m, K, w.taua);
+    team_larft<kBlock>(Va, lda, m, K, w.taua, w.x, w.ta);
+    house_qr(Vb, ldb, n, K, w.taub);
+    team_larft<kBlock>(Vb, ldb, n, K, w.taub, w.x, w.tb);
+    core_product(Va, lda, Vb, ldb, K, w.g);
   }
   __syncthreads();
-  const long t1 = wall_clock64();
-  qr_truncate<T>(Ac, m, Bc, n, K, carve_qr(cr, y, K), t.eps, tg, overflow, red, &flag, pr);
+  const long t2 = wall_clock64();
+  int sweeps = 0;
+  qr_finish<T>(Va, lda, m, Vb, ldb, n, K, w, cr == sh && K <= 32, fabs(t.eps), tg, overflow, &flag,
+               &sweeps);
   if (pr && threadIdx.x == 0) {
-    pr[0] = 2 + (Ac == sh ? 0 : (cr == sh ? 1 : 2)); pr[1] = m; pr[2] = n; pr[3] = K;
+    pr[0] = mode; pr[1] = m; pr[2] = n; pr[3] = K;
     pr[4] = (double)(wall_clock64() - t0); pr[5] = (double)(t1 - t0);
+    pr[6] = (double)(t2 - t1); pr[7] = (double)(wall_clock64() - t2) + 1e-3 * sweeps;
   }
 }
 
@@ -640,7 +858,12 @@ __global__ void __launch_bounds__(kBlock) k_rkrk(const FTask* tasks, double* wsb
     }
   }
   __syncthreads();
-  jacobi_svd(G, gr, gr, gc, V, gc, &flag);
+  if (need <= kShDoubles && gc <= 32) {
+    if (threadIdx.x < kWave) wave_jacobi_svd(G, gr, gr, gc, V, gc);
+    __syncthreads();
+  } else {
+    jacobi_svd(G, gr, gr, gc, V, gc, &flag);
+  }
   sorted_norms(G, gr, gr, gc, sig, order);
   int nk = find_k(sig, order, gc, t.eps);
   if (nk > out.cap) {
@@ -741,7 +964,7 @@ int64_t fadd_workspace(int m, int n, int K) {
   const int mn = m < n ? m : n;
   int64_t dense = K >= mn ? (int64_t)m * n + (int64_t)mn * mn + 2 * (int64_t)mn : 0;
   const int kq = K < mn ? K : mn - 1;
-  int64_t qr = (int64_t)(m + n) * kq + (int64_t)(m > n ? m : n) * kq + 2 * (int64_t)kq * kq + 5 * (int64_t)kq;
+  int64_t qr = (int64_t)(m + n) * kq + 6 * (int64_t)kq * kq + 5 * (int64_t)kq;
   return (std::max(dense, qr) + 127) / 64 * 64;
 }
 int64_t fadd_dense_workspace(int m, int n) {
@@ -754,6 +977,20 @@ int64_t rkrk_workspace(int k1, int k2) {
 }
 
 double* g_fadd_prof = nullptr;
+
+namespace {
+__global__ void k_fill_int(int* p, size_t n, int v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+}  // namespace
+
+void launch_fill_int(int* p, size_t n, int v, hipStream_t s) {
+  if (n == 0) return;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  HMX_CHECK(hipGetLastError());
+}
 
 template <typename T>
 void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile* tiles,
@@ -786,6 +1023,9 @@ void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile*
       break;
     case T_AXPY:
       hipLaunchKernelGGL(k_axpy<T>, dim3(count), dim3(256), 0, s, tasks, flags + 1);
+      break;
+    case T_GETRF:
+      hipLaunchKernelGGL(k_getrf<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
       break;
     default:
       throw std::runtime_error("unknown task type");

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libhmat_mi355x.so")
 
 HMAT_SIMPLE_PRECISION, HMAT_DOUBLE_PRECISION = 0, 1
 FACT_NONE, FACT_LU, FACT_LDLT, FACT_LLT = -1, 0, 1, 2
-KERNEL_EXPONENTIAL, KERNEL_GAUSSIAN = 0, 1
+KERNEL_EXPONENTIAL, KERNEL_GAUSSIAN, KERNEL_SHIFTED_EXPONENTIAL = 0, 1, 2
 
 _vp = C.c_void_p
 
@@ -54,7 +54,7 @@ class hmat_solve_context_t(C.Structure):
 class hmat_mi355x_kernel_t(C.Structure):
     _fields_ = [("kind", C.c_int), ("dimension", C.c_int), ("n", C.c_int),
                 ("points", C.POINTER(C.c_double)), ("length", C.c_double),
-                ("nugget", C.c_double)]
+                ("nugget", C.c_double), ("shift", C.c_double)]
 
 
 _CREATE = C.CFUNCTYPE(_vp, _vp, _vp, C.c_int, _vp)
@@ -218,12 +218,12 @@ class HMatrix:
         return self.tree.n
 
     def assemble(self, kind=KERNEL_EXPONENTIAL, length=0.5, nugget=1e-2, eps=1e-4,
-                 method="aca_plus", factorization=FACT_NONE):
+                 method="aca_plus", factorization=FACT_NONE, shift=0.0):
         lib = load()
         k = hmat_mi355x_kernel_t()
         k.kind, k.dimension, k.n = kind, self.tree.dim, self.tree.n
         k.points = self.tree.points.ctypes.data_as(C.POINTER(C.c_double))
-        k.length, k.nugget = float(length), float(nugget)
+        k.length, k.nugget, k.shift = float(length), float(nugget), float(shift)
         self._kernel = k
         ctx = hmat_assemble_context_t()
         lib.hmat_assemble_context_init(C.byref(ctx))

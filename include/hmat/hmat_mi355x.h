@@ -31,7 +31,9 @@ HMAT_API void hmat_init_mi355x_interface(hmat_interface_t *i, hmat_value_t type)
 
 typedef enum {
   HMAT_MI355X_KERNEL_EXPONENTIAL = 0, /* exp(-r / length) + nugget * delta_ij */
-  HMAT_MI355X_KERNEL_GAUSSIAN = 1     /* exp(-r^2 / length^2) + nugget * delta_ij */
+  HMAT_MI355X_KERNEL_GAUSSIAN = 1,    /* exp(-r^2 / length^2) + nugget * delta_ij */
+  HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL = 2
+  /* exp(-|x_i - x_j - shift e_z| / length) + nugget * delta_ij: non-symmetric */
 } hmat_mi355x_kernel_kind_t;
 
 typedef struct {
@@ -41,6 +43,7 @@ typedef struct {
   const double *points;  /* n * dimension, user numbering */
   double length;         /* correlation length */
   double nugget;         /* added on the diagonal (user indices equal) */
+  double shift;          /* HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL only */
 } hmat_mi355x_kernel_t;
 
 /* hmat_interaction_func_t for an hmat_mi355x_kernel_t user context (the

@@ -18,6 +18,8 @@ below (paths are relative to the hmat-oss source tree):
 * H-gemm / axpy              h_matrix.cpp:870-1073, 1202-1620, 1874-1950
 * H-LLt and triangular solves recursion.cpp:86-134, 191-233, 349-444,
                               h_matrix.cpp:2297-2564, 3010-3018
+* H-LU                       recursion.cpp:238-287, h_matrix.cpp:2567, 2772-2779,
+                              scalar_array.cpp:694-705, 852-874
 * logdet                     h_matrix.cpp:2925-2936
 * info counters              h_matrix.cpp:464-502
 
@@ -174,7 +176,7 @@ class Rk:
 
 class Block:
     __slots__ = ("rows", "cols", "kind", "children", "nrow", "ncol", "sym",
-                 "full", "rk", "tri_lower", "depth")
+                 "full", "rk", "tri_lower", "depth", "piv")
 
     def __init__(self, rows, cols, depth):
         self.rows = rows
@@ -188,6 +190,7 @@ class Block:
         self.full = None
         self.rk = None
         self.tri_lower = False
+        self.piv = None        # getrf row interchanges of an LU-factorised diagonal leaf
 
     def is_leaf(self):
         return self.kind != NODE
@@ -285,6 +288,23 @@ class ExpKernel:
 
     def block(self, ri, ci):
         d = self.p[ri][:, None, :] - self.p[ci][None, :, :]
+        r = np.sqrt((d * d).sum(axis=2))
+        k = np.exp(-r / self.l)
+        if self.nugget != 0.0:
+            k = k + self.nugget * (ri[:, None] == ci[None, :])
+        return k
+
+
+class ShiftedExpKernel(ExpKernel):
+    """K_ij = exp(-|x_i - x_j - s e_z| / l) + nugget * delta_ij (non-symmetric)."""
+
+    def __init__(self, points, length, nugget, shift):
+        super().__init__(points, length, nugget)
+        self.shift = float(shift)
+
+    def block(self, ri, ci):
+        d = self.p[ri][:, None, :] - self.p[ci][None, :, :]
+        d[:, :, 2] -= self.shift
         r = np.sqrt((d * d).sum(axis=2))
         k = np.exp(-r / self.l)
         if self.nugget != 0.0:
@@ -1233,6 +1253,206 @@ def logdet(L):
     if L.is_leaf():
         return float(np.log(np.prod(np.diag(L.full))))
     return sum(logdet(L.get(i, i)) for i in range(L.nrow))
+
+
+# ---------------------------------------------------------------------------
+# H-LU (non-symmetric storage): recursion.cpp:86-134 (UPPER), 191-233, 238-287,
+# 393-444; h_matrix.cpp:2297-2415, 2504-2540, 2567, 2772-2779;
+# leaves: scalar_array.cpp:694-705 (getrf), 852-874 (laswp + trsm)
+# ---------------------------------------------------------------------------
+def _laswp(piv, x):
+    """LAPACK laswp (forward, incx=1) of the rows of x with getrf pivots."""
+    for i, p in enumerate(piv):
+        if p != i:
+            x[[i, p]] = x[[p, i]]
+
+
+def _subs_diag(h, along_rows=True):
+    out, off = [], 0
+    n = h.nrow if along_rows else h.ncol
+    for i in range(n):
+        d = h.get(i, i)
+        size = d.n if along_rows else d.m
+        out.append((off, size))
+        off += size
+    return out
+
+
+def lu_solve_lower_left_dense(L, x):
+    """solveLowerTriangularLeft(ScalarArray) h_matrix.cpp:2338-2372 with LU,
+    UNIT, LOWER: at a leaf laswp then unit lower trsm (scalar_array.cpp:852-857)."""
+    import scipy.linalg as sla
+    if L.is_leaf():
+        _laswp(L.piv, x)
+        x[:] = sla.solve_triangular(L.full, x, lower=True, unit_diagonal=True)
+        return
+    subs = _subs_diag(L)
+    for i in range(L.nrow):
+        o_i, s_i = subs[i]
+        for j in range(i):
+            lij = L.get(i, j)
+            if lij is not None:
+                o_j, s_j = subs[j]
+                gemv(lij, "N", -1.0, x[o_j:o_j + s_j], 1.0, x[o_i:o_i + s_i])
+        lu_solve_lower_left_dense(L.get(i, i), x[o_i:o_i + s_i])
+
+
+def lu_solve_upper_left_dense(U, x):
+    """solveUpperTriangularLeft(ScalarArray) h_matrix.cpp:2504-2540 with
+    UPPER, NONUNIT: x <- U^{-1} x (u_ji = get(j, i), gemv 'N')."""
+    import scipy.linalg as sla
+    if U.is_leaf():
+        x[:] = sla.solve_triangular(U.full, x, lower=False, trans=0)
+        return
+    subs = _subs_diag(U)
+    for i in range(U.nrow - 1, -1, -1):
+        o_i, s_i = subs[i]
+        lu_solve_upper_left_dense(U.get(i, i), x[o_i:o_i + s_i])
+        for j in range(i):
+            uji = U.get(j, i)
+            if uji is not None:
+                o_j, s_j = subs[j]
+                gemv(uji, "N", -1.0, x[o_i:o_i + s_i], 1.0, x[o_j:o_j + s_j])
+
+
+def lu_solve_upper_t_left_dense(U, x):
+    """solveLowerTriangularLeft(ScalarArray) h_matrix.cpp:2338-2372 with
+    Uplo UPPER (no pivots): x <- U^{-T} x (u_ji = get(j, i), gemv 'T')."""
+    import scipy.linalg as sla
+    if U.is_leaf():
+        x[:] = sla.solve_triangular(U.full, x, lower=False, trans=1)
+        return
+    subs = _subs_diag(U)
+    for i in range(U.nrow):
+        o_i, s_i = subs[i]
+        for j in range(i):
+            uji = U.get(j, i)
+            if uji is not None:
+                o_j, s_j = subs[j]
+                gemv(uji, "T", -1.0, x[o_j:o_j + s_j], 1.0, x[o_i:o_i + s_i])
+        lu_solve_upper_t_left_dense(U.get(i, i), x[o_i:o_i + s_i])
+
+
+def lu_solve_upper_right_dense(U, x):
+    """solveUpperTriangularRight(ScalarArray) h_matrix.cpp:2417-2451 with
+    UPPER: x <- x U^{-1} (trsm RIGHT, UPPER, 'N' at the leaves)."""
+    x_t = np.ascontiguousarray(x.T)
+    lu_solve_upper_t_left_dense(U, x_t)     # U^T x^T = b^T
+    x[:] = x_t.T
+
+
+def lu_solve_lower_left_h(L, b, eps):
+    """HMatrix::solveLowerTriangularLeft(HMatrix) h_matrix.cpp:2297-2336 +
+    recursiveSolveLowerTriangularLeft recursion.cpp:191-233 (LU, UNIT, LOWER)."""
+    if b is None:
+        return
+    if not L.is_leaf() and not b.is_leaf():
+        if L.ncol == b.nrow:
+            for k in range(b.ncol):
+                for i in range(L.nrow):
+                    bik = b.get(i, k)
+                    if bik is None:
+                        continue
+                    for j in range(i):
+                        lij = L.get(i, j)
+                        bjk = b.get(j, k)
+                        if lij is not None and bjk is not None:
+                            hgemm(bik, "N", "N", -1.0, lij, bjk, eps)
+                    lu_solve_lower_left_h(L.get(i, i), bik, eps)
+        elif L.ncol > 1 and b.nrow == 1 and b.ncol > 1:
+            for k in range(b.ncol):
+                lu_solve_lower_left_h(L, b.get(0, k), eps)
+        else:
+            raise NotImplementedError("solveLowerTriangularLeft case")
+        return
+    if not b.is_leaf():
+        if b.nrow == 1:
+            for k in range(b.ncol):
+                lu_solve_lower_left_h(L, b.get(0, k), eps)
+            return
+        d = _eval(b)
+        lu_solve_lower_left_dense(L, d)
+        _clear_and_set(b, d, eps)
+        return
+    if _is_null(b):
+        return
+    if b.kind == FULL:
+        lu_solve_lower_left_dense(L, b.full)
+    else:
+        lu_solve_lower_left_dense(L, b.rk.a)
+
+
+def lu_solve_upper_right_h(U, b, eps):
+    """HMatrix::solveUpperTriangularRight(HMatrix) h_matrix.cpp:2380-2415 +
+    recursiveSolveUpperTriangularRight recursion.cpp:86-134 (LU, NONUNIT, UPPER)."""
+    if b is None:
+        return
+    if not U.is_leaf() and not b.is_leaf():
+        if U.nrow == b.ncol:
+            for k in range(b.nrow):
+                for i in range(U.nrow):
+                    bki = b.get(k, i)
+                    if bki is None:
+                        continue
+                    for j in range(i):
+                        uji = U.get(j, i)
+                        bkj = b.get(k, j)
+                        if bkj is not None and uji is not None:
+                            hgemm(bki, "N", "N", -1.0, bkj, uji, eps)
+                    lu_solve_upper_right_h(U.get(i, i), bki, eps)
+        elif U.nrow > 1 and b.ncol == 1 and b.nrow > 1:
+            for k in range(b.nrow):
+                lu_solve_upper_right_h(U, b.get(k, 0), eps)
+        else:
+            raise NotImplementedError("solveUpperTriangularRight case")
+        return
+    if not b.is_leaf():
+        d = _eval(b)
+        lu_solve_upper_right_dense(U, d)
+        _clear_and_set(b, d, eps)
+        return
+    if _is_null(b):
+        return
+    if b.kind == FULL:
+        lu_solve_upper_right_dense(U, b.full)
+    else:
+        # Xb^T U = Bb^T  <=>  U^T Xb = Bb
+        lu_solve_upper_t_left_dense(U, b.rk.b)
+
+
+def lu(h, eps):
+    """HMatrix::luDecomposition h_matrix.cpp:2567 + recursiveLuDecomposition
+    recursion.cpp:238-287; a leaf is getrf'd in place (L unit lower + U, pivots)."""
+    if h.is_leaf():
+        import scipy.linalg as sla
+        lu_, piv = sla.lu_factor(h.full, check_finite=True)
+        h.full = lu_
+        h.piv = piv
+        return
+    n = h.nrow
+    for k in range(n):
+        hkk = h.get(k, k)
+        if hkk is None:
+            continue
+        lu(hkk, eps)
+        for i in range(k + 1, n):
+            if h.get(k, i) is not None:
+                lu_solve_lower_left_h(hkk, h.get(k, i), eps)
+        for i in range(k + 1, n):
+            if h.get(i, k) is not None:
+                lu_solve_upper_right_h(hkk, h.get(i, k), eps)
+        for i in range(k + 1, n):
+            if h.get(i, k) is None:
+                continue
+            for j in range(k + 1, n):
+                if h.get(i, j) is not None and h.get(k, j) is not None:
+                    hgemm(h.get(i, j), "N", "N", -1.0, h.get(i, k), h.get(k, j), eps)
+
+
+def solve_lu(LU, b):
+    """HMatrix::solve (LU) h_matrix.cpp:2772-2779 on a (N, nrhs) array in hmat order."""
+    lu_solve_lower_left_dense(LU, b)
+    lu_solve_upper_left_dense(LU, b)
 
 
 def factor_dense(L, n):

@@ -141,3 +141,113 @@ def test_mixed_precision_refinement_reaches_1e10():
     say("fp64 direct vs refined", np.linalg.norm(x64 - x) / np.linalg.norm(x))
     h.close()
     h2.close()
+
+
+@pytest.mark.gpu
+def test_solve_graph_replays_match_eager():
+    """The first solve for a given nrhs runs its recorded task list eagerly;
+    later solves replay it as a captured HIP graph.  Every replay, with other
+    eager work and a re-capture (after another plan moved the flag buffer) in
+    between, gives bitwise the eager result."""
+    say("solve graph replays n=3000")
+    n = 3000
+    pts = ho.cylinder_points(n, 1.0, 1.2, 31)
+    t = hmat_amd.ClusterTree(pts, 128)
+    h = hmat_amd.HMatrix(t)
+    h.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    h.factorize(hmat_amd.FACT_LLT)
+    rng = np.random.default_rng(8)
+    b = rng.standard_normal((n, 2))
+    c = rng.standard_normal((n, 3))
+    x_eager = h.solve(b, permute=False)            # records the nrhs=2 plan, eager
+    for rep in range(4):
+        x = h.solve(b, permute=False)              # graph capture / replay
+        say("replay", rep, "max diff", float(np.abs(x - x_eager).max()))
+        assert np.array_equal(x, x_eager)
+        if rep == 1:
+            y_eager = h.solve(c, permute=False)    # another plan, recorded eagerly
+            y = h.solve(c, permute=False)
+            assert np.array_equal(y, y_eager)
+        if rep == 2:
+            _ = h.to_dense()                       # unrelated eager work
+    h.close()
+
+
+def _shifted_problem(n, height, seed, leaf, nugget=0.2, shift=0.3):
+    pts = ho.cylinder_points(n, 1.0, height, seed)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    h = hmat_amd.HMatrix(t, lower_symmetric=False)
+    h.assemble(kind=hmat_amd.KERNEL_SHIFTED_EXPONENTIAL, length=0.5, nugget=nugget, eps=EPS,
+               shift=shift)
+    return pts, t, h
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,height,seed,leaf,nugget,shift",
+                         [(1200, 0.5, 4, 64, 0.2, 0.3), (3000, 1.2, 6, 128, 0.4, 0.4)])
+def test_lu_fp64_matches_oracle(n, height, seed, leaf, nugget, shift):
+    """H-LU (non-symmetric storage, non-symmetric kernel, leaf interchanges
+    present) against the oracle's H-LU on the same inputs: solution and residual."""
+    say(f"lu fp64 n={n} leaf={leaf} seed={seed}")
+    pts, t, h = _shifted_problem(n, height, seed, leaf, nugget, shift)
+    H = h.to_dense(mirror=False)
+    h.factorize(hmat_amd.FACT_LU)
+    say("stats", h.stats())
+    root, idx = ho.build_cluster_tree(pts, leaf)
+    bt = ho.build_block_tree(root, root, 2.0, False)
+    k = ho.ShiftedExpKernel(pts, 0.5, nugget, shift)
+    ho.assemble(bt, idx, k)
+    ho.lu(bt, EPS)
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal((n, 2))
+    x = h.solve(b)                                   # user numbering
+    A = k.block(np.arange(n), np.arange(n))
+    res = np.linalg.norm(A @ x - b) / np.linalg.norm(b)
+    xo = b[idx].copy()
+    ho.solve_lu(bt, xo)
+    xou = np.empty_like(xo)
+    xou[idx] = xo
+    res_o = np.linalg.norm(A @ xou - b) / np.linalg.norm(b)
+    rel = np.linalg.norm(x - xou) / np.linalg.norm(xou)
+    say("residual gpu", res, "oracle", res_o, "solution rel diff", rel)
+    assert res < 5 * max(res_o, 1e-6)
+    assert rel < 100 * EPS
+    xh = h.solve(b[idx], permute=False)
+    res_h = np.linalg.norm(H @ xh - b[idx]) / np.linalg.norm(b)
+    say("residual vs H", res_h)
+    assert res_h < 5 * max(res_o, 1e-3)
+    h.close()
+
+
+@pytest.mark.gpu
+def test_lu_mixed_precision_refinement_reaches_1e10():
+    import torch
+    say("lu mixed precision refinement n=2500")
+    n = 2500
+    pts, t, h = _shifted_problem(n, 1.0, 12, 128)
+    H = h.to_dense(mirror=False)
+    h.factorize(hmat_amd.FACT_LU, mixed=True)
+    b = np.random.default_rng(9).standard_normal((n, 2))
+    bd = torch.tensor(b.T.copy(), dtype=torch.float64, device="cuda").contiguous()
+    it, rel = h.solve_refined_device(bd.data_ptr(), 2, tol=1e-10, max_iter=40)
+    torch.cuda.synchronize()
+    x = bd.cpu().numpy().T
+    res = np.linalg.norm(H @ x - b, axis=0) / np.linalg.norm(b, axis=0)
+    say("LU IR sweeps", it, "reported", rel, "checked", res.max())
+    assert res.max() <= 1e-10
+    h.close()
+
+
+@pytest.mark.gpu
+def test_lu_singular_reports_error():
+    """Two identical points and no nugget give two identical rows: getrf ends
+    on an exact zero pivot and the factorisation reports an error."""
+    say("singular LU")
+    pts = ho.cylinder_points(40, 1.0, 0.3, 2)
+    pts[7] = pts[3]
+    t = hmat_amd.ClusterTree(pts, 64)
+    h = hmat_amd.HMatrix(t, lower_symmetric=False)
+    h.assemble(length=0.5, nugget=0.0, eps=EPS)
+    with pytest.raises(hmat_amd.EngineError, match="singular"):
+        h.factorize(hmat_amd.FACT_LU)
+    h.close()

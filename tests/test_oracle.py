@@ -144,3 +144,85 @@ def test_info_counts_match_partition():
     # uncompressed size of the stored lower triangle incl. diagonal blocks
     diag = sum(b.m * b.n for b in lv if b.rows is b.cols)
     assert diag + 2 * (inf["uncompressed_size"] - diag) == n * n
+
+
+def _lu_problem(n=1200, eps=1e-4, leaf=64, shift=0.3, nugget=0.2, seed=4):
+    pts = ho.cylinder_points(n, 1.0, 0.5, seed)
+    root, idx = ho.build_cluster_tree(pts, leaf)
+    bt = ho.build_block_tree(root, root, 2.0, False)
+    k = ho.ShiftedExpKernel(pts, 0.5, nugget, shift)
+    ho.assemble(bt, idx, k, eps=eps, lowrank_eps=eps)
+    H = ho.to_dense(bt, n, n, symmetric=False)
+    return bt, idx, k, H
+
+
+def test_shifted_kernel_is_nonsymmetric_translation_invariant():
+    pts = ho.cylinder_points(50, 1.0, 0.5, 1)
+    k = ho.ShiftedExpKernel(pts, 0.5, 0.0, 0.3)
+    i = np.arange(50)
+    K = k.block(i, i)
+    assert not np.allclose(K, K.T)
+    # K(x, y) = exp(-|x - y - s e_z| / l): moving y by +s e_z maps to the symmetric kernel
+    d = pts[:, None, :] - pts[None, :, :]
+    d[:, :, 2] -= 0.3
+    assert np.allclose(K, np.exp(-np.sqrt((d * d).sum(axis=2)) / 0.5), rtol=0, atol=1e-15)
+
+
+def test_lu_single_leaf_is_getrf():
+    """A matrix that is one leaf: the H-LU is LAPACK getrf and the solve is
+    getrs (interchanges applied in getrf order, unit L, then U)."""
+    import scipy.linalg as sla
+    bt, idx, k, H = _lu_problem(n=60, leaf=64)
+    assert bt.is_leaf()
+    ho.lu(bt, 1e-4)
+    lu_, piv = sla.lu_factor(H)
+    assert np.array_equal(bt.full, lu_) and np.array_equal(bt.piv, piv)
+    assert (piv != np.arange(len(piv))).any()          # pivoting really happens
+    b = np.random.default_rng(2).standard_normal((60, 3))
+    x = b.copy()
+    ho.solve_lu(bt, x)
+    assert np.allclose(x, np.linalg.solve(H, b), rtol=0, atol=1e-10 * np.abs(x).max())
+
+
+@pytest.mark.parametrize("eps,bound", [(1e-4, 5e-3), (1e-8, 1e-6), (1e-12, 1e-10)])
+def test_hlu_solve_residual_tracks_eps(eps, bound):
+    """H-LU on non-symmetric storage of a non-symmetric kernel: the residual of
+    the solve against the assembled matrix scales with the compression eps,
+    and at eps=1e-12 the solution is the dense solve's."""
+    bt, idx, k, H = _lu_problem(eps=eps)
+    ho.lu(bt, eps)
+    pivoted = [lf for lf in ho.leaves(bt) if lf.piv is not None and (lf.piv != np.arange(len(lf.piv))).any()]
+    assert pivoted                                       # leaf interchanges are exercised
+    b = np.random.default_rng(3).standard_normal((H.shape[0], 2))
+    x = b.copy()
+    ho.solve_lu(bt, x)
+    res = np.linalg.norm(H @ x - b) / np.linalg.norm(b)
+    assert res <= bound
+    if eps == 1e-12:
+        xe = np.linalg.solve(H, b)
+        assert np.linalg.norm(x - xe) <= 1e-8 * np.linalg.norm(xe)
+
+
+def test_hlu_upper_solves_against_dense():
+    """The U-side solves of the recursion (x U^{-1}, U^{-T} x, U^{-1} x) on a
+    factorised tree agree with dense triangular solves of its U part."""
+    import scipy.linalg as sla
+    bt, idx, k, H = _lu_problem(n=700, eps=1e-12, leaf=64)
+    ho.lu(bt, 1e-12)
+    n = H.shape[0]
+    F = ho.to_dense(bt, n, n, symmetric=False)
+    # U part of the factor: upper triangles of the diagonal leaves and every
+    # block above them (block-upper storage of U)
+    U = np.triu(F)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((n, 2))
+    y = x.copy()
+    ho.lu_solve_upper_left_dense(bt, y)
+    assert np.allclose(U @ y, x, rtol=0, atol=1e-9 * np.abs(x).max())
+    y = x.copy()
+    ho.lu_solve_upper_t_left_dense(bt, y)
+    assert np.allclose(U.T @ y, x, rtol=0, atol=1e-9 * np.abs(x).max())
+    z = rng.standard_normal((3, n))
+    w = z.copy()
+    ho.lu_solve_upper_right_dense(bt, w)
+    assert np.allclose(w @ U, z, rtol=0, atol=1e-9 * np.abs(z).max())
