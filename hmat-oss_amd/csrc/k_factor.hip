@@ -769,7 +769,9 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
   __syncthreads();
   const long t2 = wall_clock64();
   int sweeps = 0;
-  qr_finish<T>(Va, lda, m, Vb, ldb, n, K, w, cr == sh && K <= 32, fabs(t.eps), tg, overflow, &flag,
+  // the core SVD runs on one wavefront when its scratch is in LDS
+  const bool core_lds = mode == 5 || mode == 6 || cr == sh;
+  qr_finish<T>(Va, lda, m, Vb, ldb, n, K, w, core_lds && K <= 32, fabs(t.eps), tg, overflow, &flag,
                &sweeps);
   if (pr && threadIdx.x == 0) {
     pr[0] = mode; pr[1] = m; pr[2] = n; pr[3] = K;

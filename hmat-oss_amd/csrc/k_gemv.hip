@@ -16,7 +16,6 @@ namespace hmx {
 
 namespace {
 
-constexpr int kMaxRowsPerLane = 4;  // destination clusters of up to 256 rows
 constexpr int kRhsChunk = 8;
 
 template <typename T>
@@ -68,20 +67,32 @@ __global__ void k_zreduce(const ZReduce* red, const int* ranks, const double* zp
   }
 }
 
+// y pass.  One workgroup per destination leaf cluster I (at most 256 rows):
+// thread t < 128 owns rows t and t + 128 of I for the whole list of
+// contributions.  A full leaf (I x J) is staged in LDS in column chunks with
+// coalesced loads (odd leading dimension); threads < 128 form M x_J row by
+// row while, for a mirrored leaf, threads >= 128 form M^T x_I column by column
+// into the leaf's slot, so neither product needs a cross-lane reduction.  Rk
+// slices (A z, or B z' of a mirrored Rk leaf) are read straight from HBM, rows
+// on threads < 128 and the second half of the rank on threads >= 128.
+constexpr int kYTile = 9216;  // doubles of the LDS tile (72 KiB: two workgroups per CU)
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const int* ranks,
                                                   const T* arena, const YItem* items,
                                                   const YContrib* contrib, const double* z,
                                                   double* slots, const T* x, int ldx, int nrhs,
                                                   double* acc_out, int ldacc) {
-  __shared__ double red[kWaves][kWave * kMaxRowsPerLane];
+  __shared__ double tile[kYTile];
   const YItem it = items[blockIdx.x];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int tid = threadIdx.x;
+  const bool lo = tid < 128;
+  const int h = lo ? tid : tid - 128;  // row (lo) or column / rank-split lane (hi)
   for (int rc = 0; rc < nrhs; rc += kRhsChunk) {
     const int nr = nrhs - rc < kRhsChunk ? nrhs - rc : kRhsChunk;
-    double acc[kMaxRowsPerLane][kRhsChunk];
+    double acc[2][kRhsChunk];
 #pragma unroll
-    for (int q = 0; q < kMaxRowsPerLane; ++q)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
       for (int r = 0; r < kRhsChunk; ++r) acc[q][r] = 0.0;
     for (int ci = it.beg; ci < it.end; ++ci) {
@@ -90,32 +101,42 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
       if (c.kind == 0) {
         const T* M = arena + d.a;
         const bool mir = c.z >= 0;
-        for (int j = w; j < d.n; j += kWaves) {
-          const T* col = M + (size_t)j * d.m;
-          double t[kRhsChunk];
+        const int m = d.m, ldt = d.m | 1;
+        const int cw_max = kYTile / ldt;
+        for (int j0 = 0; j0 < d.n; j0 += cw_max) {
+          const int cw = d.n - j0 < cw_max ? d.n - j0 : cw_max;
+          __syncthreads();
+          const T* src = M + (size_t)j0 * m;  // columns j0 .. j0+cw-1 are contiguous
+          for (int e = tid; e < m * cw; e += kBlock) tile[e % m + (e / m) * ldt] = (double)src[e];
+          __syncthreads();
+          if (lo) {
 #pragma unroll
-          for (int r = 0; r < kRhsChunk; ++r) t[r] = 0.0;
+            for (int q = 0; q < 2; ++q) {
+              const int i = h + 128 * q;
+              if (i >= m) continue;
+              for (int jj = 0; jj < cw; ++jj) {
+                const double v = tile[i + jj * ldt];
+                const T* xj = x + d.c0 + j0 + jj + (size_t)rc * ldx;
 #pragma unroll
-          for (int q = 0; q < kMaxRowsPerLane; ++q) {
-            const int i = lane + q * kWave;
-            if (i < d.m) {
-              const double v = (double)col[i];
-#pragma unroll
-              for (int r = 0; r < kRhsChunk; ++r) {
-                if (r < nr) {
-                  acc[q][r] += v * (double)x[d.c0 + j + (size_t)(rc + r) * ldx];
-                  if (mir) t[r] += v * (double)x[d.r0 + i + (size_t)(rc + r) * ldx];
-                }
+                for (int r = 0; r < kRhsChunk; ++r)
+                  if (r < nr) acc[q][r] += v * (double)xj[(size_t)r * ldx];
               }
             }
-          }
-          if (mir) {
+          } else if (mir) {
+            for (int jj = h; jj < cw; jj += 128) {
+              double t[kRhsChunk];
 #pragma unroll
-            for (int r = 0; r < kRhsChunk; ++r) {
-              if (r < nr) {
-                const double s = wave_sum(t[r]);
-                if (lane == 0) slots[c.z + j + (size_t)(rc + r) * d.n] = s;
+              for (int r = 0; r < kRhsChunk; ++r) t[r] = 0.0;
+              for (int i = 0; i < m; ++i) {
+                const double v = tile[i + jj * ldt];
+                const T* xi = x + d.r0 + i + (size_t)rc * ldx;
+#pragma unroll
+                for (int r = 0; r < kRhsChunk; ++r)
+                  if (r < nr) t[r] += v * (double)xi[(size_t)r * ldx];
               }
+#pragma unroll
+              for (int r = 0; r < kRhsChunk; ++r)
+                if (r < nr) slots[c.z + j0 + jj + (size_t)(rc + r) * d.n] = t[r];
             }
           }
         }
@@ -125,34 +146,39 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
         const int ld = isA ? d.m : d.n;
         const T* P = arena + (isA ? d.a : d.b) + c.row0;
         const double* zz = z + c.z;
-        for (int l = w; l < k; l += kWaves) {
-          const T* col = P + (size_t)l * ld;
-          double zl[kRhsChunk];
+        const int kh = (k + 1) / 2;
+        const int l0 = lo ? 0 : kh, l1 = lo ? kh : k;
 #pragma unroll
-          for (int r = 0; r < kRhsChunk; ++r) zl[r] = (r < nr) ? zz[l + (size_t)(rc + r) * d.cap] : 0.0;
+        for (int q = 0; q < 2; ++q) {
+          const int i = h + 128 * q;
+          if (i >= it.len) continue;
+          for (int l = l0; l < l1; ++l) {
+            const double v = (double)P[i + (size_t)l * ld];
+            const double* zl = zz + l + (size_t)rc * d.cap;
 #pragma unroll
-          for (int q = 0; q < kMaxRowsPerLane; ++q) {
-            const int i = lane + q * kWave;
-            if (i < it.len) {
-              const double v = (double)col[i];
-#pragma unroll
-              for (int r = 0; r < kRhsChunk; ++r) acc[q][r] += v * zl[r];
-            }
+            for (int r = 0; r < kRhsChunk; ++r)
+              if (r < nr) acc[q][r] += v * zl[(size_t)r * d.cap];
           }
         }
       }
     }
-    // fixed-order reduction over the waves
-    for (int r = 0; r < nr; ++r) {
-      __syncthreads();
+    // rows: the row owner's sum plus the rank half accumulated by its partner
+    __syncthreads();
+    if (!lo) {
 #pragma unroll
-      for (int q = 0; q < kMaxRowsPerLane; ++q) red[w][lane + q * kWave] = acc[q][r];
-      __syncthreads();
-      for (int i = threadIdx.x; i < it.len; i += kBlock) {
-        double s = 0.0;
+      for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int ww = 0; ww < kWaves; ++ww) s += red[ww][i];
-        acc_out[it.r0 + i + (size_t)(rc + r) * ldacc] = s;
+        for (int r = 0; r < kRhsChunk; ++r) tile[(h + 128 * q) * kRhsChunk + r] = acc[q][r];
+    }
+    __syncthreads();
+    if (lo) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int i = h + 128 * q;
+        if (i >= it.len) continue;
+#pragma unroll
+        for (int r = 0; r < kRhsChunk; ++r)
+          if (r < nr) acc_out[it.r0 + i + (size_t)(rc + r) * ldacc] = acc[q][r] + tile[i * kRhsChunk + r];
       }
     }
   }
