@@ -70,36 +70,54 @@ __device__ inline void strided_axpy(double* y, double f, const double* x, int i0
   for (; i < m; i += g) y[i] -= f * x[i];
 }
 
+// Sum over the whole team; every thread gets the same value.  'red' holds
+// kWaves doubles (workgroup team only).
+template <int kLanes>
+__device__ inline double team_sum(double v, double* red) {
+  v = wave_sum(v);
+  if constexpr (kLanes == kWave) {
+    return v;
+  } else {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) s += red[q];
+    return s;
+  }
+}
+
 // Householder QR, LAPACK dgeqrf conventions (dlarfg: beta = -sign(alpha)*norm,
 // tau = (beta - alpha) / beta, v = x / (alpha - beta), v[0] = 1 implicit).
 // On exit R is in the upper triangle, v_j below the diagonal, tau[j] set.
-// Column c belongs to lane group c mod (kLanes / g): that group forms the
-// reflector of column c and applies every earlier reflector to it.
+// The whole team forms each reflector (norm, scaling); trailing column c
+// belongs to lane group c mod (kLanes / g), which applies the reflector to it.
 template <int kLanes>
 __device__ inline void team_house_qr(double* A, int lda, int m, int k, double* tau) {
+  __shared__ double red[kWaves];
   const int tid = team_tid<kLanes>();
   const int steps = m < k ? m : k;
   const int g = group_lanes(k, kLanes), ngrp = kLanes / g, grp = tid / g, sub = tid % g;
   for (int j = 0; j < steps; ++j) {
     double* v = A + (size_t)j * lda;
-    if (grp == j % ngrp) {
-      double s = strided_dot(v, v, j + 1 + sub, m, g);
-      s = group_sum(s, g);
-      const double alpha = v[j];
-      double t = 0.0, beta = alpha, scal = 0.0;
-      if (s != 0.0) {
-        beta = -copysign(sqrt(alpha * alpha + s), alpha);
-        t = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
-        for (int i = j + 1 + sub; i < m; i += g) v[i] *= scal;
-      }
-      if (sub == 0) {
-        tau[j] = t;
-        v[j] = beta;
-      }
+    double s = strided_dot(v, v, j + 1 + tid, m, kLanes);
+    s = team_sum<kLanes>(s, red);
+    const double alpha = v[j];
+    double t = 0.0, beta = alpha;
+    if (s != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + s), alpha);
+      t = (beta - alpha) / beta;
+      const double scal = 1.0 / (alpha - beta);
+      for (int i = j + 1 + tid; i < m; i += kLanes) v[i] *= scal;
     }
     team_sync<kLanes>();
-    const double t = tau[j];
+    if (tid == 0) {
+      tau[j] = t;
+      v[j] = beta;
+    }
+    team_sync<kLanes>();
     if (t != 0.0) {
       int c = j + 1 + ((grp - (j + 1) % ngrp) + ngrp) % ngrp;
       for (; c < k; c += ngrp) {
@@ -200,42 +218,58 @@ __device__ inline void wave_apply_q(const double* V, int ldv, int m, int nref, c
   team_apply_q<kWave>(V, ldv, m, nref, tau, Y, ldy, r);
 }
 
+constexpr int kJacobiMaxCols = 256;  // columns whose norms the Jacobi keeps in scratch
+
 // One-sided (Hestenes) Jacobi SVD of G (rows x cols, ldg): on exit G = U Sigma
 // (columns mutually orthogonal, unsorted) and V (cols x cols, ldv) holds the
 // accumulated rotations so that G_in = (U Sigma) V^T.  Round-robin ordering:
 // the cols/2 pairs of a round are independent and each belongs to one lane
-// group.  A pair is rotated while |g| > 1e-13 sqrt(a b) and
-// |g| > 1e-30 ||G||_F^2 (columns at rounding level of ||G|| carry nothing for
-// the truncation thresholds used here).  Returns the number of sweeps.
-// 'sh2' is 2 shared doubles of scratch (workgroup team only).
+// group.  Squared column norms are kept in 'nrm' (cols doubles of shared
+// scratch), recomputed at the start of every sweep and updated exactly by
+// each rotation (a' = a - t g, b' = b + t g), so a pair costs one reduction.
+// A pair is rotated while g^2 > 1e-26 a b and |g| > 1e-30 ||G||_F^2 (columns
+// at rounding level of ||G|| carry nothing for the truncation thresholds used
+// here).  Returns the number of sweeps.  'sh2' is 2 + kWaves shared doubles of
+// scratch (workgroup team only); 'nrm' holds cols doubles or is null.
 template <int kLanes>
 __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double* V, int ldv,
-                                  double* sh2, int max_sweeps) {
+                                  double* sh2, double* nrm, int max_sweeps) {
   const int tid = team_tid<kLanes>();
   for (int e = tid; e < cols * cols; e += kLanes) {
     const int i = e % cols, j = e / cols;
     V[i + (size_t)j * ldv] = (i == j) ? 1.0 : 0.0;
   }
-  double f = 0.0;
-  for (int e = tid; e < rows * cols; e += kLanes) {
-    const double x = G[e % rows + (size_t)(e / rows) * ldg];
-    f += x * x;
+  if (cols < 2) {
+    team_sync<kLanes>();
+    return 0;
   }
-  f = wave_sum(f);
-  if constexpr (kLanes != kWave) {
-    if (threadIdx.x == 0) sh2[0] = 0.0;
-    __syncthreads();
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&sh2[0], f);
-    __syncthreads();
-    f = sh2[0];
-  }
-  team_sync<kLanes>();
-  if (cols < 2) return 0;
-  const double floor2 = 1e-30 * f, tol = 1e-13;
+  const double tol2 = 1e-26;
   const int np = (cols + 1) & ~1, rounds = np - 1, half = np / 2;
   const int g = group_lanes(half, kLanes), ngrp = kLanes / g, grp = tid / g, sub = tid % g;
+  // without scratch for every column the norms are recomputed pair by pair
+  const bool track = nrm != nullptr && cols <= kJacobiMaxCols;
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
+    // column norms (and the Frobenius floor) afresh each sweep
+    team_sync<kLanes>();
+    double f = 0.0;
+    if (track) {
+      for (int c = grp; c < cols; c += ngrp) {
+        double a = 0.0;
+        for (int i = sub; i < rows; i += g) a += G[i + (size_t)c * ldg] * G[i + (size_t)c * ldg];
+        a = group_sum(a, g);
+        if (sub == 0) nrm[c] = a;
+      }
+      team_sync<kLanes>();
+      for (int c = 0; c < cols; ++c) f += nrm[c];
+    } else {
+      for (int e = tid; e < rows * cols; e += kLanes) {
+        const double x = G[e % rows + (size_t)(e / rows) * ldg];
+        f += x * x;
+      }
+      f = team_sum<kLanes>(f, sh2 ? sh2 + 2 : nullptr);
+    }
+    const double floor2 = 1e-30 * f;
     int rotated = 0;
     for (int rd = 0; rd < rounds; ++rd) {
       for (int pk = grp; pk < half; pk += ngrp) {
@@ -244,8 +278,10 @@ __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double
           p = np - 1;
           q = rd;
         } else {
-          p = (rd + pk) % (np - 1);
-          q = (rd - pk + (np - 1)) % (np - 1);
+          p = rd + pk;
+          if (p >= np - 1) p -= np - 1;
+          q = rd - pk + (np - 1);
+          if (q >= np - 1) q -= np - 1;
         }
         if (p > q) {
           const int tmp = p;
@@ -255,19 +291,28 @@ __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double
         if (q >= cols) continue;
         double* gp = G + (size_t)p * ldg;
         double* gq = G + (size_t)q * ldg;
-        double a = 0.0, b = 0.0, c2 = 0.0;
-        for (int i = sub; i < rows; i += g) {
-          const double x = gp[i], y = gq[i];
-          a += x * x;
-          b += y * y;
-          c2 += x * y;
+        double c2 = 0.0, a, b;
+        if (track) {
+          for (int i = sub; i < rows; i += g) c2 += gp[i] * gq[i];
+          c2 = group_sum(c2, g);
+          a = nrm[p];
+          b = nrm[q];
+        } else {
+          a = 0.0;
+          b = 0.0;
+          for (int i = sub; i < rows; i += g) {
+            a += gp[i] * gp[i];
+            b += gq[i] * gq[i];
+            c2 += gp[i] * gq[i];
+          }
+          a = group_sum(a, g);
+          b = group_sum(b, g);
+          c2 = group_sum(c2, g);
         }
-        a = group_sum(a, g);
-        b = group_sum(b, g);
-        c2 = group_sum(c2, g);
-        if (!(fabs(c2) > tol * sqrt(a * b)) || !(fabs(c2) > floor2)) continue;
-        const double zeta = (b - a) / (2.0 * c2);
-        const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        if (!(c2 * c2 > tol2 * a * b) || !(fabs(c2) > floor2)) continue;
+        // t = tan(theta) of the rotation zeroing the pair's inner product
+        const double d = b - a;
+        const double t = (d < 0.0 ? -2.0 * c2 : 2.0 * c2) / (fabs(d) + sqrt(d * d + 4.0 * c2 * c2));
         const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
         for (int i = sub; i < rows; i += g) {
           const double x = gp[i], y = gq[i];
@@ -280,6 +325,10 @@ __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double
           const double x = vp[i], y = vq[i];
           vp[i] = c * x - s * y;
           vq[i] = s * x + c * y;
+        }
+        if (track && sub == 0) {
+          nrm[p] = a - t * c2;
+          nrm[q] = b + t * c2;
         }
         rotated = 1;
       }
@@ -305,17 +354,19 @@ __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double
   return sweep;
 }
 
-// Workgroup version used by the existing call sites.
+// Workgroup version used by the existing call sites (cols <= kJacobiMaxCols).
 __device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
                                   int* /*flag*/, int max_sweeps = 40) {
-  __shared__ double sh2[2];
-  team_jacobi<kBlock>(G, ldg, rows, cols, V, ldv, sh2, max_sweeps);
+  __shared__ double sh2[2 + kWaves];
+  __shared__ double nrm[kJacobiMaxCols];
+  team_jacobi<kBlock>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
 }
 
-// Single-wavefront version (G, V in LDS).
+// Single-wavefront version (G, V in LDS; cols <= kJacobiMaxCols).
 __device__ inline int wave_jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
                                       int max_sweeps = 40) {
-  return team_jacobi<kWave>(G, ldg, rows, cols, V, ldv, nullptr, max_sweeps);
+  __shared__ double nrm[kJacobiMaxCols];
+  return team_jacobi<kWave>(G, ldg, rows, cols, V, ldv, nullptr, nrm, max_sweeps);
 }
 
 // Column norms of G into sig[0..cols), then order[] = permutation sorting

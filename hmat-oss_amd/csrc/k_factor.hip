@@ -256,11 +256,10 @@ __global__ void __launch_bounds__(kBlock) k_getrf(const FTask* tasks, int* info)
 // ---------------------------------------------------------------------------
 constexpr int kShDoubles = 19456;  // 152 KiB of LDS per workgroup
 
+// Substitution with one thread per independent row / column (L or X too
+// large for the blocked-inverse kernel below).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* fail) {
-  __shared__ double sh[kShDoubles];
-  if (*fail) return;
-  const FTask& t = tasks[blockIdx.x];
+__device__ void trsm_rowwise(const FTask& t, double* sh) {
   const int n = t.n.s;  // order of L
   const int other = eff(t.m);
   if (other <= 0) return;
@@ -380,6 +379,161 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
       const int i = e % xr, j = e / xr;
       X[i + (size_t)j * ldx] = (T)Xs[i + j * xl];
     }
+  }
+}
+
+constexpr int kTrsmRegs = 8;  // block-step results per thread: 16 x 8 x 256 covers 128 columns
+
+// Blocked TRSM with L (n x n) and X staged in LDS.  The 16 x 16 diagonal
+// blocks of L are inverted first (one thread per column of each block), then
+// every 16-row (16-column for mode 0) step is a small dense product with that
+// inverse followed by a rank-16 update of the rest, each spread over the whole
+// workgroup: the serial chain is 3 barriers per block instead of one thread
+// walking the whole triangle.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* fail) {
+  __shared__ double sh[kShDoubles];
+  if (*fail) return;
+  const FTask& t = tasks[blockIdx.x];
+  const int n = t.n.s;
+  const int other = eff(t.m);
+  if (other <= 0) return;
+  const int mode = t.mode & kTrsmBase;
+  const bool transa = (t.mode & kTrsmTransA) != 0, unit = (t.mode & kTrsmUnit) != 0;
+  const int nb = (n + kNb - 1) / kNb;
+  const int xr = mode == 0 ? other : n, xc = mode == 0 ? n : other;
+  const int xl = xr | 1;
+  // every block step keeps at most kTrsmRegs results per thread in registers
+  const int64_t need = (int64_t)n * n + (int64_t)nb * kNb * kNb + (int64_t)xl * xc;
+  if (need > kShDoubles || other > kNb * kTrsmRegs) {
+    trsm_rowwise<T>(t, sh);
+    return;
+  }
+  double* Ls = sh;                          // n x n, ld n (L, or U^T with transa)
+  double* Inv = Ls + (size_t)n * n;         // nb blocks of 16 x 16 (ld 16)
+  double* Xs = Inv + (size_t)nb * kNb * kNb;  // xr x xc, ld xl
+  const T* Lg = (const T*)t.A;
+  const int ldl = t.lda;
+  T* X = (T*)t.C;
+  const int ldx = t.ldc;
+  for (int e = threadIdx.x; e < n * n; e += kBlock) {
+    const int i = e % n, j = e / n;
+    Ls[e] = transa ? (double)Lg[j + (size_t)i * ldl] : (double)Lg[i + (size_t)j * ldl];
+  }
+  for (int e = threadIdx.x; e < xr * xc; e += kBlock) {
+    const int i = e % xr, j = e / xr;
+    Xs[i + j * xl] = (double)X[i + (size_t)j * ldx];
+  }
+  __syncthreads();
+  if ((t.mode & kTrsmPiv) && mode != 0) {
+    const int* piv = (const int*)t.B;
+    for (int c = threadIdx.x; c < other; c += kBlock)
+      for (int i = 0; i < n; ++i) {
+        int p = piv[i];
+        p = p < 0 ? 0 : (p >= n ? n - 1 : p);
+        if (p != i) {
+          const double a = Xs[i + c * xl];
+          Xs[i + c * xl] = Xs[p + c * xl];
+          Xs[p + c * xl] = a;
+        }
+      }
+  }
+  // inverses of the diagonal blocks: thread (b, c) forms column c of block b
+  for (int w = threadIdx.x; w < nb * kNb; w += kBlock) {
+    const int b = w / kNb, c = w % kNb, r0 = b * kNb;
+    const int bs = n - r0 < kNb ? n - r0 : kNb;
+    double* inv = Inv + (size_t)b * kNb * kNb;
+    if (c >= bs) continue;
+    for (int r = 0; r < c; ++r) inv[r + c * kNb] = 0.0;
+    for (int r = c; r < bs; ++r) {
+      double s = r == c ? 1.0 : 0.0;
+      for (int l = c; l < r; ++l) s -= Ls[(r0 + r) + (size_t)(r0 + l) * n] * inv[l + c * kNb];
+      inv[r + c * kNb] = unit ? s : s / Ls[(r0 + r) + (size_t)(r0 + r) * n];
+    }
+  }
+  __syncthreads();
+  if (mode == 1) {
+    // X <- L^{-1} X: block rows top to bottom
+    for (int b = 0; b < nb; ++b) {
+      const int r0 = b * kNb, bs = n - r0 < kNb ? n - r0 : kNb;
+      const double* inv = Inv + (size_t)b * kNb * kNb;
+      // X_b <- inv X_b (results held in registers across the barrier)
+      double v[kTrsmRegs];
+      int nv = 0;
+      for (int e = threadIdx.x; e < bs * other; e += kBlock, ++nv) {
+        const int r = e % bs, c = e / bs;
+        double s = 0.0;
+        for (int l = 0; l <= r; ++l) s += inv[r + l * kNb] * Xs[(r0 + l) + c * xl];
+        v[nv] = s;
+      }
+      __syncthreads();
+      nv = 0;
+      for (int e = threadIdx.x; e < bs * other; e += kBlock, ++nv) Xs[(r0 + e % bs) + (e / bs) * xl] = v[nv];
+      __syncthreads();
+      const int rest = n - r0 - bs;
+      for (int e = threadIdx.x; e < rest * other; e += kBlock) {
+        const int i = r0 + bs + e % rest, c = e / rest;
+        double s = Xs[i + c * xl];
+        for (int l = 0; l < bs; ++l) s -= Ls[i + (size_t)(r0 + l) * n] * Xs[(r0 + l) + c * xl];
+        Xs[i + c * xl] = s;
+      }
+      __syncthreads();
+    }
+  } else if (mode == 2) {
+    // X <- L^{-T} X: block rows bottom to top, with inv^T
+    for (int b = nb - 1; b >= 0; --b) {
+      const int r0 = b * kNb, bs = n - r0 < kNb ? n - r0 : kNb;
+      const double* inv = Inv + (size_t)b * kNb * kNb;
+      double v[kTrsmRegs];
+      int nv = 0;
+      for (int e = threadIdx.x; e < bs * other; e += kBlock, ++nv) {
+        const int r = e % bs, c = e / bs;
+        double s = 0.0;
+        for (int l = r; l < bs; ++l) s += inv[l + r * kNb] * Xs[(r0 + l) + c * xl];
+        v[nv] = s;
+      }
+      __syncthreads();
+      nv = 0;
+      for (int e = threadIdx.x; e < bs * other; e += kBlock, ++nv) Xs[(r0 + e % bs) + (e / bs) * xl] = v[nv];
+      __syncthreads();
+      for (int e = threadIdx.x; e < r0 * other; e += kBlock) {
+        const int i = e % r0, c = e / r0;
+        double s = Xs[i + c * xl];
+        for (int l = 0; l < bs; ++l) s -= Ls[(r0 + l) + (size_t)i * n] * Xs[(r0 + l) + c * xl];
+        Xs[i + c * xl] = s;
+      }
+      __syncthreads();
+    }
+  } else {
+    // X <- X L^{-T}: block columns left to right, X_b <- X_b inv^T
+    for (int b = 0; b < nb; ++b) {
+      const int c0 = b * kNb, bs = n - c0 < kNb ? n - c0 : kNb;
+      const double* inv = Inv + (size_t)b * kNb * kNb;
+      double v[kTrsmRegs];
+      int nv = 0;
+      for (int e = threadIdx.x; e < other * bs; e += kBlock, ++nv) {
+        const int r = e % other, j = e / other;
+        double s = 0.0;
+        for (int l = 0; l <= j; ++l) s += Xs[r + (c0 + l) * xl] * inv[j + l * kNb];
+        v[nv] = s;
+      }
+      __syncthreads();
+      nv = 0;
+      for (int e = threadIdx.x; e < other * bs; e += kBlock, ++nv) Xs[e % other + (c0 + e / other) * xl] = v[nv];
+      __syncthreads();
+      const int rest = n - c0 - bs;
+      for (int e = threadIdx.x; e < other * rest; e += kBlock) {
+        const int r = e % other, j = c0 + bs + e / other;
+        double s = Xs[r + j * xl];
+        for (int l = 0; l < bs; ++l) s -= Xs[r + (c0 + l) * xl] * Ls[j + (size_t)(c0 + l) * n];
+        Xs[r + j * xl] = s;
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = threadIdx.x; e < xr * xc; e += kBlock) {
+    const int i = e % xr, j = e / xr;
+    X[i + (size_t)j * ldx] = (T)Xs[i + j * xl];
   }
 }
 

@@ -70,31 +70,41 @@ __global__ void k_zreduce(const ZReduce* red, const int* ranks, const double* zp
 // y pass.  One workgroup per destination leaf cluster I (at most 256 rows):
 // thread t < 128 owns rows t and t + 128 of I for the whole list of
 // contributions.  A full leaf (I x J) is staged in LDS in column chunks with
-// coalesced loads (odd leading dimension); threads < 128 form M x_J row by
-// row while, for a mirrored leaf, threads >= 128 form M^T x_I column by column
-// into the leaf's slot, so neither product needs a cross-lane reduction.  Rk
-// slices (A z, or B z' of a mirrored Rk leaf) are read straight from HBM, rows
-// on threads < 128 and the second half of the rank on threads >= 128.
-constexpr int kYTile = 9216;  // doubles of the LDS tile (72 KiB: two workgroups per CU)
+// coalesced loads (odd leading dimension), together with x_J; threads < 128
+// form M x_J row by row while, for a mirrored leaf, threads >= 128 form
+// M^T x_I column by column into the leaf's slot (x_I staged once), so neither
+// product needs a cross-lane reduction.  Rk slices (A z, or B z' of a
+// mirrored Rk leaf) stream from HBM with z staged in LDS, rows on threads
+// < 128 and the second half of the rank on threads >= 128.  kR right-hand
+// sides per pass.
+constexpr int kYTile = 6144;   // doubles of the leaf tile (48 KiB)
+constexpr int kYVec = 256 * 8; // doubles of each staged vector block (x_I, x_J / z)
 
-template <typename T>
+template <typename T, int kR>
 __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const int* ranks,
                                                   const T* arena, const YItem* items,
                                                   const YContrib* contrib, const double* z,
                                                   double* slots, const T* x, int ldx, int nrhs,
                                                   double* acc_out, int ldacc) {
   __shared__ double tile[kYTile];
+  __shared__ double xI[kYVec];
+  __shared__ double xJ[kYVec];  // x_J of a full leaf, or z of an Rk slice
   const YItem it = items[blockIdx.x];
   const int tid = threadIdx.x;
   const bool lo = tid < 128;
   const int h = lo ? tid : tid - 128;  // row (lo) or column / rank-split lane (hi)
-  for (int rc = 0; rc < nrhs; rc += kRhsChunk) {
-    const int nr = nrhs - rc < kRhsChunk ? nrhs - rc : kRhsChunk;
-    double acc[2][kRhsChunk];
+  for (int rc = 0; rc < nrhs; rc += kR) {
+    const int nr = nrhs - rc < kR ? nrhs - rc : kR;
+    __syncthreads();
+    for (int e = tid; e < it.len * kR; e += kBlock) {
+      const int i = e % it.len, r = e / it.len;
+      xI[i + r * 256] = r < nr ? (double)x[it.r0 + i + (size_t)(rc + r) * ldx] : 0.0;
+    }
+    double acc[2][kR];
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int r = 0; r < kRhsChunk; ++r) acc[q][r] = 0.0;
+      for (int r = 0; r < kR; ++r) acc[q][r] = 0.0;
     for (int ci = it.beg; ci < it.end; ++ci) {
       const YContrib c = contrib[ci];
       const LeafDesc d = desc[c.leaf];
@@ -102,40 +112,44 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
         const T* M = arena + d.a;
         const bool mir = c.z >= 0;
         const int m = d.m, ldt = d.m | 1;
-        const int cw_max = kYTile / ldt;
+        const int cw_max = kYTile / ldt < 256 ? kYTile / ldt : 256;  // x_J block holds 256 rows
         for (int j0 = 0; j0 < d.n; j0 += cw_max) {
           const int cw = d.n - j0 < cw_max ? d.n - j0 : cw_max;
           __syncthreads();
           const T* src = M + (size_t)j0 * m;  // columns j0 .. j0+cw-1 are contiguous
           for (int e = tid; e < m * cw; e += kBlock) tile[e % m + (e / m) * ldt] = (double)src[e];
+          for (int e = tid; e < cw * kR; e += kBlock) {
+            const int j = e % cw, r = e / cw;
+            xJ[j + r * 256] = r < nr ? (double)x[d.c0 + j0 + j + (size_t)(rc + r) * ldx] : 0.0;
+          }
           __syncthreads();
           if (lo) {
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
               const int i = h + 128 * q;
               if (i >= m) continue;
+              const double* row = tile + i;
+#pragma unroll 4
               for (int jj = 0; jj < cw; ++jj) {
-                const double v = tile[i + jj * ldt];
-                const T* xj = x + d.c0 + j0 + jj + (size_t)rc * ldx;
+                const double v = row[jj * ldt];
 #pragma unroll
-                for (int r = 0; r < kRhsChunk; ++r)
-                  if (r < nr) acc[q][r] += v * (double)xj[(size_t)r * ldx];
+                for (int r = 0; r < kR; ++r) acc[q][r] += v * xJ[jj + r * 256];
               }
             }
           } else if (mir) {
             for (int jj = h; jj < cw; jj += 128) {
-              double t[kRhsChunk];
+              double t[kR];
 #pragma unroll
-              for (int r = 0; r < kRhsChunk; ++r) t[r] = 0.0;
+              for (int r = 0; r < kR; ++r) t[r] = 0.0;
+              const double* col = tile + jj * ldt;
+#pragma unroll 4
               for (int i = 0; i < m; ++i) {
-                const double v = tile[i + jj * ldt];
-                const T* xi = x + d.r0 + i + (size_t)rc * ldx;
+                const double v = col[i];
 #pragma unroll
-                for (int r = 0; r < kRhsChunk; ++r)
-                  if (r < nr) t[r] += v * (double)xi[(size_t)r * ldx];
+                for (int r = 0; r < kR; ++r) t[r] += v * xI[i + r * 256];
               }
 #pragma unroll
-              for (int r = 0; r < kRhsChunk; ++r)
+              for (int r = 0; r < kR; ++r)
                 if (r < nr) slots[c.z + j0 + jj + (size_t)(rc + r) * d.n] = t[r];
             }
           }
@@ -146,18 +160,24 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
         const int ld = isA ? d.m : d.n;
         const T* P = arena + (isA ? d.a : d.b) + c.row0;
         const double* zz = z + c.z;
+        __syncthreads();
+        for (int e = tid; e < k * kR; e += kBlock) {
+          const int l = e % k, r = e / k;
+          xJ[l + r * 256] = r < nr ? zz[l + (size_t)(rc + r) * d.cap] : 0.0;
+        }
+        __syncthreads();
         const int kh = (k + 1) / 2;
         const int l0 = lo ? 0 : kh, l1 = lo ? kh : k;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           const int i = h + 128 * q;
           if (i >= it.len) continue;
+          const T* pi = P + i;
+#pragma unroll 4
           for (int l = l0; l < l1; ++l) {
-            const double v = (double)P[i + (size_t)l * ld];
-            const double* zl = zz + l + (size_t)rc * d.cap;
+            const double v = (double)pi[(size_t)l * ld];
 #pragma unroll
-            for (int r = 0; r < kRhsChunk; ++r)
-              if (r < nr) acc[q][r] += v * zl[(size_t)r * d.cap];
+            for (int r = 0; r < kR; ++r) acc[q][r] += v * xJ[l + r * 256];
           }
         }
       }
@@ -168,7 +188,7 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
 #pragma unroll
       for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int r = 0; r < kRhsChunk; ++r) tile[(h + 128 * q) * kRhsChunk + r] = acc[q][r];
+        for (int r = 0; r < kR; ++r) tile[(h + 128 * q) * kR + r] = acc[q][r];
     }
     __syncthreads();
     if (lo) {
@@ -177,8 +197,8 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
         const int i = h + 128 * q;
         if (i >= it.len) continue;
 #pragma unroll
-        for (int r = 0; r < kRhsChunk; ++r)
-          if (r < nr) acc_out[it.r0 + i + (size_t)(rc + r) * ldacc] = acc[q][r] + tile[i * kRhsChunk + r];
+        for (int r = 0; r < kR; ++r)
+          if (r < nr) acc_out[it.r0 + i + (size_t)(rc + r) * ldacc] = acc[q][r] + tile[i * kR + r];
       }
     }
   }
@@ -267,8 +287,12 @@ void launch_gemv_ypass(const LeafDesc* desc, const int* ranks, const T* arena, c
                        int count, const YContrib* contrib, const double* z, double* slots,
                        const T* x, int ldx, int nrhs, double* acc, int ldacc, hipStream_t s) {
   if (count == 0) return;
-  hipLaunchKernelGGL(k_ypass<T>, dim3(count), dim3(kBlock), 0, s, desc, ranks, arena, items,
-                     contrib, z, slots, x, ldx, nrhs, acc, ldacc);
+  if (nrhs == 1)
+    hipLaunchKernelGGL((k_ypass<T, 1>), dim3(count), dim3(kBlock), 0, s, desc, ranks, arena, items,
+                       contrib, z, slots, x, ldx, nrhs, acc, ldacc);
+  else
+    hipLaunchKernelGGL((k_ypass<T, kRhsChunk>), dim3(count), dim3(kBlock), 0, s, desc, ranks, arena,
+                       items, contrib, z, slots, x, ldx, nrhs, acc, ldacc);
   HMX_CHECK(hipGetLastError());
 }
 
