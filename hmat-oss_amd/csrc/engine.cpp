@@ -274,6 +274,8 @@ void Matrix::build_gemv_plan(int nrhs) {
   for (size_t q = 0; q < RL.size(); ++q) {
     if (RL[q]->size > 64 * 4) throw std::runtime_error("gemv: leaf clusters above 256 rows");
     YItem it{RL[q]->offset, RL[q]->size, (int)contrib.size(), 0, (int)slist.size(), 0};
+    // full leaves first, then the Rk slices, which the y pass stages in batches
+    std::stable_partition(per[q].begin(), per[q].end(), [](const YContrib& c) { return c.kind == 0; });
     contrib.insert(contrib.end(), per[q].begin(), per[q].end());
     slist.insert(slist.end(), slots_for[q].begin(), slots_for[q].end());
     it.end = (int)contrib.size();

@@ -67,18 +67,21 @@ __global__ void k_zreduce(const ZReduce* red, const int* ranks, const double* zp
   }
 }
 
-// y pass.  One workgroup per destination leaf cluster I (at most 256 rows):
-// thread t < 128 owns rows t and t + 128 of I for the whole list of
-// contributions.  A full leaf (I x J) is staged in LDS in column chunks with
-// coalesced loads (odd leading dimension), together with x_J; threads < 128
-// form M x_J row by row while, for a mirrored leaf, threads >= 128 form
-// M^T x_I column by column into the leaf's slot (x_I staged once), so neither
-// product needs a cross-lane reduction.  Rk slices (A z, or B z' of a
-// mirrored Rk leaf) stream from HBM with z staged in LDS, rows on threads
-// < 128 and the second half of the rank on threads >= 128.  kR right-hand
-// sides per pass.
-constexpr int kYTile = 6144;   // doubles of the leaf tile (48 KiB)
-constexpr int kYVec = 256 * 8; // doubles of each staged vector block (x_I, x_J / z)
+// y pass.  One workgroup per destination leaf cluster I (at most 256 rows).
+// The cluster's contribution list is first described in LDS (one parallel
+// load of its records), so the loops below issue data loads only.
+//  * Full leaves (I x J) are staged in LDS in column chunks with x_J; threads
+//    < 128 form M x_J (rows t, t + 128), and for a mirrored leaf threads >= 128
+//    form M^T x_I column by column into the leaf's slot.  No cross-lane
+//    reductions.
+//  * Rk slices (A z, or B z' of a mirrored Rk leaf) are taken four at a time,
+//    one per wavefront; a lane covers rows lane + 64 q of I, so one round trip
+//    to HBM serves four slices.
+// The partial sums of the row owners are added in a fixed order at the end.
+// kR right-hand sides per pass.
+constexpr int kYTile = 5120;   // doubles of the leaf tile (40 KiB)
+constexpr int kYVec = 256 * 8; // doubles of each staged vector block (x_I, x_J)
+constexpr int kYMeta = 128;    // contribution records described in LDS at a time
 
 template <typename T, int kR>
 __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const int* ranks,
@@ -88,11 +91,14 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
                                                   double* acc_out, int ldacc) {
   __shared__ double tile[kYTile];
   __shared__ double xI[kYVec];
-  __shared__ double xJ[kYVec];  // x_J of a full leaf, or z of an Rk slice
+  __shared__ double xJ[kYVec];
+  __shared__ const T* mp[kYMeta];      // full: leaf data; rk: first row of the slice
+  __shared__ const double* mz[kYMeta]; // rk: z; full mirrored: slot (as double*)
+  __shared__ int mkind[kYMeta], mk[kYMeta], mld[kYMeta], mcap[kYMeta], mc0[kYMeta], mn[kYMeta];
   const YItem it = items[blockIdx.x];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const bool lo = tid < 128;
-  const int h = lo ? tid : tid - 128;  // row (lo) or column / rank-split lane (hi)
+  const int h = lo ? tid : tid - 128;
   for (int rc = 0; rc < nrhs; rc += kR) {
     const int nr = nrhs - rc < kR ? nrhs - rc : kR;
     __syncthreads();
@@ -100,27 +106,65 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
       const int i = e % it.len, r = e / it.len;
       xI[i + r * 256] = r < nr ? (double)x[it.r0 + i + (size_t)(rc + r) * ldx] : 0.0;
     }
-    double acc[2][kR];
+    double accf[2][kR];  // full leaves: rows h, h + 128 (threads < 128)
+    double accr[4][kR];  // Rk slices: rows lane + 64 q
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
+    for (int r = 0; r < kR; ++r) {
+      accf[0][r] = accf[1][r] = 0.0;
 #pragma unroll
-      for (int r = 0; r < kR; ++r) acc[q][r] = 0.0;
-    for (int ci = it.beg; ci < it.end; ++ci) {
-      const YContrib c = contrib[ci];
-      const LeafDesc d = desc[c.leaf];
-      if (c.kind == 0) {
-        const T* M = arena + d.a;
-        const bool mir = c.z >= 0;
-        const int m = d.m, ldt = d.m | 1;
-        const int cw_max = kYTile / ldt < 256 ? kYTile / ldt : 256;  // x_J block holds 256 rows
-        for (int j0 = 0; j0 < d.n; j0 += cw_max) {
-          const int cw = d.n - j0 < cw_max ? d.n - j0 : cw_max;
+      for (int q = 0; q < 4; ++q) accr[q][r] = 0.0;
+    }
+    for (int cb = it.beg; cb < it.end; cb += kYMeta) {
+      const int nc = it.end - cb < kYMeta ? it.end - cb : kYMeta;
+      __syncthreads();
+      if (tid < nc) {
+        const YContrib c = contrib[cb + tid];
+        const LeafDesc d = desc[c.leaf];
+        mkind[tid] = c.kind;
+        if (c.kind == 0) {
+          mp[tid] = arena + d.a;
+          mz[tid] = c.z >= 0 ? slots + c.z : nullptr;
+          mk[tid] = d.m;
+          mn[tid] = d.n;
+          mc0[tid] = d.c0;
+        } else {
+          const bool isA = c.kind == 2;
+          mp[tid] = arena + (isA ? d.a : d.b) + c.row0;
+          mz[tid] = z + c.z;
+          mk[tid] = ranks[c.leaf];
+          mld[tid] = isA ? d.m : d.n;
+          mcap[tid] = d.cap;
+        }
+      }
+      __syncthreads();
+      int ci = 0;
+      // full leaves (the plan lists them first)
+      for (; ci < nc && mkind[ci] == 0; ++ci) {
+        const T* M = mp[ci];
+        const int m = mk[ci], n = mn[ci], c0 = mc0[ci];
+        double* slot = const_cast<double*>(mz[ci]);
+        const int ldt = m | 1;
+        const int cw_max = kYTile / ldt < 256 ? kYTile / ldt : 256;
+        for (int j0 = 0; j0 < n; j0 += cw_max) {
+          const int cw = n - j0 < cw_max ? n - j0 : cw_max;
           __syncthreads();
-          const T* src = M + (size_t)j0 * m;  // columns j0 .. j0+cw-1 are contiguous
-          for (int e = tid; e < m * cw; e += kBlock) tile[e % m + (e / m) * ldt] = (double)src[e];
+          const T* src = M + (size_t)j0 * m;
+          for (int base = tid; base < m * cw; base += 8 * kBlock) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int e = base + u * kBlock;
+              v[u] = e < m * cw ? (double)src[e] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int e = base + u * kBlock;
+              if (e < m * cw) tile[e % m + (e / m) * ldt] = v[u];
+            }
+          }
           for (int e = tid; e < cw * kR; e += kBlock) {
             const int j = e % cw, r = e / cw;
-            xJ[j + r * 256] = r < nr ? (double)x[d.c0 + j0 + j + (size_t)(rc + r) * ldx] : 0.0;
+            xJ[j + r * 256] = r < nr ? (double)x[c0 + j0 + j + (size_t)(rc + r) * ldx] : 0.0;
           }
           __syncthreads();
           if (lo) {
@@ -133,10 +177,10 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
               for (int jj = 0; jj < cw; ++jj) {
                 const double v = row[jj * ldt];
 #pragma unroll
-                for (int r = 0; r < kR; ++r) acc[q][r] += v * xJ[jj + r * 256];
+                for (int r = 0; r < kR; ++r) accf[q][r] += v * xJ[jj + r * 256];
               }
             }
-          } else if (mir) {
+          } else if (slot) {
             for (int jj = h; jj < cw; jj += 128) {
               double t[kR];
 #pragma unroll
@@ -150,56 +194,84 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
               }
 #pragma unroll
               for (int r = 0; r < kR; ++r)
-                if (r < nr) slots[c.z + j0 + jj + (size_t)(rc + r) * d.n] = t[r];
+                if (r < nr) slot[j0 + jj + (size_t)(rc + r) * n] = t[r];
             }
           }
         }
-      } else {
-        const int k = ranks[c.leaf];
-        const bool isA = c.kind == 2;
-        const int ld = isA ? d.m : d.n;
-        const T* P = arena + (isA ? d.a : d.b) + c.row0;
-        const double* zz = z + c.z;
-        __syncthreads();
-        for (int e = tid; e < k * kR; e += kBlock) {
-          const int l = e % k, r = e / k;
-          xJ[l + r * 256] = r < nr ? zz[l + (size_t)(rc + r) * d.cap] : 0.0;
-        }
-        __syncthreads();
-        const int kh = (k + 1) / 2;
-        const int l0 = lo ? 0 : kh, l1 = lo ? kh : k;
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int i = h + 128 * q;
-          if (i >= it.len) continue;
-          const T* pi = P + i;
+      }
+      // Rk slices: wavefront wv takes slices ci + wv, ci + wv + 4, ...; with
+      // one right-hand side it streams two slices together (twice the loads
+      // in flight), with more the accumulators use the registers instead
+      if constexpr (kR > 1) {
+        for (int cj = ci + wv; cj < nc; cj += kWaves) {
+          const T* P = mp[cj];
+          const double* zz = mz[cj] + (size_t)rc * mcap[cj];
+          const int k = mk[cj], ld = mld[cj], cap = mcap[cj];
 #pragma unroll 4
-          for (int l = l0; l < l1; ++l) {
-            const double v = (double)pi[(size_t)l * ld];
+          for (int l = 0; l < k; ++l) {
+            double zl[kR];
 #pragma unroll
-            for (int r = 0; r < kR; ++r) acc[q][r] += v * xJ[l + r * 256];
+            for (int r = 0; r < kR; ++r) zl[r] = r < nr ? zz[l + (size_t)r * cap] : 0.0;
+            const T* pl = P + (size_t)l * ld;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int i = lane + 64 * q;
+              if (i < it.len) {
+                const double v = (double)pl[i];
+#pragma unroll
+                for (int r = 0; r < kR; ++r) accr[q][r] += v * zl[r];
+              }
+            }
+          }
+        }
+      } else
+      for (int cj = ci + wv; cj < nc; cj += 2 * kWaves) {
+        const int cj2 = cj + kWaves < nc ? cj + kWaves : cj;
+        const T* P0 = mp[cj];
+        const T* P1 = mp[cj2];
+        const double* z0 = mz[cj] + (size_t)rc * mcap[cj];
+        const double* z1 = mz[cj2] + (size_t)rc * mcap[cj2];
+        const int k0 = mk[cj], ld0 = mld[cj], cap0 = mcap[cj];
+        const int k1 = cj2 != cj ? mk[cj2] : 0, ld1 = mld[cj2], cap1 = mcap[cj2];
+        const int kmax = k0 > k1 ? k0 : k1;
+#pragma unroll 2
+        for (int l = 0; l < kmax; ++l) {
+          double za[kR], zb[kR];
+#pragma unroll
+          for (int r = 0; r < kR; ++r) {
+            za[r] = (l < k0 && r < nr) ? z0[l + (size_t)r * cap0] : 0.0;
+            zb[r] = (l < k1 && r < nr) ? z1[l + (size_t)r * cap1] : 0.0;
+          }
+          const T* pa = P0 + (size_t)(l < k0 ? l : 0) * ld0;
+          const T* pb = P1 + (size_t)(l < k1 ? l : 0) * ld1;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int i = lane + 64 * q;
+            if (i < it.len) {
+              const double va = l < k0 ? (double)pa[i] : 0.0;
+              const double vb = l < k1 ? (double)pb[i] : 0.0;
+#pragma unroll
+              for (int r = 0; r < kR; ++r) accr[q][r] += va * za[r] + vb * zb[r];
+            }
           }
         }
       }
     }
-    // rows: the row owner's sum plus the rank half accumulated by its partner
-    __syncthreads();
-    if (!lo) {
+    // fixed-order sum of the partials: full part (threads < 128) + the four waves
 #pragma unroll
-      for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int r = 0; r < kR; ++r) tile[(h + 128 * q) * kR + r] = acc[q][r];
-    }
-    __syncthreads();
-    if (lo) {
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int i = h + 128 * q;
-        if (i >= it.len) continue;
-#pragma unroll
-        for (int r = 0; r < kR; ++r)
-          if (r < nr) acc_out[it.r0 + i + (size_t)(rc + r) * ldacc] = acc[q][r] + tile[i * kR + r];
+    for (int r = 0; r < kR; ++r) {
+      __syncthreads();
+      if (lo) {
+        tile[4 * 256 + h] = accf[0][r];
+        tile[4 * 256 + h + 128] = accf[1][r];
       }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) tile[wv * 256 + lane + 64 * q] = accr[q][r];
+      __syncthreads();
+      if (r < nr)
+        for (int i = tid; i < it.len; i += kBlock)
+          acc_out[it.r0 + i + (size_t)(rc + r) * ldacc] =
+              tile[4 * 256 + i] + ((tile[i] + tile[256 + i]) + (tile[512 + i] + tile[768 + i]));
     }
   }
 }
