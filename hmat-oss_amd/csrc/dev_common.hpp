@@ -47,6 +47,40 @@ __host__ __device__ inline double kernel_value(const KernelParams& k, double dx,
   return v;
 }
 
+// ---- staging global -> LDS ----------------------------------------------------
+// Copies an m x n block into LDS, dst[i + j * ldd] = src[i * si + j * sj] for
+// i < mv, j < nv and 0 elsewhere.  Each thread issues kStage independent loads
+// before storing them: a plain "for (e...) lds[..] = global[..]" loop waits
+// for every load before its store, one memory round trip per iteration.
+constexpr int kStage = 8;
+
+template <typename D, typename S>
+__device__ inline void stage_in(D* dst, int ldd, const S* src, long si, long sj, int m, int n,
+                                int mv, int nv, double scale = 1.0) {
+  const int total = m * n;
+  for (int base = threadIdx.x; base < total; base += kStage * kBlock) {
+    D v[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int e = base + u * kBlock;
+      v[u] = (D)0;
+      if (e < total) {
+        const int i = e % m, j = e / m;
+        if (i < mv && j < nv) v[u] = (D)(scale * (double)src[i * si + j * sj]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; ++u) {
+      const int e = base + u * kBlock;
+      if (e < total) dst[e % m + (e / m) * ldd] = v[u];
+    }
+  }
+}
+template <typename D, typename S>
+__device__ inline void stage_in(D* dst, int ldd, const S* src, long si, long sj, int m, int n) {
+  stage_in(dst, ldd, src, si, sj, m, n, m, n);
+}
+
 // ---- reductions -------------------------------------------------------------
 // Lane exchange through DPP (a VALU operand modifier, no LDS round trip).
 // kCtrl: 0xB1 swaps neighbours (quad_perm 1,0,3,2), 0x4E swaps pairs

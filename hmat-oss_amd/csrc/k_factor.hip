@@ -22,7 +22,7 @@ __device__ inline int rank_of(const RkRef& r) {
 // ---------------------------------------------------------------------------
 // GEMM: C (m x n, ldc) = beta C + alpha op(A) op(B), 64 x 64 tiles.
 // ---------------------------------------------------------------------------
-constexpr int TM = 64, TN = 64, TK = 16;
+constexpr int TM = 64, TN = 64, TK = 32;
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_gemm(const FTask* tasks, const GemmTile* tiles,
@@ -45,20 +45,16 @@ __global__ void __launch_bounds__(256) k_gemm(const FTask* tasks, const GemmTile
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = 0;
   for (int k0 = 0; k0 < P; k0 += TK) {
-    for (int e = threadIdx.x; e < TK * TM; e += 256) {
-      const int ii = e % TM, kk = e / TM;
-      const int i = i0 + ii, k = k0 + kk;
-      T v = 0;
-      if (i < M && k < P) v = t.ta ? A[k + (size_t)i * t.lda] : A[i + (size_t)k * t.lda];
-      As[kk][ii] = v;
-    }
-    for (int e = threadIdx.x; e < TK * TN; e += 256) {
-      const int kk = e % TK, jj = e / TK;
-      const int j = j0 + jj, k = k0 + kk;
-      T v = 0;
-      if (j < N && k < P) v = t.tb ? B[j + (size_t)k * t.ldb] : B[k + (size_t)j * t.ldb];
-      Bs[kk][jj] = v;
-    }
+    // As[kk][ii] = op(A)(i0 + ii, k0 + kk), Bs[kk][jj] = op(B)(k0 + kk, j0 + jj)
+    const long la = t.lda, lb = t.ldb;
+    if (t.ta)
+      stage_in(&As[0][0], TM + 1, A + k0 + (size_t)i0 * la, la, 1L, TM, TK, M - i0, P - k0);
+    else
+      stage_in(&As[0][0], TM + 1, A + i0 + (size_t)k0 * la, 1L, la, TM, TK, M - i0, P - k0);
+    if (t.tb)
+      stage_in(&Bs[0][0], TN + 1, B + j0 + (size_t)k0 * lb, 1L, lb, TN, TK, N - j0, P - k0);
+    else
+      stage_in(&Bs[0][0], TN + 1, B + k0 + (size_t)j0 * lb, lb, 1L, TN, TK, N - j0, P - k0);
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < TK; ++kk) {
@@ -142,10 +138,7 @@ __global__ void __launch_bounds__(kBlock) k_potrf(const FTask* tasks, int* info)
   T* A = (T*)t.A;
   const int lda = t.lda;
   if (n <= kLdsDense) {
-    for (int e = threadIdx.x; e < n * n; e += kBlock) {
-      const int i = e % n, j = e / n;
-      lds[i + j * n] = A[i + (size_t)j * lda];
-    }
+    stage_in(lds, n, A, 1L, (long)lda, n, n);
     __syncthreads();
     potrf_body(lds, n, n, info);
     for (int e = threadIdx.x; e < n * n; e += kBlock) {
@@ -231,7 +224,7 @@ __global__ void __launch_bounds__(kBlock) k_getrf(const FTask* tasks, int* info)
   const int lda = t.lda;
   int* piv = (int*)t.B;
   if (n <= kLdsDense) {
-    for (int e = threadIdx.x; e < n * n; e += kBlock) lds[e] = A[e % n + (size_t)(e / n) * lda];
+    stage_in(lds, n, A, 1L, (long)lda, n, n);
     __syncthreads();
     getrf_body(lds, n, n, piv, info);
     __syncthreads();
@@ -416,14 +409,11 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
   const int ldl = t.lda;
   T* X = (T*)t.C;
   const int ldx = t.ldc;
-  for (int e = threadIdx.x; e < n * n; e += kBlock) {
-    const int i = e % n, j = e / n;
-    Ls[e] = transa ? (double)Lg[j + (size_t)i * ldl] : (double)Lg[i + (size_t)j * ldl];
-  }
-  for (int e = threadIdx.x; e < xr * xc; e += kBlock) {
-    const int i = e % xr, j = e / xr;
-    Xs[i + j * xl] = (double)X[i + (size_t)j * ldx];
-  }
+  if (transa)
+    stage_in(Ls, n, Lg, (long)ldl, 1L, n, n);
+  else
+    stage_in(Ls, n, Lg, 1L, (long)ldl, n, n);
+  stage_in(Xs, xl, X, 1L, (long)ldx, xr, xc);
   __syncthreads();
   if ((t.mode & kTrsmPiv) && mode != 0) {
     const int* piv = (const int*)t.B;
@@ -641,10 +631,7 @@ __device__ void concat_side(double* dst, int ld, int rows, bool bside, const FTa
   const RkRef& tg = t.tgt;
   const T* src = (const T*)(bside ? tg.b : tg.a);
   const int lds = bside ? tg.lb : tg.la;
-  for (int e = threadIdx.x; e < rows * kt; e += kBlock) {
-    const int i = e % rows, c = e / rows;
-    dst[i + (size_t)c * ld] = (double)src[i + (size_t)c * lds];
-  }
+  stage_in(dst, ld, src, 1L, (long)lds, rows, kt);
   int off = kt;
   for (int q = 0; q < t.nparts; ++q) {
     const Part& p = t.parts[q];
@@ -654,9 +641,17 @@ __device__ void concat_side(double* dst, int ld, int rows, bool bside, const FTa
     const int ldp = bside ? p.rk.lb : p.rk.la;
     const int o = bside ? p.coff : p.roff, pr = bside ? p.rk.n : p.rk.m;
     const double f = bside ? 1.0 : p.alpha;
-    for (int e = threadIdx.x; e < rows * kq; e += kBlock) {
-      const int i = e % rows, c = e / rows, ii = i - o;
-      dst[i + (size_t)(off + c) * ld] = (ii >= 0 && ii < pr) ? f * (double)P[ii + (size_t)c * ldp] : 0.0;
+    // the part covers rows [o, o + pr) of the target; zero elsewhere
+    const int r_lo = o > 0 ? o : 0, r_hi = o + pr < rows ? o + pr : rows;
+    double* col0 = dst + (size_t)off * ld;
+    if (r_hi > r_lo)
+      stage_in(col0 + r_lo, ld, P + (r_lo - o), 1L, (long)ldp, r_hi - r_lo, kq, r_hi - r_lo, kq, f);
+    const int nz = rows - (r_hi > r_lo ? r_hi - r_lo : 0);
+    for (int e = threadIdx.x; e < nz * kq; e += kBlock) {
+      int i = e % nz;
+      const int c = e / nz;
+      if (r_hi > r_lo && i >= r_lo) i += r_hi - r_lo;
+      col0[i + (size_t)c * ld] = 0.0;
     }
     off += kq;
   }
