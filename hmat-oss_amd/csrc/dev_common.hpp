@@ -96,6 +96,16 @@ __device__ inline float dpp_swap(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
 }
 
+// Value of a lane, read into every lane (readlane; ignores the exec mask).
+__device__ inline double lane_read(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float lane_read(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 // Sum over aligned groups of g lanes (g a power of two <= 64, uniform across
 // the wavefront; every lane of a group must be active).  Each step adds the
 // value of a partner lane, so every lane of a group ends with the same,
@@ -106,20 +116,21 @@ __device__ inline T group_sum(T v, int g) {
   if (g >= 4) v += dpp_swap<0x4E>(v);
   if (g >= 8) v += dpp_swap<0x141>(v);
   if (g >= 16) v += dpp_swap<0x140>(v);
+  if (g == 64) return (lane_read(v, 0) + lane_read(v, 16)) + (lane_read(v, 32) + lane_read(v, 48));
   if (g >= 32) v += __shfl_xor(v, 16, kWave);
-  if (g >= 64) v += __shfl_xor(v, 32, kWave);
   return v;
 }
 
+// Sum over the whole wavefront (all 64 lanes active): DPP sums within each
+// row of 16 lanes, then the four row sums are read out with readlane and
+// added in a fixed order, so every lane returns the same value.
 template <typename T>
 __device__ inline T wave_sum(T v) {
   v += dpp_swap<0xB1>(v);
   v += dpp_swap<0x4E>(v);
   v += dpp_swap<0x141>(v);
   v += dpp_swap<0x140>(v);
-  v += __shfl_xor(v, 16, kWave);
-  v += __shfl_xor(v, 32, kWave);
-  return v;
+  return (lane_read(v, 0) + lane_read(v, 16)) + (lane_read(v, 32) + lane_read(v, 48));
 }
 
 // Sum over the whole workgroup (blockDim.x == kBlock).  All threads get the result.

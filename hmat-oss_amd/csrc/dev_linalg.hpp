@@ -136,6 +136,128 @@ __device__ inline void team_house_qr(double* A, int lda, int m, int k, double* t
   team_sync<kLanes>();
 }
 
+// Householder QR as team_house_qr with row ownership: thread t of the team
+// keeps rows t + kLanes * r (r < kQrRows) of the current column in registers,
+// and trailing columns are updated four at a time, their loads issued
+// together and their four dot products reduced in one exchange.  A step thus
+// costs a few memory round trips whether the panel lives in LDS or in global
+// memory.  Requires m <= kQrRows * kLanes; 'red' holds 4 * kWaves doubles of
+// shared scratch (workgroup team only).
+constexpr int kQrRows = 8;
+
+template <int kLanes, int kC>
+__device__ inline void team_sum_n(double (&v)[kC], double* red) {
+#pragma unroll
+  for (int c = 0; c < kC; ++c) v[c] = wave_sum(v[c]);
+  if constexpr (kLanes != kWave) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+      for (int c = 0; c < kC; ++c) red[w * kC + c] = v[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kC; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < kWaves; ++q) s += red[q * kC + c];
+      v[c] = s;
+    }
+  }
+}
+
+template <int kLanes, int R>
+__device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, double* tau) {
+  __shared__ double red[4 * kWaves];
+  const int tid = team_tid<kLanes>();
+  const int steps = m < k ? m : k;
+  // rows of this thread; loads use a clamped row and a select instead of a
+  // branch, so all of them are in flight together
+  int row[R], rc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    row[r] = tid + kLanes * r;
+    rc[r] = row[r] < m ? row[r] : m - 1;
+  }
+  for (int j = 0; j < steps; ++j) {
+    double* vcol = A + (size_t)j * lda;
+    double v[R];
+    double s1[1] = {0.0};
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double x = vcol[rc[r]];
+      v[r] = (row[r] > j && row[r] < m) ? x : 0.0;
+      s1[0] += v[r] * v[r];
+    }
+    team_sum_n<kLanes, 1>(s1, red);
+    const double s = s1[0];
+    const double alpha = vcol[j];
+    double t = 0.0, beta = alpha;
+    if (s != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + s), alpha);
+      t = (beta - alpha) / beta;
+      const double scal = 1.0 / (alpha - beta);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        v[r] *= scal;
+        if (row[r] > j && row[r] < m) vcol[row[r]] = v[r];
+      }
+    }
+    team_sync<kLanes>();
+    if (tid == 0) {
+      tau[j] = t;
+      vcol[j] = beta;
+    }
+    if (t != 0.0) {
+      for (int c0 = j + 1; c0 < k; c0 += 4) {
+        double a[4][R];
+        double d[4], aj[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int cc = c0 + c < k ? c0 + c : k - 1;
+          const double* ac = A + (size_t)cc * lda;
+          aj[c] = ac[j];  // read by all before thread 0 rewrites it
+#pragma unroll
+          for (int r = 0; r < R; ++r) a[c][r] = ac[rc[r]];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          d[c] = 0.0;
+#pragma unroll
+          for (int r = 0; r < R; ++r) d[c] += v[r] * a[c][r];  // v is 0 outside rows (j, m)
+        }
+        team_sum_n<kLanes, 4>(d, red);
+        team_sync<kLanes>();
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (c0 + c >= k) continue;
+          double* ac = A + (size_t)(c0 + c) * lda;
+          const double f = t * (d[c] + aj[c]);
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            if (row[r] > j && row[r] < m) ac[row[r]] = a[c][r] - f * v[r];
+          if (tid == 0) ac[j] = aj[c] - f;
+        }
+      }
+    }
+    team_sync<kLanes>();
+  }
+  if (tid == 0)
+    for (int j = steps; j < k; ++j) tau[j] = 0.0;
+  team_sync<kLanes>();
+}
+
+// Picks the rows-per-thread count for m (m <= kQrRows * kLanes).
+template <int kLanes>
+__device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, double* tau) {
+  if (m <= 2 * kLanes)
+    team_house_qr_rows<kLanes, 2>(A, lda, m, k, tau);
+  else if (m <= 4 * kLanes)
+    team_house_qr_rows<kLanes, 4>(A, lda, m, k, tau);
+  else
+    team_house_qr_rows<kLanes, kQrRows>(A, lda, m, k, tau);
+}
+
 // Y (m x r, ldy) <- Q Y with Q = H_0 ... H_{nref-1} from team_house_qr;
 // column c of Y belongs to lane group c mod (kLanes / g).
 template <int kLanes>
@@ -203,7 +325,10 @@ __device__ inline void team_larft(const double* V, int ldv, int m, int k, const 
 // Workgroup-wide wrappers (the workgroup's own reduction scratch is unused).
 __device__ inline void house_qr(double* A, int lda, int m, int k, double* tau, double* = nullptr,
                                 double* = nullptr) {
-  team_house_qr<kBlock>(A, lda, m, k, tau);
+  if (m <= kQrRows * kBlock)
+    team_house_qr_rows<kBlock>(A, lda, m, k, tau);
+  else
+    team_house_qr<kBlock>(A, lda, m, k, tau);
 }
 __device__ inline void apply_q(const double* V, int ldv, int m, int nref, const double* tau,
                                double* Y, int ldy, int r, double* = nullptr, double* = nullptr) {
@@ -211,7 +336,10 @@ __device__ inline void apply_q(const double* V, int ldv, int m, int nref, const 
 }
 // Single-wavefront versions (operands in LDS).
 __device__ inline void wave_house_qr(double* A, int lda, int m, int k, double* tau) {
-  team_house_qr<kWave>(A, lda, m, k, tau);
+  if (m <= kQrRows * kWave)
+    team_house_qr_rows<kWave>(A, lda, m, k, tau);
+  else
+    team_house_qr<kWave>(A, lda, m, k, tau);
 }
 __device__ inline void wave_apply_q(const double* V, int ldv, int m, int nref, const double* tau,
                                     double* Y, int ldy, int r) {
