@@ -354,9 +354,54 @@ class Exec {
   static Dim dim(int s, const int* d = nullptr) { return Dim{s, d}; }
 
   // C (+)= alpha op(A) op(B); A, B dense views (op from their trans flag).
+  // GEMMs with a long static inner dimension and a small output (the z = B^T X
+  // of a large Rk panel) are split along the inner dimension: independent
+  // partial products into a temporary, then a fixed-order sum.
+  static constexpr int kSplitP = 4096, kChunkP = 2048;
+
   void gemm(const DV& C, bool accumulate, double alpha, const DV& A, const DV& B, Dim m, Dim n,
             Dim p) {
     if (m.s <= 0 || n.s <= 0) return;
+    if (p.d == nullptr && p.s > kSplitP && (int64_t)m.s * n.s <= 256 * 256) {
+      const int nch = (p.s + kChunkP - 1) / kChunkP;
+      DV tmp = temp_dense(m.s, n.s * nch, -1);
+      std::vector<DV> slice(nch);
+      for (int q = 0; q < nch; ++q) {
+        const int p0 = q * kChunkP, pc = std::min(kChunkP, p.s - p0);
+        DV As = A, Bs = B;
+        As.p = (T*)A.p + (A.trans ? (size_t)p0 : (size_t)p0 * A.ld);
+        Bs.p = (T*)B.p + (B.trans ? (size_t)p0 * B.ld : (size_t)p0);
+        // each slice is its own dependency unit, so the partial products share a wave
+        slice[q] = tmp;
+        slice[q].p = (T*)tmp.p + (size_t)q * m.s * n.s;
+        slice[q].n = n.s;
+        slice[q].buf = kTempBase + fs_.next_temp++;
+        gemm_task(slice[q], false, alpha, As, Bs, m, n, dim(pc), A, B, slice[q]);
+      }
+      FTask t{};
+      t.type = T_SUMK;
+      t.mode = accumulate ? 1 : 0;
+      t.m = m;
+      t.n = n;
+      t.p = dim(nch);
+      t.A = tmp.p;
+      t.lda = m.s;
+      t.ldb = m.s * n.s;
+      t.C = C.p;
+      t.ldc = C.ld;
+      for (const DV& d : slice) rd(d);
+      if (accumulate) rd(C);
+      wr(C);
+      emit(t);
+      release();
+      return;
+    }
+    gemm_task(C, accumulate, alpha, A, B, m, n, p, A, B, C);
+  }
+
+  // One GEMM task; rA, rB, wC name the buffers whose dependency keys it uses.
+  void gemm_task(const DV& C, bool accumulate, double alpha, const DV& A, const DV& B, Dim m, Dim n,
+                 Dim p, const DV& rA, const DV& rB, const DV& wC) {
     FTask t{};
     t.type = T_GEMM;
     t.mode = accumulate ? 1 : 0;
@@ -372,10 +417,10 @@ class Exec {
     t.lda = A.ld;
     t.ldb = B.ld;
     t.ldc = C.ld;
-    rd(A);
-    rd(B);
-    if (accumulate) rd(C);
-    wr(C);
+    rd(rA);
+    rd(rB);
+    if (accumulate) rd(wC);
+    wr(wC);
     emit(t);
   }
 

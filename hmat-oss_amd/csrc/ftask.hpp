@@ -22,6 +22,7 @@ enum TaskType : int {
   T_ZERO,         // C = 0
   T_AXPY,         // C += alpha op(A)
   T_GETRF,        // A <- P L U in place (L unit lower), row interchanges into B (int*)
+  T_SUMK,         // C (+)= sum over p.s slices of A (ld lda, slice stride ldb): split-K GEMM
   T_NTYPES
 };
 

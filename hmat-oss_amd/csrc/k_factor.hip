@@ -528,6 +528,26 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
 }
 
 // ---------------------------------------------------------------------------
+// SUMK: C (m x n) = (mode ? C : 0) + sum_q W_q, the partial products of a
+// GEMM split along its inner dimension, added in a fixed order.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_sumk(const FTask* tasks, const int* fail) {
+  if (*fail) return;
+  const FTask& t = tasks[blockIdx.x];
+  const int M = eff(t.m), N = eff(t.n), S = t.p.s;
+  const T* W = (const T*)t.A;
+  T* C = (T*)t.C;
+  for (int e = threadIdx.x; e < M * N; e += blockDim.x) {
+    const int i = e % M, j = e / M;
+    T* pc = C + i + (size_t)j * t.ldc;
+    double s = t.mode ? (double)*pc : 0.0;
+    for (int q = 0; q < S; ++q) s += (double)W[i + (size_t)j * t.lda + (size_t)q * t.ldb];
+    *pc = (T)s;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // zero fill and axpy
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1177,6 +1197,9 @@ void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile*
       break;
     case T_GETRF:
       hipLaunchKernelGGL(k_getrf<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
+      break;
+    case T_SUMK:
+      hipLaunchKernelGGL(k_sumk<T>, dim3(count), dim3(256), 0, s, tasks, flags + 1);
       break;
     default:
       throw std::runtime_error("unknown task type");
