@@ -482,19 +482,161 @@ __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double
   return sweep;
 }
 
+// team_jacobi with the pair's rows held in registers: lane 'sub' of a group
+// owns rows sub + g r (r < R) of G and of V; the loads use clamped rows and a
+// select, the columns are read once per round and written back after the
+// rotation.  Requires rows, cols <= R g and tracked norms.
+template <int kLanes, int R>
+__device__ inline int team_jacobi_rows(double* G, int ldg, int rows, int cols, double* V, int ldv,
+                                       double* sh2, double* nrm, int max_sweeps) {
+  const int tid = team_tid<kLanes>();
+  for (int e = tid; e < cols * cols; e += kLanes) {
+    const int i = e % cols, j = e / cols;
+    V[i + (size_t)j * ldv] = (i == j) ? 1.0 : 0.0;
+  }
+  if (cols < 2) {
+    team_sync<kLanes>();
+    return 0;
+  }
+  const double tol2 = 1e-26;
+  const int np = (cols + 1) & ~1, rounds = np - 1, half = np / 2;
+  const int g = group_lanes(half, kLanes), ngrp = kLanes / g, grp = tid / g, sub = tid % g;
+  int rg[R], rgc[R], rv[R], rvc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    rg[r] = sub + g * r;
+    rgc[r] = rg[r] < rows ? rg[r] : rows - 1;
+    rv[r] = sub + g * r;
+    rvc[r] = rv[r] < cols ? rv[r] : cols - 1;
+  }
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    team_sync<kLanes>();
+    for (int c = grp; c < cols; c += ngrp) {
+      double a = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double x = G[rgc[r] + (size_t)c * ldg];
+        a += rg[r] < rows ? x * x : 0.0;
+      }
+      a = group_sum(a, g);
+      if (sub == 0) nrm[c] = a;
+    }
+    team_sync<kLanes>();
+    double f = 0.0;
+    for (int c = 0; c < cols; ++c) f += nrm[c];
+    const double floor2 = 1e-30 * f;
+    int rotated = 0;
+    for (int rd = 0; rd < rounds; ++rd) {
+      for (int pk = grp; pk < half; pk += ngrp) {
+        int p, q;
+        if (pk == 0) {
+          p = np - 1;
+          q = rd;
+        } else {
+          p = rd + pk;
+          if (p >= np - 1) p -= np - 1;
+          q = rd - pk + (np - 1);
+          if (q >= np - 1) q -= np - 1;
+        }
+        if (p > q) {
+          const int tmp = p;
+          p = q;
+          q = tmp;
+        }
+        if (q >= cols) continue;
+        double* gp = G + (size_t)p * ldg;
+        double* gq = G + (size_t)q * ldg;
+        double* vp = V + (size_t)p * ldv;
+        double* vq = V + (size_t)q * ldv;
+        // all loads of the pair issued before the first wait
+        double xp[R], xq[R], up[R], uq[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          xp[r] = gp[rgc[r]];
+          xq[r] = gq[rgc[r]];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          up[r] = vp[rvc[r]];
+          uq[r] = vq[rvc[r]];
+        }
+        const double a = nrm[p], b = nrm[q];
+        double c2 = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) c2 += rg[r] < rows ? xp[r] * xq[r] : 0.0;
+        c2 = group_sum(c2, g);
+        if (!(c2 * c2 > tol2 * a * b) || !(fabs(c2) > floor2)) continue;
+        const double d = b - a;
+        const double t = (d < 0.0 ? -2.0 * c2 : 2.0 * c2) / (fabs(d) + sqrt(d * d + 4.0 * c2 * c2));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (rg[r] < rows) {
+            gp[rg[r]] = c * xp[r] - s * xq[r];
+            gq[rg[r]] = s * xp[r] + c * xq[r];
+          }
+          if (rv[r] < cols) {
+            vp[rv[r]] = c * up[r] - s * uq[r];
+            vq[rv[r]] = s * up[r] + c * uq[r];
+          }
+        }
+        if (sub == 0) {
+          nrm[p] = a - t * c2;
+          nrm[q] = b + t * c2;
+        }
+        rotated = 1;
+      }
+      team_sync<kLanes>();
+    }
+    int any;
+    if constexpr (kLanes == kWave) {
+      any = __ballot(rotated) != 0;
+    } else {
+      if (threadIdx.x == 0) sh2[1] = 0.0;
+      __syncthreads();
+      if (rotated) sh2[1] = 1.0;
+      __syncthreads();
+      any = sh2[1] != 0.0;
+      __syncthreads();
+    }
+    if (!any) {
+      ++sweep;
+      break;
+    }
+  }
+  team_sync<kLanes>();
+  return sweep;
+}
+
+// Dispatch: register rows when rows and cols fit R * g for R <= 8.
+template <int kLanes>
+__device__ inline int team_jacobi_any(double* G, int ldg, int rows, int cols, double* V, int ldv,
+                                      double* sh2, double* nrm, int max_sweeps) {
+  if (nrm != nullptr && cols <= kJacobiMaxCols && cols >= 2 && rows >= 1) {
+    const int half = ((cols + 1) & ~1) / 2;
+    const int g = group_lanes(half, kLanes);
+    const int need2 = ((rows > cols ? rows : cols) + g - 1) / g;
+    if (need2 <= 2) return team_jacobi_rows<kLanes, 2>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+    if (need2 <= 4) return team_jacobi_rows<kLanes, 4>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+    if (need2 <= 8) return team_jacobi_rows<kLanes, 8>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+  }
+  return team_jacobi<kLanes>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+}
+
 // Workgroup version used by the existing call sites (cols <= kJacobiMaxCols).
 __device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
                                   int* /*flag*/, int max_sweeps = 40) {
   __shared__ double sh2[2 + kWaves];
   __shared__ double nrm[kJacobiMaxCols];
-  team_jacobi<kBlock>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+  team_jacobi_any<kBlock>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
 }
 
 // Single-wavefront version (G, V in LDS; cols <= kJacobiMaxCols).
 __device__ inline int wave_jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
                                       int max_sweeps = 40) {
   __shared__ double nrm[kJacobiMaxCols];
-  return team_jacobi<kWave>(G, ldg, rows, cols, V, ldv, nullptr, nrm, max_sweeps);
+  return team_jacobi_any<kWave>(G, ldg, rows, cols, V, ldv, nullptr, nrm, max_sweeps);
 }
 
 // Column norms of G into sig[0..cols), then order[] = permutation sorting

@@ -691,16 +691,23 @@ __device__ inline void core_product(const double* Ra, int lra, const double* Rb,
 
 // out (rows x nk, ld ldo) = Q [U; 0] with Q = I - V Tq V^T (rows x K panel of
 // reflectors V, ld ldv) and U[i, c] = src[i, order[c]] * scale[c] (K x nk,
-// ld K): X = V1^T U, W = Tq X, out = [U; 0] - V W.  Every output element is
-// an independent K-term sum.
+// ld K): X = V1^T U, W = Tq X (K x nk, from the unit lower K x K top V1 of V
+// staged in w.w), then out = [U; 0] - V W with each thread owning rows and
+// streaming a row of V once per eight output columns.
 template <typename T>
 __device__ void wy_apply_out(const double* V, int ldv, int rows, int K, const double* Tq,
                              const double* src, const QrScratch& w, int nk, T* out, int ldo) {
+  for (int e = threadIdx.x; e < K * K; e += kBlock) {
+    const int i = e % K, l = e / K;
+    w.w[e] = i > l ? V[i + (size_t)l * ldv] : (i == l ? 1.0 : 0.0);
+  }
+  __syncthreads();
   for (int e = threadIdx.x; e < K * nk; e += kBlock) {
     const int l = e % K, c = e / K;
     const double* u = src + (size_t)w.order[c] * K;
-    double s = u[l];
-    for (int i = l + 1; i < K; ++i) s += V[i + (size_t)l * ldv] * u[i];
+    const double* vl = w.w + (size_t)l * K;
+    double s = 0.0;
+    for (int i = l; i < K; ++i) s += vl[i] * u[i];
     w.x[l + (size_t)c * K] = s * w.scale[c];
   }
   __syncthreads();
@@ -708,18 +715,43 @@ __device__ void wy_apply_out(const double* V, int ldv, int rows, int K, const do
     const int l = e % K, c = e / K;
     double s = 0.0;
     for (int r = l; r < K; ++r) s += Tq[l + (size_t)r * K] * w.x[r + (size_t)c * K];
-    w.w[l + (size_t)c * K] = s;
+    w.w[l + (size_t)c * K] = s;  // the staged V1 is no longer read
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < rows * nk; e += kBlock) {
-    const int i = e % rows, c = e / rows;
-    const double* wc = w.w + (size_t)c * K;
-    double s = 0.0;
-    const int lmax = i < K ? i : K;
-    for (int l = 0; l < lmax; ++l) s += V[i + (size_t)l * ldv] * wc[l];
-    double y = -s;
-    if (i < K) y += src[i + (size_t)w.order[c] * K] * w.scale[c] - wc[i];
-    out[i + (size_t)c * ldo] = (T)y;
+  constexpr int kC = 8;
+  for (int c0 = 0; c0 < nk; c0 += kC) {
+    const double* wc[kC];
+#pragma unroll
+    for (int b = 0; b < kC; ++b) wc[b] = w.w + (size_t)(c0 + b < nk ? c0 + b : nk - 1) * K;
+    for (int i = threadIdx.x; i < rows; i += kBlock) {
+      double acc[kC];
+#pragma unroll
+      for (int b = 0; b < kC; ++b) acc[b] = 0.0;
+      const int lmax = i < K ? i : K;  // entries strictly below the unit diagonal
+      const double* vi = V + i;
+      int l = 0;
+      for (; l + 8 <= lmax; l += 8) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = vi[(size_t)(l + q) * ldv];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int b = 0; b < kC; ++b) acc[b] += v[q] * wc[b][l + q];
+      }
+      for (; l < lmax; ++l) {
+        const double v = vi[(size_t)l * ldv];
+#pragma unroll
+        for (int b = 0; b < kC; ++b) acc[b] += v * wc[b][l];
+      }
+#pragma unroll
+      for (int b = 0; b < kC; ++b) {
+        if (c0 + b >= nk) break;
+        double y = -acc[b];
+        if (i < K) y += src[i + (size_t)w.order[c0 + b] * K] * w.scale[c0 + b] - wc[b][i];
+        out[i + (size_t)(c0 + b) * ldo] = (T)y;
+      }
+    }
   }
   __syncthreads();
 }
