@@ -359,14 +359,40 @@ constexpr int kJacobiMaxCols = 256;  // columns whose norms the Jacobi keeps in 
 // at rounding level of ||G|| carry nothing for the truncation thresholds used
 // here).  Returns the number of sweeps.  'sh2' is 2 + kWaves shared doubles of
 // scratch (workgroup team only); 'nrm' holds cols doubles or is null.
+// Rotation of one-sided Jacobi zeroing the inner product c2 of two columns
+// with squared norms a and b: t = tan = sign(d) 2 c2 / (|d| + sqrt(d^2 + 4 c2^2))
+// with d = b - a, c = 1 / sqrt(1 + t^2), s = c t.  d and 2 c2 are first
+// scaled by a power of two (exact); the square roots and the quotient start
+// from the hardware reciprocal and reciprocal square root and take Newton
+// steps (two for c, so that c^2 + s^2 = 1 to a few ulp).
+__device__ inline void jacobi_rot(double a, double b, double c2, double& t, double& c, double& s) {
+  const double d = b - a, e = 2.0 * c2;
+  const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(d), fabs(e)));
+  const double ds = __builtin_amdgcn_ldexp(d, -ex), es = __builtin_amdgcn_ldexp(e, -ex);
+  const double q = ds * ds + es * es;  // in [0.25, 2)
+  double r = __builtin_amdgcn_rsq(q);
+  r = r * (1.5 - 0.5 * q * r * r);
+  const double den = fabs(ds) + q * r;  // |d| + sqrt(d^2 + e^2), scaled
+  double inv = __builtin_amdgcn_rcp(den);
+  inv = inv * (2.0 - den * inv);
+  t = (d < 0.0 ? -es : es) * inv;
+  const double w = 1.0 + t * t;  // in [1, 2]
+  double y = __builtin_amdgcn_rsq(w);
+  y = y * (1.5 - 0.5 * w * y * y);
+  y = y * (1.5 - 0.5 * w * y * y);
+  c = y;
+  s = y * t;
+}
+
 template <int kLanes>
 __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double* V, int ldv,
                                   double* sh2, double* nrm, int max_sweeps) {
   const int tid = team_tid<kLanes>();
-  for (int e = tid; e < cols * cols; e += kLanes) {
-    const int i = e % cols, j = e / cols;
-    V[i + (size_t)j * ldv] = (i == j) ? 1.0 : 0.0;
-  }
+  if (V != nullptr)
+    for (int e = tid; e < cols * cols; e += kLanes) {
+      const int i = e % cols, j = e / cols;
+      V[i + (size_t)j * ldv] = (i == j) ? 1.0 : 0.0;
+    }
   if (cols < 2) {
     team_sync<kLanes>();
     return 0;
@@ -439,20 +465,21 @@ __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double
         }
         if (!(c2 * c2 > tol2 * a * b) || !(fabs(c2) > floor2)) continue;
         // t = tan(theta) of the rotation zeroing the pair's inner product
-        const double d = b - a;
-        const double t = (d < 0.0 ? -2.0 * c2 : 2.0 * c2) / (fabs(d) + sqrt(d * d + 4.0 * c2 * c2));
-        const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+        double t, c, s;
+        jacobi_rot(a, b, c2, t, c, s);
         for (int i = sub; i < rows; i += g) {
           const double x = gp[i], y = gq[i];
           gp[i] = c * x - s * y;
           gq[i] = s * x + c * y;
         }
-        double* vp = V + (size_t)p * ldv;
-        double* vq = V + (size_t)q * ldv;
-        for (int i = sub; i < cols; i += g) {
-          const double x = vp[i], y = vq[i];
-          vp[i] = c * x - s * y;
-          vq[i] = s * x + c * y;
+        if (V != nullptr) {
+          double* vp = V + (size_t)p * ldv;
+          double* vq = V + (size_t)q * ldv;
+          for (int i = sub; i < cols; i += g) {
+            const double x = vp[i], y = vq[i];
+            vp[i] = c * x - s * y;
+            vq[i] = s * x + c * y;
+          }
         }
         if (track && sub == 0) {
           nrm[p] = a - t * c2;
@@ -482,17 +509,32 @@ __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double
   return sweep;
 }
 
-// team_jacobi with the pair's rows held in registers: lane 'sub' of a group
-// owns rows sub + g r (r < R) of G and of V; the loads use clamped rows and a
-// select, the columns are read once per round and written back after the
-// rotation.  Requires rows, cols <= R g and tracked norms.
-template <int kLanes, int R>
+// Sum over aligned groups of kG lanes (compile-time kG <= 16, see group_sum).
+template <int kG>
+__device__ inline double group_sum_c(double v) {
+  if constexpr (kG >= 2) v += dpp_swap<0xB1>(v);
+  if constexpr (kG >= 4) v += dpp_swap<0x4E>(v);
+  if constexpr (kG >= 8) v += dpp_swap<0x141>(v);
+  if constexpr (kG >= 16) v += dpp_swap<0x140>(v);
+  return v;
+}
+
+// team_jacobi with the pair's rows held in registers and a compile-time
+// group of kG lanes per pair: lane 'sub' of a group owns rows sub + kG r
+// (r < R) of G (and of V when kV); the loads use clamped rows and a select,
+// the columns are read once per round and written back after the rotation.
+// Requires rows (and cols when kV) <= R kG, kG == group_lanes(pairs, kLanes)
+// and tracked norms.  Without V (kV false) the caller recovers the right
+// singular vectors from the input, as G_in^T U.
+template <int kLanes, int R, int kG, bool kV>
 __device__ inline int team_jacobi_rows(double* G, int ldg, int rows, int cols, double* V, int ldv,
                                        double* sh2, double* nrm, int max_sweeps) {
   const int tid = team_tid<kLanes>();
-  for (int e = tid; e < cols * cols; e += kLanes) {
-    const int i = e % cols, j = e / cols;
-    V[i + (size_t)j * ldv] = (i == j) ? 1.0 : 0.0;
+  if constexpr (kV) {
+    for (int e = tid; e < cols * cols; e += kLanes) {
+      const int i = e % cols, j = e / cols;
+      V[i + (size_t)j * ldv] = (i == j) ? 1.0 : 0.0;
+    }
   }
   if (cols < 2) {
     team_sync<kLanes>();
@@ -500,14 +542,14 @@ __device__ inline int team_jacobi_rows(double* G, int ldg, int rows, int cols, d
   }
   const double tol2 = 1e-26;
   const int np = (cols + 1) & ~1, rounds = np - 1, half = np / 2;
-  const int g = group_lanes(half, kLanes), ngrp = kLanes / g, grp = tid / g, sub = tid % g;
-  int rg[R], rgc[R], rv[R], rvc[R];
+  constexpr int ngrp = kLanes / kG;
+  const int grp = tid / kG, sub = tid % kG;
+  int rg[R], rgc[R], rvc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    rg[r] = sub + g * r;
+    rg[r] = sub + kG * r;
     rgc[r] = rg[r] < rows ? rg[r] : rows - 1;
-    rv[r] = sub + g * r;
-    rvc[r] = rv[r] < cols ? rv[r] : cols - 1;
+    rvc[r] = rg[r] < cols ? rg[r] : cols - 1;
   }
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
@@ -519,7 +561,7 @@ __device__ inline int team_jacobi_rows(double* G, int ldg, int rows, int cols, d
         const double x = G[rgc[r] + (size_t)c * ldg];
         a += rg[r] < rows ? x * x : 0.0;
       }
-      a = group_sum(a, g);
+      a = group_sum_c<kG>(a);
       if (sub == 0) nrm[c] = a;
     }
     team_sync<kLanes>();
@@ -547,8 +589,6 @@ __device__ inline int team_jacobi_rows(double* G, int ldg, int rows, int cols, d
         if (q >= cols) continue;
         double* gp = G + (size_t)p * ldg;
         double* gq = G + (size_t)q * ldg;
-        double* vp = V + (size_t)p * ldv;
-        double* vq = V + (size_t)q * ldv;
         // all loads of the pair issued before the first wait
         double xp[R], xq[R], up[R], uq[R];
 #pragma unroll
@@ -556,29 +596,39 @@ __device__ inline int team_jacobi_rows(double* G, int ldg, int rows, int cols, d
           xp[r] = gp[rgc[r]];
           xq[r] = gq[rgc[r]];
         }
+        if constexpr (kV) {
+          const double* vp = V + (size_t)p * ldv;
+          const double* vq = V + (size_t)q * ldv;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          up[r] = vp[rvc[r]];
-          uq[r] = vq[rvc[r]];
+          for (int r = 0; r < R; ++r) {
+            up[r] = vp[rvc[r]];
+            uq[r] = vq[rvc[r]];
+          }
         }
         const double a = nrm[p], b = nrm[q];
         double c2 = 0.0;
 #pragma unroll
         for (int r = 0; r < R; ++r) c2 += rg[r] < rows ? xp[r] * xq[r] : 0.0;
-        c2 = group_sum(c2, g);
+        c2 = group_sum_c<kG>(c2);
         if (!(c2 * c2 > tol2 * a * b) || !(fabs(c2) > floor2)) continue;
-        const double d = b - a;
-        const double t = (d < 0.0 ? -2.0 * c2 : 2.0 * c2) / (fabs(d) + sqrt(d * d + 4.0 * c2 * c2));
-        const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+        double t, c, s;
+        jacobi_rot(a, b, c2, t, c, s);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           if (rg[r] < rows) {
             gp[rg[r]] = c * xp[r] - s * xq[r];
             gq[rg[r]] = s * xp[r] + c * xq[r];
           }
-          if (rv[r] < cols) {
-            vp[rv[r]] = c * up[r] - s * uq[r];
-            vq[rv[r]] = s * up[r] + c * uq[r];
+        }
+        if constexpr (kV) {
+          double* vp = V + (size_t)p * ldv;
+          double* vq = V + (size_t)q * ldv;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            if (rg[r] < cols) {
+              vp[rg[r]] = c * up[r] - s * uq[r];
+              vq[rg[r]] = s * up[r] + c * uq[r];
+            }
           }
         }
         if (sub == 0) {
@@ -609,22 +659,43 @@ __device__ inline int team_jacobi_rows(double* G, int ldg, int rows, int cols, d
   return sweep;
 }
 
-// Dispatch: register rows when rows and cols fit R * g for R <= 8.
+// Dispatch on the group size and rows per lane; V == nullptr skips the
+// accumulation of V.  Shapes outside the instantiated ones use team_jacobi.
+template <int kLanes, bool kV>
+__device__ inline int team_jacobi_sel(double* G, int ldg, int rows, int cols, double* V, int ldv,
+                                      double* sh2, double* nrm, int max_sweeps) {
+  const int half = ((cols + 1) & ~1) / 2;
+  const int g = group_lanes(half, kLanes);
+  const int span = kV ? (rows > cols ? rows : cols) : rows;
+  const int need = (span + g - 1) / g;
+#define HMX_JR(G_, R_)                                                                             \
+  if (g == G_ && need <= R_)                                                                       \
+    return team_jacobi_rows<kLanes, R_, G_, kV>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+  if constexpr (kLanes == kWave) {
+    HMX_JR(16, 1)
+    HMX_JR(8, 2)
+    HMX_JR(4, 6)
+    HMX_JR(4, 8)
+  } else {
+    HMX_JR(16, 2)
+    HMX_JR(8, 8)
+  }
+#undef HMX_JR
+  return team_jacobi<kLanes>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+}
+
 template <int kLanes>
 __device__ inline int team_jacobi_any(double* G, int ldg, int rows, int cols, double* V, int ldv,
                                       double* sh2, double* nrm, int max_sweeps) {
   if (nrm != nullptr && cols <= kJacobiMaxCols && cols >= 2 && rows >= 1) {
-    const int half = ((cols + 1) & ~1) / 2;
-    const int g = group_lanes(half, kLanes);
-    const int need2 = ((rows > cols ? rows : cols) + g - 1) / g;
-    if (need2 <= 2) return team_jacobi_rows<kLanes, 2>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
-    if (need2 <= 4) return team_jacobi_rows<kLanes, 4>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
-    if (need2 <= 8) return team_jacobi_rows<kLanes, 8>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+    if (V != nullptr) return team_jacobi_sel<kLanes, true>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+    return team_jacobi_sel<kLanes, false>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
   }
   return team_jacobi<kLanes>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
 }
 
-// Workgroup version used by the existing call sites (cols <= kJacobiMaxCols).
+// Workgroup version used by the existing call sites (cols <= kJacobiMaxCols);
+// V == nullptr leaves V out.
 __device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
                                   int* /*flag*/, int max_sweeps = 40) {
   __shared__ double sh2[2 + kWaves];
@@ -632,7 +703,8 @@ __device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double
   team_jacobi_any<kBlock>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
 }
 
-// Single-wavefront version (G, V in LDS; cols <= kJacobiMaxCols).
+// Single-wavefront version (G, V in LDS; cols <= kJacobiMaxCols); V == nullptr
+// leaves V out.
 __device__ inline int wave_jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
                                       int max_sweeps = 40) {
   __shared__ double nrm[kJacobiMaxCols];

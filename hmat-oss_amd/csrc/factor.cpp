@@ -127,6 +127,10 @@ struct FactorState {
 };
 
 namespace {
+// launch time per task type under HMX_TRACE_LAUNCHES (diagnostics)
+double g_type_seconds[32];
+long g_type_launches[32];
+
 template <typename T>
 void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool capturing = false) {
   fs.ws.ensure((size_t)std::max<int64_t>(P.ws, 1) * sizeof(double));
@@ -156,6 +160,10 @@ void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool cap
       float ms = 0.f;
       HMX_CHECK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
       const SegLaunch& L = P.launches[q];
+      if (L.type >= 0 && L.type < 32) {
+        g_type_seconds[L.type] += ms * 1e-3;
+        ++g_type_launches[L.type];
+      }
       if (ms * 1e-3 > trace)
         fprintf(stderr, "launch %zu type %d tasks %d tiles %d: %.4f s\n", q, L.type, L.count, L.tcount, ms * 1e-3);
     }
@@ -1399,8 +1407,13 @@ class Exec {
   void set_lu(bool v) { lu_mode_ = v; }
   void report(double total) const {
     if (getenv("HMX_TRACE_LAUNCHES"))
+    {
       fprintf(stderr, "factorisation: total %.4f s, segments %.4f s, temp allocations %.4f s, %zu units\n",
               total, t_seg_, t_alloc_, units_.size());
+      for (int q = 0; q < 32; ++q)
+        if (g_type_launches[q])
+          fprintf(stderr, "  task type %d: %ld launches, %.4f s\n", q, g_type_launches[q], g_type_seconds[q]);
+    }
   }
 
  private:

@@ -763,14 +763,18 @@ __device__ void qr_finish(const double* Va, int lda, int m, const double* Vb, in
                           const QrScratch& w, bool wave_core, double eps, RkRef out, int* overflow,
                           int* flag, int* sweeps_out) {
   __shared__ int sweeps;
+  // one-sided Jacobi on G without V: G_in (kept in w.v) gives the right side
+  // afterwards as G_in^T U, so the rounds only rotate G
+  for (int e = threadIdx.x; e < K * K; e += kBlock) w.v[e] = w.g[e];
+  __syncthreads();
   if (wave_core) {
     if (threadIdx.x < kWave) {
-      const int sw = wave_jacobi_svd(w.g, K, K, K, w.v, K);
+      const int sw = wave_jacobi_svd(w.g, K, K, K, nullptr, K);
       if (threadIdx.x == 0) sweeps = sw;
     }
     __syncthreads();
   } else {
-    jacobi_svd(w.g, K, K, K, w.v, K, flag);
+    jacobi_svd(w.g, K, K, K, nullptr, K, flag);
     if (threadIdx.x == 0) sweeps = 0;
     __syncthreads();
   }
@@ -781,12 +785,23 @@ __device__ void qr_finish(const double* Va, int lda, int m, const double* Vb, in
     if (threadIdx.x == 0) atomicOr(overflow, 1);
     nk = out.cap;
   }
+  // A = Qa U sqrt(S): column j of the rotated G is sigma_j u_j
   for (int c = threadIdx.x; c < nk; c += kBlock) w.scale[c] = 1.0 / sqrt(w.sig[w.order[c]]);
   __syncthreads();
   wy_apply_out<T>(Va, lda, m, K, w.ta, w.g, w, nk, (T*)out.a, out.la);
-  for (int c = threadIdx.x; c < nk; c += kBlock) w.scale[c] = sqrt(w.sig[w.order[c]]);
+  // B = Qb V sqrt(S) with v_j sigma_j = G_in^T u_j, u_j = g_j / sigma_j: column j
+  // of w.ta (the A side's block factor is done) gets G_in^T u_j, and the same
+  // scale 1 / sqrt(sigma_j) as the A side applies
+  for (int e = threadIdx.x; e < K * nk; e += kBlock) {
+    const int i = e % K, j = w.order[e / K];
+    const double* gi = w.v + (size_t)i * K;
+    const double* gj = w.g + (size_t)j * K;
+    double s = 0.0;
+    for (int l = 0; l < K; ++l) s += gi[l] * gj[l];
+    w.ta[i + (size_t)j * K] = s / w.sig[j];
+  }
   __syncthreads();
-  wy_apply_out<T>(Vb, ldb, n, K, w.tb, w.v, w, nk, (T*)out.b, out.lb);
+  wy_apply_out<T>(Vb, ldb, n, K, w.tb, w.ta, w, nk, (T*)out.b, out.lb);
   if (threadIdx.x == 0) *out.rank = nk;
 }
 
