@@ -282,6 +282,26 @@ __device__ inline void team_apply_q(const double* V, int ldv, int m, int nref, c
   team_sync<kLanes>();
 }
 
+// T of team_larft from the strict upper triangle of S = V^T V: T[j, j] = tau_j,
+// T[l, j] = -tau_j sum_{r=l}^{j-1} T[l, r] S[r, j].  The recurrence runs on
+// the team's first wavefront; row l of T is written by lane l mod 64 only.
+template <int kLanes>
+__device__ inline void larft_recurrence(int k, const double* tau, const double* S, double* T) {
+  const int tid = team_tid<kLanes>();
+  if (tid < kWave) {
+    for (int j = 0; j < k; ++j) {
+      const double tj = tau[j];
+      for (int l = tid; l < j; l += kWave) {
+        double a = 0.0;
+        for (int r = l; r < j; ++r) a += T[l + (size_t)r * k] * S[r + (size_t)j * k];
+        T[l + (size_t)j * k] = -tj * a;
+      }
+      if (tid == (j & (kWave - 1))) T[j + (size_t)j * k] = tj;
+    }
+  }
+  team_sync<kLanes>();
+}
+
 // Triangular factor T (k x k, ld k, upper) of the block reflector
 // H_0 H_1 ... H_{k-1} = I - V T V^T of team_house_qr (LAPACK dlarft, forward,
 // columnwise): S = V^T V (strict upper part, into S, k x k), then
@@ -308,18 +328,7 @@ __device__ inline void team_larft(const double* V, int ldv, int m, int k, const 
     }
   }
   team_sync<kLanes>();
-  if (tid < kWave) {
-    for (int j = 0; j < k; ++j) {
-      const double tj = tau[j];
-      for (int l = tid; l < j; l += kWave) {
-        double a = 0.0;
-        for (int r = l; r < j; ++r) a += T[l + (size_t)r * k] * S[r + (size_t)j * k];
-        T[l + (size_t)j * k] = -tj * a;
-      }
-      if (tid == (j & (kWave - 1))) T[j + (size_t)j * k] = tj;
-    }
-  }
-  team_sync<kLanes>();
+  larft_recurrence<kLanes>(k, tau, S, T);
 }
 
 // Workgroup-wide wrappers (the workgroup's own reduction scratch is unused).

@@ -428,3 +428,86 @@ __device__ inline void wg_apply_q_any(const double* V, int ldv, int m, int K, co
 }
 
 }  // namespace hmx
+
+namespace hmx {
+
+// ---------------------------------------------------------------------------
+// Gram matrix of Householder reflectors on the matrix cores
+// ---------------------------------------------------------------------------
+typedef double hmx_double4 __attribute__((ext_vector_type(4)));
+
+// Entry (r, c) of the unit lower trapezoidal reflector matrix stored in V
+// (m x K, ld ldv: reflector entries below the diagonal), zero outside.
+__device__ inline double refl_entry(const double* V, int ldv, int m, int K, int r, int c) {
+  const int rc = r < m ? r : m - 1, cc = c < K ? c : K - 1;
+  const double x = V[rc + (size_t)cc * ldv];
+  return (r >= m || c >= K) ? 0.0 : (r > c ? x : (r == c ? 1.0 : 0.0));
+}
+
+// S = V^T V (strict upper triangle, K x K, ld K) for the reflectors in V,
+// with v_mfma_f64_16x16x4f64 over 4-row slabs: lane i + 16k holds entry
+// (k, i) of a slab's 16-column block, which is both the A (= V^T) and the B
+// operand; tile (a, b) of 16 x 16 comes back with D[i][j] in lane
+// j + 16 (i mod 4), element i / 4.  The four wavefronts take interleaved
+// slabs; their tiles go through 'part' (4 * kNb (kNb + 1) / 2 * 256
+// doubles of LDS) and are summed in a fixed order.
+template <int kNb>
+__device__ void wg_reflector_gram(const double* V, int ldv, int m, int K, double* S, double* part) {
+  constexpr int kTiles = kNb * (kNb + 1) / 2;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int kk = lane >> 4, ii = lane & 15;
+  hmx_double4 acc[kTiles];
+#pragma unroll
+  for (int t = 0; t < kTiles; ++t) acc[t] = hmx_double4{0.0, 0.0, 0.0, 0.0};
+  constexpr int kU = 4;  // slabs per step, loads issued together
+  for (int r0 = 4 * w; r0 < m; r0 += 16 * kU) {
+    double x[kU][kNb];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int b = 0; b < kNb; ++b) x[u][b] = refl_entry(V, ldv, m, K, r0 + 16 * u + kk, 16 * b + ii);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      int t = 0;
+#pragma unroll
+      for (int a = 0; a < kNb; ++a)
+#pragma unroll
+        for (int b = a; b < kNb; ++b, ++t)
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[u][a], x[u][b], acc[t], 0, 0, 0);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < kTiles; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) part[((size_t)(w * kTiles + t) * 4 + v) * kWave + lane] = acc[t][v];
+  __syncthreads();
+  for (int e = threadIdx.x; e < K * K; e += kBlock) {
+    const int i = e % K, j = e / K;
+    if (i >= j) continue;
+    const int a = i >> 4, b = j >> 4, si = i & 15, sj = j & 15;
+    const int t = a * kNb - a * (a - 1) / 2 + (b - a);
+    const int ln = sj + 16 * (si & 3), v = si >> 2;
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) s += part[((size_t)(q * kTiles + t) * 4 + v) * kWave + ln];
+    S[i + (size_t)j * K] = s;
+  }
+  __syncthreads();
+}
+
+// doubles of 'part' for K columns
+__host__ __device__ constexpr int gram_part(int K) {
+  return 4 * (((K + 15) / 16) * ((K + 15) / 16 + 1) / 2) * 256;
+}
+
+__device__ inline void wg_reflector_gram_any(const double* V, int ldv, int m, int K, double* S, double* part) {
+  switch ((K + 15) / 16) {
+    case 1: wg_reflector_gram<1>(V, ldv, m, K, S, part); break;
+    case 2: wg_reflector_gram<2>(V, ldv, m, K, S, part); break;
+    case 3: wg_reflector_gram<3>(V, ldv, m, K, S, part); break;
+    default: wg_reflector_gram<4>(V, ldv, m, K, S, part); break;
+  }
+}
+
+}  // namespace hmx

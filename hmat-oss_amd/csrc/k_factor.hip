@@ -3,6 +3,7 @@
 // tasks of a launch are independent.  Dense arithmetic runs in the storage
 // type T (FP64 or FP32); QR/SVD of the recompressions run in FP64.
 #include "dev_linalg.hpp"
+#include "dev_panel.hpp"
 #include "factor_kernels.hpp"
 
 namespace hmx {
@@ -976,10 +977,23 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     concat_side<T>(Va, lda, m, false, t, kt);
     concat_side<T>(Vb, ldb, n, true, t, kt);
     __syncthreads();
+    // V^T V of the global panels on the matrix cores when its LDS staging fits
+    double* part = cr == sh ? sh + core : sh;
+    const bool mfma_gram = K <= 64 && (part - sh) + gram_part(K) <= kShDoubles;
     house_qr(Va, lda, m, K, w.taua);
-    team_larft<kBlock>(Va, lda, m, K, w.taua, w.x, w.ta);
+    if (mfma_gram) {
+      wg_reflector_gram_any(Va, lda, m, K, w.x, part);
+      larft_recurrence<kBlock>(K, w.taua, w.x, w.ta);
+    } else {
+      team_larft<kBlock>(Va, lda, m, K, w.taua, w.x, w.ta);
+    }
     house_qr(Vb, ldb, n, K, w.taub);
-    team_larft<kBlock>(Vb, ldb, n, K, w.taub, w.x, w.tb);
+    if (mfma_gram) {
+      wg_reflector_gram_any(Vb, ldb, n, K, w.x, part);
+      larft_recurrence<kBlock>(K, w.taub, w.x, w.tb);
+    } else {
+      team_larft<kBlock>(Vb, ldb, n, K, w.taub, w.x, w.tb);
+    }
     core_product(Va, lda, Vb, ldb, K, w.g);
   }
   __syncthreads();
