@@ -28,9 +28,13 @@ __device__ inline void team_sync() {
     __syncthreads();
 }
 
+// Thread index within its team; teams are aligned groups of kLanes threads
+// (a power of two: a wavefront, half or all of the workgroup).  Teams
+// smaller than the workgroup but larger than a wavefront synchronise with
+// workgroup barriers, so all teams must pass the same number of them.
 template <int kLanes>
 __device__ inline int team_tid() {
-  return kLanes == kWave ? (int)(threadIdx.x & (kWave - 1)) : (int)threadIdx.x;
+  return (int)(threadIdx.x & (kLanes - 1));
 }
 
 // Largest power of two g <= kWave with g * items <= lanes (at least 1).
@@ -150,7 +154,9 @@ __device__ inline void team_sum_n(double (&v)[kC], double* red) {
 #pragma unroll
   for (int c = 0; c < kC; ++c) v[c] = wave_sum(v[c]);
   if constexpr (kLanes != kWave) {
+    constexpr int kTeamWaves = kLanes / kWave;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int w0 = w - w % kTeamWaves;  // first wavefront of the team
     __syncthreads();
     if (lane == 0)
 #pragma unroll
@@ -160,7 +166,7 @@ __device__ inline void team_sum_n(double (&v)[kC], double* red) {
     for (int c = 0; c < kC; ++c) {
       double s = 0.0;
 #pragma unroll
-      for (int q = 0; q < kWaves; ++q) s += red[q * kC + c];
+      for (int q = 0; q < kTeamWaves; ++q) s += red[(w0 + q) * kC + c];
       v[c] = s;
     }
   }
@@ -208,7 +214,8 @@ __device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, doub
       tau[j] = t;
       vcol[j] = beta;
     }
-    if (t != 0.0) {
+    {
+      // also when t == 0 (a no-op then), so that every team passes the same barriers
       for (int c0 = j + 1; c0 < k; c0 += 4) {
         double a[4][R];
         double d[4], aj[4];
@@ -254,8 +261,22 @@ __device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, doub
     team_house_qr_rows<kLanes, 2>(A, lda, m, k, tau);
   else if (m <= 4 * kLanes)
     team_house_qr_rows<kLanes, 4>(A, lda, m, k, tau);
-  else
+  else if (kLanes != kBlock / 2 || m <= 8 * kLanes)
     team_house_qr_rows<kLanes, kQrRows>(A, lda, m, k, tau);
+  else
+    team_house_qr_rows<kLanes, 2 * kQrRows>(A, lda, m, k, tau);
+}
+
+// Householder QR of two panels at once, one per half of the workgroup
+// (rows <= 16 * 128 each, k columns each, k <= min(m, n)).  Both halves
+// take the same k steps, so their workgroup barriers pair up.
+__device__ inline void house_qr_pair(double* A, int lda, int m, double* tau_a, double* B, int ldb, int n,
+                                     double* tau_b, int k) {
+  if (threadIdx.x < kBlock / 2)
+    team_house_qr_rows<kBlock / 2>(A, lda, m, k, tau_a);
+  else
+    team_house_qr_rows<kBlock / 2>(B, ldb, n, k, tau_b);
+  __syncthreads();
 }
 
 // Y (m x r, ldy) <- Q Y with Q = H_0 ... H_{nref-1} from team_house_qr;

@@ -980,14 +980,20 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     // V^T V of the global panels on the matrix cores when its LDS staging fits
     double* part = cr == sh ? sh + core : sh;
     const bool mfma_gram = K <= 64 && (part - sh) + gram_part(K) <= kShDoubles;
-    house_qr(Va, lda, m, K, w.taua);
+    // both panels at once, one per half of the workgroup, when each has at
+    // most 16 rows per thread
+    const bool pair = m <= 16 * (kBlock / 2) && n <= 16 * (kBlock / 2);
+    if (pair)
+      house_qr_pair(Va, lda, m, w.taua, Vb, ldb, n, w.taub, K);
+    else
+      house_qr(Va, lda, m, K, w.taua);
     if (mfma_gram) {
       wg_reflector_gram_any(Va, lda, m, K, w.x, part);
       larft_recurrence<kBlock>(K, w.taua, w.x, w.ta);
     } else {
       team_larft<kBlock>(Va, lda, m, K, w.taua, w.x, w.ta);
     }
-    house_qr(Vb, ldb, n, K, w.taub);
+    if (!pair) house_qr(Vb, ldb, n, K, w.taub);
     if (mfma_gram) {
       wg_reflector_gram_any(Vb, ldb, n, K, w.x, part);
       larft_recurrence<kBlock>(K, w.taub, w.x, w.tb);
