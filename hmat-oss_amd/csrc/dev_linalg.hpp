@@ -145,7 +145,7 @@ __device__ inline void team_house_qr(double* A, int lda, int m, int k, double* t
 // and trailing columns are updated four at a time, their loads issued
 // together and their four dot products reduced in one exchange.  A step thus
 // costs a few memory round trips whether the panel lives in LDS or in global
-// memory.  Requires m <= kQrRows * kLanes; 'red' holds 4 * kWaves doubles of
+// memory.  Requires m <= R * kLanes; 'red' holds 4 * kWaves doubles of
 // shared scratch (workgroup team only).
 constexpr int kQrRows = 8;
 
@@ -254,14 +254,15 @@ __device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, doub
   team_sync<kLanes>();
 }
 
-// Picks the rows-per-thread count for m (m <= kQrRows * kLanes).
+// Picks the rows-per-thread count for m (m <= kQrRows * kLanes for a
+// wavefront, 2 * kQrRows * kLanes for larger teams).
 template <int kLanes>
 __device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, double* tau) {
   if (m <= 2 * kLanes)
     team_house_qr_rows<kLanes, 2>(A, lda, m, k, tau);
   else if (m <= 4 * kLanes)
     team_house_qr_rows<kLanes, 4>(A, lda, m, k, tau);
-  else if (kLanes != kBlock / 2 || m <= 8 * kLanes)
+  else if (kLanes == kWave || m <= 8 * kLanes)
     team_house_qr_rows<kLanes, kQrRows>(A, lda, m, k, tau);
   else
     team_house_qr_rows<kLanes, 2 * kQrRows>(A, lda, m, k, tau);
@@ -355,7 +356,7 @@ __device__ inline void team_larft(const double* V, int ldv, int m, int k, const 
 // Workgroup-wide wrappers (the workgroup's own reduction scratch is unused).
 __device__ inline void house_qr(double* A, int lda, int m, int k, double* tau, double* = nullptr,
                                 double* = nullptr) {
-  if (m <= kQrRows * kBlock)
+  if (m <= 2 * kQrRows * kBlock)
     team_house_qr_rows<kBlock>(A, lda, m, k, tau);
   else
     team_house_qr<kBlock>(A, lda, m, k, tau);

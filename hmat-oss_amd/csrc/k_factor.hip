@@ -1073,7 +1073,9 @@ __global__ void __launch_bounds__(kBlock) k_rkrk(const FTask* tasks, double* wsb
   const bool tr = k1 < k2;
   const int gr = tr ? k2 : k1, gc = tr ? k1 : k2;
   const int64_t need = (int64_t)gr * gc + (int64_t)gc * gc + 2 * gc;
-  double* ws = need <= kShDoubles ? sh : wsbase + t.ws;
+  // SVD scratch, then the output factors (at most (gr + gc) x gc)
+  const int64_t need_full = need + 1 + (int64_t)(gr + gc) * gc;
+  double* ws = need_full <= kShDoubles ? sh : wsbase + t.ws;
   double* G = ws; ws += (size_t)gr * gc;
   double* V = ws; ws += (size_t)gc * gc;
   double* sig = ws; ws += gc;
@@ -1082,21 +1084,29 @@ __global__ void __launch_bounds__(kBlock) k_rkrk(const FTask* tasks, double* wsb
   const T* B1 = (const T*)r1.b;
   const T* A2 = (const T*)r2.a;
   const T* B2 = (const T*)r2.b;
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  for (int e = w; e < k1 * k2; e += kWaves) {
-    const int i = e % k1, j = e / k1;
-    double s = 0.0;
-    for (int r = lane; r < p; r += kWave) s += (double)B1[r + (size_t)i * r1.lb] * (double)A2[r + (size_t)j * r2.la];
-    s = wave_sum(s);
-    if (lane == 0) {
-      if (tr)
-        G[j + (size_t)i * gr] = s;
-      else
-        G[i + (size_t)j * gr] = s;
+  // core B1^T A2 (k1 x k2; stored transposed when tr): on the matrix cores
+  // when both ranks are <= 32 and the LDS staging fits, else a wavefront per
+  // entry
+  double* part = G == sh ? sh + need + 1 : sh;  // reused by the output factors afterwards
+  const bool mfma = k1 <= 32 && k2 <= 32 && (part - sh) + xty_part(k1, k2) <= kShDoubles &&
+                    wg_xty_any<T>(B1, r1.lb, k1, A2, r2.la, k2, p, G, gr, tr, part);
+  if (!mfma) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    for (int e = w; e < k1 * k2; e += kWaves) {
+      const int i = e % k1, j = e / k1;
+      double s = 0.0;
+      for (int r = lane; r < p; r += kWave) s += (double)B1[r + (size_t)i * r1.lb] * (double)A2[r + (size_t)j * r2.la];
+      s = wave_sum(s);
+      if (lane == 0) {
+        if (tr)
+          G[j + (size_t)i * gr] = s;
+        else
+          G[i + (size_t)j * gr] = s;
+      }
     }
+    __syncthreads();
   }
-  __syncthreads();
-  if (need <= kShDoubles && gc <= 32) {
+  if (G == sh && gc <= 32) {
     if (threadIdx.x < kWave) wave_jacobi_svd(G, gr, gr, gc, V, gc);
     __syncthreads();
   } else {
@@ -1108,31 +1118,26 @@ __global__ void __launch_bounds__(kBlock) k_rkrk(const FTask* tasks, double* wsb
     if (threadIdx.x == 0) atomicOr(overflow, 1);
     nk = out.cap;
   }
-  // tmp = U S V^T with U = G(:,j)/sig (k1 side), V (k2 side); swapped when tr
-  T* OA = (T*)out.a;
-  T* OB = (T*)out.b;
-  const int m = r1.m, n = r2.n;
-  for (int e = threadIdx.x; e < m * nk; e += kBlock) {
-    const int i = e % m, c = e / m, j = order[c];
+  // tmp = U S V^T with U = G(:,j)/sig (k1 side), V (k2 side); swapped when tr.
+  // The small factors Uc (k1 x nk) = U sqrt(S) and Vc (k2 x nk) = V sqrt(S)
+  // go after the SVD scratch, then the outputs stream A1 and B2 once per
+  // eight columns.
+  double* Uc = (double*)(order + gc + (gc & 1));
+  double* Vc = Uc + (size_t)k1 * nk;
+  for (int e = threadIdx.x; e < k1 * nk; e += kBlock) {
+    const int l = e % k1, c = e / k1, j = order[c];
     const double sq = sqrt(sig[j]);
-    double s = 0.0;
-    for (int l = 0; l < k1; ++l) {
-      const double u = tr ? V[l + (size_t)j * gc] * sq : G[l + (size_t)j * gr] / sq;
-      s += (double)A1[i + (size_t)l * r1.la] * u;
-    }
-    OA[i + (size_t)c * out.la] = (T)s;
+    Uc[e] = tr ? V[l + (size_t)j * gc] * sq : G[l + (size_t)j * gr] / sq;
   }
-  for (int e = threadIdx.x; e < n * nk; e += kBlock) {
-    const int i = e % n, c = e / n, j = order[c];
+  for (int e = threadIdx.x; e < k2 * nk; e += kBlock) {
+    const int l = e % k2, c = e / k2, j = order[c];
     const double sq = sqrt(sig[j]);
-    double s = 0.0;
-    for (int l = 0; l < k2; ++l) {
-      const double vv = tr ? G[l + (size_t)j * gr] / sq : V[l + (size_t)j * gc] * sq;
-      s += (double)B2[i + (size_t)l * r2.lb] * vv;
-    }
-    OB[i + (size_t)c * out.lb] = (T)s;
+    Vc[e] = tr ? G[l + (size_t)j * gr] / sq : V[l + (size_t)j * gc] * sq;
   }
   __syncthreads();
+  const int m = r1.m, n = r2.n;
+  rows_times_small<T>(A1, r1.la, m, k1, Uc, k1, nk, (T*)out.a, out.la);
+  rows_times_small<T>(B2, r2.lb, n, k2, Vc, k2, nk, (T*)out.b, out.lb);
   if (threadIdx.x == 0) *out.rank = nk;
 }
 
@@ -1211,7 +1216,8 @@ int64_t fadd_dense_workspace(int m, int n) {
 }
 int64_t rkrk_workspace(int k1, int k2) {
   const int64_t a = k1 > k2 ? k1 : k2, b = k1 < k2 ? k1 : k2;
-  return (a * b + b * b + 2 * b + 127) / 64 * 64;
+  // G, V, sig, order (+ pad), then the two output factors (a + b) x b
+  return (a * b + b * b + 2 * b + 1 + (a + b) * b + 127) / 64 * 64;
 }
 
 double* g_fadd_prof = nullptr;
