@@ -384,11 +384,24 @@ constexpr int kTrsmRegs = 8;  // block-step results per thread: 16 x 8 x 256 cov
 // inverse followed by a rank-16 update of the rest, each spread over the whole
 // workgroup: the serial chain is 3 barriers per block instead of one thread
 // walking the whole triangle.
+// Two workgroups per task: the columns of X (its rows in mode 0) are
+// independent, so workgroup h of a task solves the h-th half of them.
+constexpr int kTrsmSplit = 2;
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* fail) {
   __shared__ double sh[kShDoubles];
   if (*fail) return;
-  const FTask& t = tasks[blockIdx.x];
+  FTask t = tasks[blockIdx.x / kTrsmSplit];
+  {
+    const int all = eff(t.m), half = blockIdx.x % kTrsmSplit;
+    const int per = (all + kTrsmSplit - 1) / kTrsmSplit;
+    const int lo = half * per, cnt = all - lo < per ? all - lo : per;
+    if (cnt <= 0) return;
+    t.m.s = cnt;
+    t.m.d = nullptr;
+    t.C = (void*)((T*)t.C + ((t.mode & kTrsmBase) == 0 ? (size_t)lo : (size_t)lo * t.ldc));
+  }
   const int n = t.n.s;
   const int other = eff(t.m);
   if (other <= 0) return;
@@ -443,84 +456,82 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
     }
   }
   __syncthreads();
-  if (mode == 1) {
-    // X <- L^{-1} X: block rows top to bottom
-    for (int b = 0; b < nb; ++b) {
-      const int r0 = b * kNb, bs = n - r0 < kNb ? n - r0 : kNb;
-      const double* inv = Inv + (size_t)b * kNb * kNb;
-      // X_b <- inv X_b (results held in registers across the barrier)
-      double v[kTrsmRegs];
-      int nv = 0;
-      for (int e = threadIdx.x; e < bs * other; e += kBlock, ++nv) {
-        const int r = e % bs, c = e / bs;
-        double s = 0.0;
-        for (int l = 0; l <= r; ++l) s += inv[r + l * kNb] * Xs[(r0 + l) + c * xl];
-        v[nv] = s;
-      }
-      __syncthreads();
-      nv = 0;
-      for (int e = threadIdx.x; e < bs * other; e += kBlock, ++nv) Xs[(r0 + e % bs) + (e / bs) * xl] = v[nv];
-      __syncthreads();
-      const int rest = n - r0 - bs;
-      for (int e = threadIdx.x; e < rest * other; e += kBlock) {
-        const int i = r0 + bs + e % rest, c = e / rest;
-        double s = Xs[i + c * xl];
-        for (int l = 0; l < bs; ++l) s -= Ls[i + (size_t)(r0 + l) * n] * Xs[(r0 + l) + c * xl];
-        Xs[i + c * xl] = s;
-      }
-      __syncthreads();
+  // Every block step is two small products on the matrix cores (16 x 16
+  // tiles, inner dimension the block's bs <= 16 columns of L): the block of
+  // X times the inverted diagonal block, held in registers across a barrier,
+  // then the rank-bs update of the remaining rows (columns in mode 0).
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int ot = (other + 15) / 16;  // 16-wide tiles along 'other' (<= 8)
+  for (int step = 0; step < nb; ++step) {
+    const int b = mode == 2 ? nb - 1 - step : step;
+    const int r0 = b * kNb, bs = n - r0 < kNb ? n - r0 : kNb;
+    const double* inv = Inv + (size_t)b * kNb * kNb;
+    hmx_double4 hold[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int tt = w + kWaves * q;
+      if (tt >= ot) continue;
+      const int o0 = 16 * tt;
+      if (mode == 1)  // X_b <- inv X_b
+        hold[q] = mfma_tile16(
+            [&](int ii, int k) { return ii < bs ? inv[ii + k * kNb] : 0.0; },
+            [&](int k, int jj) { return o0 + jj < other ? Xs[(r0 + k) + (o0 + jj) * xl] : 0.0; }, bs);
+      else if (mode == 2)  // X_b <- inv^T X_b
+        hold[q] = mfma_tile16(
+            [&](int ii, int k) { return ii < bs ? inv[k + ii * kNb] : 0.0; },
+            [&](int k, int jj) { return o0 + jj < other ? Xs[(r0 + k) + (o0 + jj) * xl] : 0.0; }, bs);
+      else  // X_b <- X_b inv^T (rows of X along 'other')
+        hold[q] = mfma_tile16(
+            [&](int ii, int k) { return o0 + ii < other ? Xs[(o0 + ii) + (r0 + k) * xl] : 0.0; },
+            [&](int k, int jj) { return jj < bs ? inv[jj + k * kNb] : 0.0; }, bs);
     }
-  } else if (mode == 2) {
-    // X <- L^{-T} X: block rows bottom to top, with inv^T
-    for (int b = nb - 1; b >= 0; --b) {
-      const int r0 = b * kNb, bs = n - r0 < kNb ? n - r0 : kNb;
-      const double* inv = Inv + (size_t)b * kNb * kNb;
-      double v[kTrsmRegs];
-      int nv = 0;
-      for (int e = threadIdx.x; e < bs * other; e += kBlock, ++nv) {
-        const int r = e % bs, c = e / bs;
-        double s = 0.0;
-        for (int l = r; l < bs; ++l) s += inv[l + r * kNb] * Xs[(r0 + l) + c * xl];
-        v[nv] = s;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int tt = w + kWaves * q;
+      if (tt >= ot) continue;
+      const int o0 = 16 * tt, jj = lane & 15;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ii = (lane >> 4) + 4 * v;
+        if (mode == 0) {
+          if (o0 + ii < other && jj < bs) Xs[(o0 + ii) + (r0 + jj) * xl] = hold[q][v];
+        } else {
+          if (ii < bs && o0 + jj < other) Xs[(r0 + ii) + (o0 + jj) * xl] = hold[q][v];
+        }
       }
-      __syncthreads();
-      nv = 0;
-      for (int e = threadIdx.x; e < bs * other; e += kBlock, ++nv) Xs[(r0 + e % bs) + (e / bs) * xl] = v[nv];
-      __syncthreads();
-      for (int e = threadIdx.x; e < r0 * other; e += kBlock) {
-        const int i = e % r0, c = e / r0;
-        double s = Xs[i + c * xl];
-        for (int l = 0; l < bs; ++l) s -= Ls[(r0 + l) + (size_t)i * n] * Xs[(r0 + l) + c * xl];
-        Xs[i + c * xl] = s;
-      }
-      __syncthreads();
     }
-  } else {
-    // X <- X L^{-T}: block columns left to right, X_b <- X_b inv^T
-    for (int b = 0; b < nb; ++b) {
-      const int c0 = b * kNb, bs = n - c0 < kNb ? n - c0 : kNb;
-      const double* inv = Inv + (size_t)b * kNb * kNb;
-      double v[kTrsmRegs];
-      int nv = 0;
-      for (int e = threadIdx.x; e < other * bs; e += kBlock, ++nv) {
-        const int r = e % other, j = e / other;
-        double s = 0.0;
-        for (int l = 0; l <= j; ++l) s += Xs[r + (c0 + l) * xl] * inv[j + l * kNb];
-        v[nv] = s;
+    __syncthreads();
+    // the rest: rows [r0 + bs, n) (mode 1), [0, r0) (mode 2), columns [r0 + bs, n) (mode 0)
+    const int lo = mode == 2 ? 0 : r0 + bs, rest = mode == 2 ? r0 : n - r0 - bs;
+    const int rt = (rest + 15) / 16;
+    for (int tt = w; tt < rt * ot; tt += kWaves) {
+      const int x0 = lo + 16 * (tt % rt), o0 = 16 * (tt / rt);
+      hmx_double4 acc;
+      if (mode == 1)  // X_rest -= L(rest, b) X_b
+        acc = mfma_tile16(
+            [&](int ii, int k) { return x0 + ii < lo + rest ? Ls[(x0 + ii) + (size_t)(r0 + k) * n] : 0.0; },
+            [&](int k, int jj) { return o0 + jj < other ? Xs[(r0 + k) + (o0 + jj) * xl] : 0.0; }, bs);
+      else if (mode == 2)  // X_rest -= L(b, rest)^T X_b
+        acc = mfma_tile16(
+            [&](int ii, int k) { return x0 + ii < lo + rest ? Ls[(r0 + k) + (size_t)(x0 + ii) * n] : 0.0; },
+            [&](int k, int jj) { return o0 + jj < other ? Xs[(r0 + k) + (o0 + jj) * xl] : 0.0; }, bs);
+      else  // X(:, rest) -= X_b L(rest, b)^T, rows of X along 'other'
+        acc = mfma_tile16(
+            [&](int ii, int k) { return o0 + ii < other ? Xs[(o0 + ii) + (r0 + k) * xl] : 0.0; },
+            [&](int k, int jj) { return x0 + jj < lo + rest ? Ls[(x0 + jj) + (size_t)(r0 + k) * n] : 0.0; }, bs);
+      const int jj = lane & 15;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ii = (lane >> 4) + 4 * v;
+        if (mode == 0) {
+          if (o0 + ii < other && x0 + jj < lo + rest) Xs[(o0 + ii) + (x0 + jj) * xl] -= acc[v];
+        } else {
+          if (x0 + ii < lo + rest && o0 + jj < other) Xs[(x0 + ii) + (o0 + jj) * xl] -= acc[v];
+        }
       }
-      __syncthreads();
-      nv = 0;
-      for (int e = threadIdx.x; e < other * bs; e += kBlock, ++nv) Xs[e % other + (c0 + e / other) * xl] = v[nv];
-      __syncthreads();
-      const int rest = n - c0 - bs;
-      for (int e = threadIdx.x; e < other * rest; e += kBlock) {
-        const int r = e % other, j = c0 + bs + e / other;
-        double s = Xs[r + j * xl];
-        for (int l = 0; l < bs; ++l) s -= Xs[r + (c0 + l) * xl] * Ls[j + (size_t)(c0 + l) * n];
-        Xs[r + j * xl] = s;
-      }
-      __syncthreads();
     }
+    __syncthreads();
   }
   for (int e = threadIdx.x; e < xr * xc; e += kBlock) {
     const int i = e % xr, j = e / xr;
@@ -1248,7 +1259,7 @@ void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile*
       hipLaunchKernelGGL(k_potrf<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
       break;
     case T_TRSM:
-      hipLaunchKernelGGL(k_trsm<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
+      hipLaunchKernelGGL(k_trsm<T>, dim3(count * kTrsmSplit), dim3(kBlock), 0, s, tasks, flags + 1);
       break;
     case T_FADD:
       hipLaunchKernelGGL(k_fadd<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags, g_fadd_prof);
