@@ -20,6 +20,28 @@ __device__ inline int rank_of(const RkRef& r) {
   return v < r.cap ? v : r.cap;
 }
 
+// 16 x 16 x 4 matrix-core products in FP64 and FP32.  D[i][j] is returned to
+// lane j + 16 (i mod 4) as element i / 4 in FP64, to lane j + 16 (i / 4) as
+// element i mod 4 in FP32.
+template <typename T>
+struct MfmaAcc;
+template <>
+struct MfmaAcc<double> {
+  typedef double type __attribute__((ext_vector_type(4)));
+  __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <>
+struct MfmaAcc<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+__device__ inline MfmaAcc<double>::type mfma16x16x4(double a, double b, MfmaAcc<double>::type c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline MfmaAcc<float>::type mfma16x16x4(float a, float b, MfmaAcc<float>::type c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
 // ---------------------------------------------------------------------------
 // GEMM: C (m x n, ldc) = beta C + alpha op(A) op(B), 64 x 64 tiles.
 // ---------------------------------------------------------------------------
@@ -39,12 +61,18 @@ __global__ void __launch_bounds__(256) k_gemm(const FTask* tasks, const GemmTile
   const T* A = (const T*)t.A;
   const T* B = (const T*)t.B;
   T* C = (T*)t.C;
-  const int tr = threadIdx.x % 16, tc = threadIdx.x / 16;
-  T acc[4][4];
+  // wavefront w computes the 32 x 32 quadrant (w & 1, w >> 1) of the tile as
+  // 2 x 2 matrix-core tiles of 16 x 16: A[i][k] comes from lane i + 16 k,
+  // B[k][j] from lane j + 16 k (inner dimension 4 per instruction)
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int qi = (w & 1) * 32, qj = (w >> 1) * 32;
+  using Acc = typename MfmaAcc<T>::type;
+  Acc acc[2][2];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[r][c] = 0;
+    for (int b = 0; b < 2; ++b) acc[a][b] = Acc{0, 0, 0, 0};
   for (int k0 = 0; k0 < P; k0 += TK) {
     // As[kk][ii] = op(A)(i0 + ii, k0 + kk), Bs[kk][jj] = op(B)(k0 + kk, j0 + jj)
     const long la = t.lda, lb = t.ldb;
@@ -58,32 +86,31 @@ __global__ void __launch_bounds__(256) k_gemm(const FTask* tasks, const GemmTile
       stage_in(&Bs[0][0], TN + 1, B + k0 + (size_t)j0 * lb, lb, 1L, TN, TK, N - j0, P - k0);
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < TK; ++kk) {
-      T a[4], b[4];
+    for (int kk = 0; kk < TK; kk += 4) {
+      T fa[2], fb[2];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = As[kk][tr + 16 * r];
+      for (int a = 0; a < 2; ++a) fa[a] = As[kk + lk][qi + 16 * a + li];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) b[c] = Bs[kk][tc + 16 * c];
+      for (int b = 0; b < 2; ++b) fb[b] = Bs[kk + lk][qj + 16 * b + li];
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
+        for (int b = 0; b < 2; ++b) acc[a][b] = mfma16x16x4(fa[a], fb[b], acc[a][b]);
     }
     __syncthreads();
   }
   const T alpha = (T)t.alpha;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int j = j0 + tc + 16 * c;
-    if (j >= N) continue;
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = i0 + tr + 16 * r;
-      if (i >= M) continue;
-      T* pc = C + i + (size_t)j * t.ldc;
-      *pc = t.mode ? *pc + alpha * acc[r][c] : alpha * acc[r][c];
-    }
-  }
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + qi + 16 * a + MfmaAcc<T>::row(lane, v), j = j0 + qj + 16 * b + li;
+        if (i >= M || j >= N) continue;
+        T* pc = C + i + (size_t)j * t.ldc;
+        *pc = t.mode ? *pc + alpha * acc[a][b][v] : alpha * acc[a][b][v];
+      }
 }
 
 // ---------------------------------------------------------------------------
