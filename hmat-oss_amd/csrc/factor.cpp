@@ -365,12 +365,18 @@ class Exec {
   // GEMMs with a long static inner dimension and a small output (the z = B^T X
   // of a large Rk panel) are split along the inner dimension: independent
   // partial products into a temporary, then a fixed-order sum.
+  // Small outputs (at most 256 x 64 by capacity: the right-hand sides of a
+  // solve, a rank-sized core) split from 512 on, into slices of 256.
   static constexpr int kSplitP = 4096, kChunkP = 2048;
+  static constexpr int kSplitSmallP = 512, kChunkSmallP = 256;
 
   void gemm(const DV& C, bool accumulate, double alpha, const DV& A, const DV& B, Dim m, Dim n,
             Dim p) {
     if (m.s <= 0 || n.s <= 0) return;
-    if (p.d == nullptr && p.s > kSplitP && (int64_t)m.s * n.s <= 256 * 256) {
+    const bool small_out = (int64_t)m.s * n.s <= 256 * 64;
+    const int split_p = small_out ? kSplitSmallP : kSplitP, chunk_p = small_out ? kChunkSmallP : kChunkP;
+    if (p.d == nullptr && p.s > split_p && (int64_t)m.s * n.s <= 256 * 256) {
+      const int kChunkP = chunk_p;
       const int nch = (p.s + kChunkP - 1) / kChunkP;
       DV tmp = temp_dense(m.s, n.s * nch, -1);
       std::vector<DV> slice(nch);
@@ -1532,7 +1538,17 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st) {
       ex.solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
       ex.solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
     }
+    if (getenv("HMX_TRACE_LAUNCHES")) {
+      for (int q = 0; q < 32; ++q) g_type_seconds[q] = 0.0, g_type_launches[q] = 0;
+    }
     ex.flush(false);
+    if (getenv("HMX_TRACE_LAUNCHES")) {
+      HMX_CHECK(hipStreamSynchronize(stream));
+      fprintf(stderr, "solve recording (nrhs %d): %.4f s\n", nrhs, now_s() - t0);
+      for (int q = 0; q < 32; ++q)
+        if (g_type_launches[q])
+          fprintf(stderr, "  task type %d: %ld launches, %.4f s\n", q, g_type_launches[q], g_type_seconds[q]);
+    }
     it = fs.solve_plans.emplace(key, std::move(plans)).first;
   } else {
     HMX_CHECK(hipMemcpy2DAsync(rhs.p, (size_t)n * sizeof(T), b, (size_t)ldb * sizeof(T),

@@ -470,17 +470,27 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
       }
   }
   // inverses of the diagonal blocks: thread (b, c) forms column c of block b
+  // by forward substitution, the column held in registers (rows beyond the
+  // block read a clamped entry and are discarded)
   for (int w = threadIdx.x; w < nb * kNb; w += kBlock) {
     const int b = w / kNb, c = w % kNb, r0 = b * kNb;
     const int bs = n - r0 < kNb ? n - r0 : kNb;
     double* inv = Inv + (size_t)b * kNb * kNb;
     if (c >= bs) continue;
-    for (int r = 0; r < c; ++r) inv[r + c * kNb] = 0.0;
-    for (int r = c; r < bs; ++r) {
+    double x[kNb];
+#pragma unroll
+    for (int r = 0; r < kNb; ++r) {
+      const int rr = r < bs ? r : bs - 1;
+      const double* lrow = Ls + (r0 + rr);
       double s = r == c ? 1.0 : 0.0;
-      for (int l = c; l < r; ++l) s -= Ls[(r0 + r) + (size_t)(r0 + l) * n] * inv[l + c * kNb];
-      inv[r + c * kNb] = unit ? s : s / Ls[(r0 + r) + (size_t)(r0 + r) * n];
+#pragma unroll
+      for (int l = 0; l < r; ++l) s -= (l >= c) ? lrow[(size_t)(r0 + l) * n] * x[l] : 0.0;
+      const double d = unit ? 1.0 : lrow[(size_t)(r0 + rr) * n];
+      x[r] = r < c ? 0.0 : s / d;
     }
+#pragma unroll
+    for (int r = 0; r < kNb; ++r)
+      if (r < bs) inv[r + c * kNb] = x[r];
   }
   __syncthreads();
   // Every block step is two small products on the matrix cores (16 x 16
