@@ -118,7 +118,7 @@ struct FactorState {
   // right-hand sides of each recorded solve plan: the plan's tasks hold this
   // buffer's address, so every (precision, nrhs) keeps its own
   std::map<std::pair<int, int>, DevBuf> solve_rhs;
-  DevBuf ir_x, ir_r, ir_norms;
+  DevBuf ir_x, ir_r, ir_norms, ir_p, ir_q, ir_z, ir_dots;
   void drop_graphs() {
     for (auto& g : solve_graphs) g.second.release();
     solve_graphs.clear();
@@ -1647,6 +1647,15 @@ double Matrix::logdet() {
 
 // Iterative refinement: x_{k+1} = x_k + (L L^T)^{-1} (b - A x_k) with the
 // FP32 factor for the correction and the FP64 H-matrix for the residual.
+// Refinement of the FP32 factor's solution against the FP64 H-matrix.  For
+// an LU factor (A not symmetric) plain sweeps x += M (b - A x); for LLt,
+// accelerated by conjugate gradients: per right-hand side, CG on A x = b
+// with the FP32 H-LLt solve as preconditioner (z = (L L^T)^{-1} r in FP32),
+// A p by the FP64 H-gemv.  Each iteration costs one FP32 solve and one FP64
+// gemv, as a plain refinement sweep, and converges at the rate of the
+// preconditioned CG instead of the contraction of the sweep.  When the
+// recursive residual meets tol, the true residual b - A x is checked; if it
+// does not, CG restarts from it.  Returns the iterations (FP32 solves) used.
 int Matrix::solve_refined(double* b, int nrhs, int ldb, double tol, int max_iter, double* resid) {
   if (!factor32 || factorization == -1) throw std::runtime_error("refined solve needs hmx_factorize_mixed first");
   if (!s64.valid) throw std::runtime_error("refined solve needs the FP64 matrix");
@@ -1655,46 +1664,88 @@ int Matrix::solve_refined(double* b, int nrhs, int ldb, double tol, int max_iter
   FactorState& fs = *fstate_;
   const int n = rows->n;
   const size_t cnt = (size_t)n * nrhs;
-  fs.ir_x.ensure(cnt * sizeof(double));
-  fs.ir_r.ensure(cnt * sizeof(double));
+  for (DevBuf* d : {&fs.ir_x, &fs.ir_r, &fs.ir_p, &fs.ir_q, &fs.ir_z}) d->ensure(cnt * sizeof(double));
   fs.ir_norms.ensure(2 * nrhs * sizeof(double));
+  fs.ir_dots.ensure(4 * nrhs * sizeof(double));
   DevBuf d32;
   d32.alloc(cnt * sizeof(float));
   double* x = fs.ir_x.as<double>();
   double* r = fs.ir_r.as<double>();
+  double* p = fs.ir_p.as<double>();
+  double* q = fs.ir_q.as<double>();
+  double* z = fs.ir_z.as<double>();
+  double* rz[2] = {fs.ir_dots.as<double>(), fs.ir_dots.as<double>() + nrhs};
+  double* pq = fs.ir_dots.as<double>() + 2 * nrhs;
+  std::vector<double> bn(nrhs), rn(nrhs);
+  auto residual_rel = [&](const double* v) {
+    launch_col_sumsq(v, n, nrhs, fs.ir_norms.as<double>(), stream);
+    HMX_CHECK(hipMemcpyAsync(rn.data(), fs.ir_norms.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HMX_CHECK(hipStreamSynchronize(stream));
+    double rel = 0.0;
+    for (int j = 0; j < nrhs; ++j) rel = std::max(rel, bn[j] > 0 ? std::sqrt(rn[j] / bn[j]) : 0.0);
+    return rel;
+  };
   HMX_CHECK(hipMemcpy2DAsync(r, (size_t)n * sizeof(double), b, (size_t)ldb * sizeof(double),
                              (size_t)n * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream));
   HMX_CHECK(hipMemsetAsync(x, 0, cnt * sizeof(double), stream));
-  std::vector<double> bn(nrhs), rn(nrhs);
   launch_col_sumsq(r, n, nrhs, fs.ir_norms.as<double>(), stream);
   HMX_CHECK(hipMemcpyAsync(bn.data(), fs.ir_norms.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, stream));
   HMX_CHECK(hipStreamSynchronize(stream));
-  int it = 0;
+  int it = 0, cur = 0;
   double rel = 1.0;
   const bool saved = user_stream;
   user_stream = true;  // keep the inner calls stream-ordered
+  // z = M r (FP32 solve), rz[cur] = r . z
+  auto precondition = [&]() {
+    launch_convert<double, float>(r, d32.as<float>(), cnt, stream);
+    solve_device<float>(d32.as<float>(), nrhs, n, s32);
+    launch_convert<float, double>(d32.as<float>(), z, cnt, stream);
+    ++it;
+  };
   try {
-    for (it = 1; it <= max_iter; ++it) {
-      launch_convert<double, float>(r, d32.as<float>(), cnt, stream);
-      solve_device<float>(d32.as<float>(), nrhs, n, s32);
-      launch_vec_add_f2d(x, d32.as<float>(), cnt, stream);
-      HMX_CHECK(hipMemcpy2DAsync(r, (size_t)n * sizeof(double), b, (size_t)ldb * sizeof(double),
-                                 (size_t)n * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream));
-      gemv_device<double>(-1.0, x, 1.0, r, nrhs, n, n, s64);
-      launch_col_sumsq(r, n, nrhs, fs.ir_norms.as<double>(), stream);
-      HMX_CHECK(hipMemcpyAsync(rn.data(), fs.ir_norms.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, stream));
-      HMX_CHECK(hipStreamSynchronize(stream));
-      rel = 0.0;
-      for (int j = 0; j < nrhs; ++j)
-        rel = std::max(rel, bn[j] > 0 ? std::sqrt(rn[j] / bn[j]) : 0.0);
-      if (rel <= tol) break;
+    if (factorization == hmat_factorization_lu) {
+      // non-symmetric A: plain refinement sweeps x += M r, r = b - A x
+      while (it < max_iter) {
+        precondition();
+        launch_vec_add(x, z, cnt, stream);
+        HMX_CHECK(hipMemcpy2DAsync(r, (size_t)n * sizeof(double), b, (size_t)ldb * sizeof(double),
+                                   (size_t)n * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream));
+        gemv_device<double>(-1.0, x, 1.0, r, nrhs, n, n, s64);
+        rel = residual_rel(r);
+        if (rel <= tol) break;
+      }
+    }
+    bool fresh = true;  // p must restart from z
+    while (factorization != hmat_factorization_lu && it < max_iter) {
+      precondition();
+      const int nxt = cur ^ 1;
+      launch_col_dot(r, z, n, nrhs, rz[nxt], stream);
+      if (fresh) {
+        HMX_CHECK(hipMemcpyAsync(p, z, cnt * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        fresh = false;
+      } else {
+        launch_cg_p(p, z, rz[nxt], rz[cur], n, nrhs, stream);
+      }
+      cur = nxt;
+      gemv_device<double>(1.0, p, 0.0, q, nrhs, n, n, s64);
+      launch_col_dot(p, q, n, nrhs, pq, stream);
+      launch_cg_xr(x, r, p, q, rz[cur], pq, n, nrhs, stream);
+      rel = residual_rel(r);
+      if (rel <= tol) {
+        // confirm with the true residual; restart from it if the recursion drifted
+        HMX_CHECK(hipMemcpy2DAsync(r, (size_t)n * sizeof(double), b, (size_t)ldb * sizeof(double),
+                                   (size_t)n * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream));
+        gemv_device<double>(-1.0, x, 1.0, r, nrhs, n, n, s64);
+        rel = residual_rel(r);
+        if (rel <= tol) break;
+        fresh = true;
+      }
     }
   } catch (...) {
     user_stream = saved;
     throw;
   }
   user_stream = saved;
-  if (it > max_iter) it = max_iter;
   HMX_CHECK(hipMemcpy2DAsync(b, (size_t)ldb * sizeof(double), x, (size_t)n * sizeof(double),
                              (size_t)n * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream));
   HMX_CHECK(hipStreamSynchronize(stream));

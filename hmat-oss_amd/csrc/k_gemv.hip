@@ -334,6 +334,65 @@ void launch_col_sumsq(const double* v, int n, int nrhs, double* out, hipStream_t
   hipLaunchKernelGGL(k_col_sumsq, dim3(nrhs), dim3(kBlock), 0, s, v, n, out);
   HMX_CHECK(hipGetLastError());
 }
+
+// ---- conjugate-gradient steps on n x nrhs column blocks (one system per column)
+namespace {
+__global__ void k_col_dot(const double* a, const double* b, int n, double* out) {
+  __shared__ double red[kWaves];
+  const double* ca = a + (size_t)blockIdx.x * n;
+  const double* cb = b + (size_t)blockIdx.x * n;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) s += ca[i] * cb[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+// alpha_j = rz_j / pq_j (0 when pq_j is 0): x += alpha p, r -= alpha q
+__global__ void k_cg_xr(double* x, double* r, const double* p, const double* q, const double* rz,
+                        const double* pq, int n, int nrhs) {
+  const size_t cnt = (size_t)n * nrhs;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < cnt; e += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / n);
+    const double a = pq[j] != 0.0 ? rz[j] / pq[j] : 0.0;
+    x[e] += a * p[e];
+    r[e] -= a * q[e];
+  }
+}
+// beta_j = rz_new_j / rz_old_j (0 when rz_old_j is 0): p = z + beta p
+__global__ void k_cg_p(double* p, const double* z, const double* rz_new, const double* rz_old, int n, int nrhs) {
+  const size_t cnt = (size_t)n * nrhs;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < cnt; e += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / n);
+    const double b = rz_old[j] != 0.0 ? rz_new[j] / rz_old[j] : 0.0;
+    p[e] = z[e] + b * p[e];
+  }
+}
+unsigned grid_for(size_t cnt) { return (unsigned)std::min<size_t>((cnt + 255) / 256, 65536); }
+}  // namespace
+
+namespace {
+__global__ void k_vec_add(double* x, const double* d, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] += d[i];
+}
+}  // namespace
+void launch_vec_add(double* x, const double* d, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_vec_add, dim3(grid_for(n)), dim3(256), 0, s, x, d, n);
+  HMX_CHECK(hipGetLastError());
+}
+void launch_col_dot(const double* a, const double* b, int n, int nrhs, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_col_dot, dim3(nrhs), dim3(kBlock), 0, s, a, b, n, out);
+  HMX_CHECK(hipGetLastError());
+}
+void launch_cg_xr(double* x, double* r, const double* p, const double* q, const double* rz, const double* pq,
+                  int n, int nrhs, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_xr, dim3(grid_for((size_t)n * nrhs)), dim3(256), 0, s, x, r, p, q, rz, pq, n, nrhs);
+  HMX_CHECK(hipGetLastError());
+}
+void launch_cg_p(double* p, const double* z, const double* rz_new, const double* rz_old, int n, int nrhs,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_p, dim3(grid_for((size_t)n * nrhs)), dim3(256), 0, s, p, z, rz_new, rz_old, n, nrhs);
+  HMX_CHECK(hipGetLastError());
+}
 template void launch_convert<float, double>(const float*, double*, size_t, hipStream_t);
 
 template <typename T>

@@ -110,6 +110,13 @@ void launch_gemv_yfinal(const YItem* items, int count, const int64_t* slot_list,
 // Refinement helpers on column-major N x nrhs vectors (ld = n).
 void launch_vec_add_f2d(double* x, const float* d, size_t n, hipStream_t s);   // x += d
 void launch_col_sumsq(const double* v, int n, int nrhs, double* out, hipStream_t s);  // out[r] = sum v^2
+void launch_vec_add(double* x, const double* d, size_t n, hipStream_t s);  // x += d
+// conjugate gradients on n x nrhs blocks, one system per column
+void launch_col_dot(const double* a, const double* b, int n, int nrhs, double* out, hipStream_t s);
+void launch_cg_xr(double* x, double* r, const double* p, const double* q, const double* rz, const double* pq,
+                  int n, int nrhs, hipStream_t s);  // alpha = rz / pq: x += alpha p, r -= alpha q
+void launch_cg_p(double* p, const double* z, const double* rz_new, const double* rz_old, int n, int nrhs,
+                 hipStream_t s);  // p = z + (rz_new / rz_old) p
 
 // Elementwise conversion of a whole arena (same layout in both precisions).
 template <typename S, typename D>
