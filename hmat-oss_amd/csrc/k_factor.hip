@@ -73,18 +73,38 @@ __global__ void __launch_bounds__(256) k_gemm(const FTask* tasks, const GemmTile
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = Acc{0, 0, 0, 0};
+  // As[kk][ii] = op(A)(i0 + ii, k0 + kk), Bs[kk][jj] = op(B)(k0 + kk, j0 + jj):
+  // each thread loads 8 + 8 entries of a k-step into registers; the loads of
+  // step k0 + TK are in flight while step k0 is multiplied
+  constexpr int kPer = TM * TK / 256;
+  const long la = t.lda, lb = t.ldb;
+  T ra[kPer], rb[kPer];
+  auto load_step = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 256 * u, i = e % TM, j = e / TM;
+      const bool ok = i < M - i0 && j < P - k0;
+      ra[u] = ok ? A[t.ta ? (size_t)(k0 + j) + (size_t)(i0 + i) * la : (size_t)(i0 + i) + (size_t)(k0 + j) * la]
+                 : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 256 * u, i = e % TN, j = e / TN;
+      const bool ok = i < N - j0 && j < P - k0;
+      rb[u] = ok ? B[t.tb ? (size_t)(j0 + i) + (size_t)(k0 + j) * lb : (size_t)(k0 + j) + (size_t)(j0 + i) * lb]
+                 : (T)0;
+    }
+  };
+  load_step(0);
   for (int k0 = 0; k0 < P; k0 += TK) {
-    // As[kk][ii] = op(A)(i0 + ii, k0 + kk), Bs[kk][jj] = op(B)(k0 + kk, j0 + jj)
-    const long la = t.lda, lb = t.ldb;
-    if (t.ta)
-      stage_in(&As[0][0], TM + 1, A + k0 + (size_t)i0 * la, la, 1L, TM, TK, M - i0, P - k0);
-    else
-      stage_in(&As[0][0], TM + 1, A + i0 + (size_t)k0 * la, 1L, la, TM, TK, M - i0, P - k0);
-    if (t.tb)
-      stage_in(&Bs[0][0], TN + 1, B + j0 + (size_t)k0 * lb, 1L, lb, TN, TK, N - j0, P - k0);
-    else
-      stage_in(&Bs[0][0], TN + 1, B + k0 + (size_t)j0 * lb, lb, 1L, TN, TK, N - j0, P - k0);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      As[e / TM][e % TM] = ra[u];
+      Bs[e / TN][e % TN] = rb[u];
+    }
     __syncthreads();
+    if (k0 + TK < P) load_step(k0 + TK);
 #pragma unroll
     for (int kk = 0; kk < TK; kk += 4) {
       T fa[2], fb[2];
