@@ -52,16 +52,18 @@ __host__ __device__ inline double kernel_value(const KernelParams& k, double dx,
 // i < mv, j < nv and 0 elsewhere.  Each thread issues kStage independent loads
 // before storing them: a plain "for (e...) lds[..] = global[..]" loop waits
 // for every load before its store, one memory round trip per iteration.
+// kS (default kStage) loads per thread per round; one-shot staging of a
+// whole leaf takes a larger kS so that it is one or two round trips.
 constexpr int kStage = 8;
 
-template <typename D, typename S>
+template <typename D, typename S, int kS = kStage>
 __device__ inline void stage_in(D* dst, int ldd, const S* src, long si, long sj, int m, int n,
                                 int mv, int nv, double scale = 1.0) {
   const int total = m * n;
-  for (int base = threadIdx.x; base < total; base += kStage * kBlock) {
-    D v[kStage];
+  for (int base = threadIdx.x; base < total; base += kS * kBlock) {
+    D v[kS];
 #pragma unroll
-    for (int u = 0; u < kStage; ++u) {
+    for (int u = 0; u < kS; ++u) {
       const int e = base + u * kBlock;
       v[u] = (D)0;
       if (e < total) {
@@ -70,15 +72,15 @@ __device__ inline void stage_in(D* dst, int ldd, const S* src, long si, long sj,
       }
     }
 #pragma unroll
-    for (int u = 0; u < kStage; ++u) {
+    for (int u = 0; u < kS; ++u) {
       const int e = base + u * kBlock;
       if (e < total) dst[e % m + (e / m) * ldd] = v[u];
     }
   }
 }
-template <typename D, typename S>
+template <typename D, typename S, int kS = kStage>
 __device__ inline void stage_in(D* dst, int ldd, const S* src, long si, long sj, int m, int n) {
-  stage_in(dst, ldd, src, si, sj, m, n, m, n);
+  stage_in<D, S, kS>(dst, ldd, src, si, sj, m, n, m, n);
 }
 
 // ---- reductions -------------------------------------------------------------

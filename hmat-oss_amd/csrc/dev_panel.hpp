@@ -633,19 +633,27 @@ __device__ void rows_times_small(const T* M, int ldm, int rows, int k, const dou
 namespace hmx {
 
 // One 16 x 16 tile D = A B on the matrix cores, inner dimension kmax <= 16
-// (entries k >= kmax count as zero): a(ii, k) gives A[ii][k], b(k, jj) gives
-// B[k][jj] (both must return 0 outside their matrices).  The wavefront's
+// (entries k >= kmax count as zero; 1 <= kmax): a(ii, k) gives A[ii][k],
+// b(k, jj) gives B[k][jj] for k < kmax (both must return 0 outside their
+// matrices, best by a clamped index and a select so that the loads stay
+// unconditional).  The wavefront's
 // lane jj + 16 (ii mod 4) receives D[ii][jj] in element ii / 4.
 template <typename FA, typename FB>
 __device__ inline hmx_double4 mfma_tile16(FA a, FB b, int kmax) {
   const int lane = threadIdx.x & (kWave - 1), ii = lane & 15, kk = lane >> 4;
+  // all eight operand loads first (clamped k, then a select), then the MFMAs
+  double av[4], bv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int k = 4 * q + kk, kc = k < kmax ? k : kmax - 1;
+    av[q] = a(ii, kc);
+    bv[q] = b(kc, ii);
+  }
   hmx_double4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int k4 = 0; k4 < 16; k4 += 4) {
-    const int k = k4 + kk;
-    const double av = k < kmax ? a(ii, k) : 0.0;
-    const double bv = k < kmax ? b(k, ii) : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  for (int q = 0; q < 4; ++q) {
+    const bool in = 4 * q + kk < kmax;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(in ? av[q] : 0.0, in ? bv[q] : 0.0, acc, 0, 0, 0);
   }
   return acc;
 }

@@ -1538,10 +1538,23 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st) {
       ex.solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
       ex.solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
     }
+    DevBuf tprof;
     if (getenv("HMX_TRACE_LAUNCHES")) {
       for (int q = 0; q < 32; ++q) g_type_seconds[q] = 0.0, g_type_launches[q] = 0;
+      tprof.ensure(8 * sizeof(double));
+      HMX_CHECK(hipMemset(tprof.p, 0, 8 * sizeof(double)));
+      g_trsm_prof = tprof.as<double>();
     }
     ex.flush(false);
+    if (g_trsm_prof) {
+      double h[8];
+      HMX_CHECK(hipStreamSynchronize(stream));
+      HMX_CHECK(hipMemcpy(h, g_trsm_prof, sizeof(h), hipMemcpyDeviceToHost));
+      g_trsm_prof = nullptr;
+      const double nwg = h[4] > 0 ? h[4] : 1.0;
+      fprintf(stderr, "trsm per workgroup (cycles): staging %.0f, inverses %.0f, steps %.0f, write %.0f; %.0f workgroups, %.0f fallbacks\n",
+              h[0] / nwg, h[1] / nwg, h[2] / nwg, h[3] / nwg, h[4], h[7]);
+    }
     if (getenv("HMX_TRACE_LAUNCHES")) {
       HMX_CHECK(hipStreamSynchronize(stream));
       fprintf(stderr, "solve recording (nrhs %d): %.4f s\n", nrhs, now_s() - t0);

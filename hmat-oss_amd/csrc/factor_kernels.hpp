@@ -20,6 +20,9 @@ int64_t rkrk_workspace(int k1, int k2);
 // (path, m, n, K, total clocks, set-up, QR, SVD + output; the SVD field
 // carries 1e-3 x the Jacobi sweep count) for performance analysis.
 extern double* g_fadd_prof;
+// When non-null, k_trsm adds its phase cycles (staging, block inverses, block
+// steps, write-back), its workgroup count and its fallbacks at [0..8) (diagnostics).
+extern double* g_trsm_prof;
 
 // p[0..n) = v on stream s (a kernel, so it may be captured in a graph).
 void launch_fill_int(int* p, size_t n, int v, hipStream_t s);

@@ -436,9 +436,11 @@ constexpr int kTrsmRegs = 8;  // block-step results per thread: 16 x 8 x 256 cov
 constexpr int kTrsmSplit = 2;
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* fail) {
+__global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* fail, double* prof) {
   __shared__ double sh[kShDoubles];
   if (*fail) return;
+  long tp[6];
+  tp[0] = clock64();
   FTask t = tasks[blockIdx.x / kTrsmSplit];
   {
     const int all = eff(t.m), half = blockIdx.x % kTrsmSplit;
@@ -461,6 +463,7 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
   const int64_t need = (int64_t)n * n + (int64_t)nb * kNb * kNb + (int64_t)xl * xc;
   if (need > kShDoubles || other > kNb * kTrsmRegs) {
     trsm_rowwise<T>(t, sh);
+    if (prof && threadIdx.x == 0) atomicAdd(prof + 7, 1.0);
     return;
   }
   double* Ls = sh;                          // n x n, ld n (L, or U^T with transa)
@@ -476,6 +479,7 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
     stage_in(Ls, n, Lg, 1L, (long)ldl, n, n);
   stage_in(Xs, xl, X, 1L, (long)ldx, xr, xc);
   __syncthreads();
+  tp[1] = clock64();
   if ((t.mode & kTrsmPiv) && mode != 0) {
     const int* piv = (const int*)t.B;
     for (int c = threadIdx.x; c < other; c += kBlock)
@@ -504,7 +508,11 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
       const double* lrow = Ls + (r0 + rr);
       double s = r == c ? 1.0 : 0.0;
 #pragma unroll
-      for (int l = 0; l < r; ++l) s -= (l >= c) ? lrow[(size_t)(r0 + l) * n] * x[l] : 0.0;
+      for (int l = 0; l < r; ++l) {
+        // x[l] == 0 for l < c; columns past the matrix only feed discarded rows
+        const int cl = r0 + l < n ? r0 + l : n - 1;
+        s -= lrow[(size_t)cl * n] * x[l];
+      }
       const double d = unit ? 1.0 : lrow[(size_t)(r0 + rr) * n];
       x[r] = r < c ? 0.0 : s / d;
     }
@@ -513,6 +521,7 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
       if (r < bs) inv[r + c * kNb] = x[r];
   }
   __syncthreads();
+  tp[2] = clock64();
   // Every block step is two small products on the matrix cores (16 x 16
   // tiles, inner dimension the block's bs <= 16 columns of L): the block of
   // X times the inverted diagonal block, held in registers across a barrier,
@@ -531,16 +540,16 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
       const int o0 = 16 * tt;
       if (mode == 1)  // X_b <- inv X_b
         hold[q] = mfma_tile16(
-            [&](int ii, int k) { return ii < bs ? inv[ii + k * kNb] : 0.0; },
-            [&](int k, int jj) { return o0 + jj < other ? Xs[(r0 + k) + (o0 + jj) * xl] : 0.0; }, bs);
+            [&](int ii, int k) { const double v = inv[(ii < bs ? ii : bs - 1) + k * kNb]; return ii < bs ? v : 0.0; },
+            [&](int k, int jj) { const int c = o0 + jj < other ? o0 + jj : other - 1; const double v = Xs[(r0 + k) + c * xl]; return o0 + jj < other ? v : 0.0; }, bs);
       else if (mode == 2)  // X_b <- inv^T X_b
         hold[q] = mfma_tile16(
-            [&](int ii, int k) { return ii < bs ? inv[k + ii * kNb] : 0.0; },
-            [&](int k, int jj) { return o0 + jj < other ? Xs[(r0 + k) + (o0 + jj) * xl] : 0.0; }, bs);
+            [&](int ii, int k) { const double v = inv[k + (ii < bs ? ii : bs - 1) * kNb]; return ii < bs ? v : 0.0; },
+            [&](int k, int jj) { const int c = o0 + jj < other ? o0 + jj : other - 1; const double v = Xs[(r0 + k) + c * xl]; return o0 + jj < other ? v : 0.0; }, bs);
       else  // X_b <- X_b inv^T (rows of X along 'other')
         hold[q] = mfma_tile16(
-            [&](int ii, int k) { return o0 + ii < other ? Xs[(o0 + ii) + (r0 + k) * xl] : 0.0; },
-            [&](int k, int jj) { return jj < bs ? inv[jj + k * kNb] : 0.0; }, bs);
+            [&](int ii, int k) { const int r = o0 + ii < other ? o0 + ii : other - 1; const double v = Xs[r + (r0 + k) * xl]; return o0 + ii < other ? v : 0.0; },
+            [&](int k, int jj) { const double v = inv[(jj < bs ? jj : bs - 1) + k * kNb]; return jj < bs ? v : 0.0; }, bs);
     }
     __syncthreads();
 #pragma unroll
@@ -567,16 +576,16 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
       hmx_double4 acc;
       if (mode == 1)  // X_rest -= L(rest, b) X_b
         acc = mfma_tile16(
-            [&](int ii, int k) { return x0 + ii < lo + rest ? Ls[(x0 + ii) + (size_t)(r0 + k) * n] : 0.0; },
-            [&](int k, int jj) { return o0 + jj < other ? Xs[(r0 + k) + (o0 + jj) * xl] : 0.0; }, bs);
+            [&](int ii, int k) { const int r = x0 + ii < lo + rest ? x0 + ii : lo + rest - 1; const double v = Ls[r + (size_t)(r0 + k) * n]; return x0 + ii < lo + rest ? v : 0.0; },
+            [&](int k, int jj) { const int c = o0 + jj < other ? o0 + jj : other - 1; const double v = Xs[(r0 + k) + c * xl]; return o0 + jj < other ? v : 0.0; }, bs);
       else if (mode == 2)  // X_rest -= L(b, rest)^T X_b
         acc = mfma_tile16(
-            [&](int ii, int k) { return x0 + ii < lo + rest ? Ls[(r0 + k) + (size_t)(x0 + ii) * n] : 0.0; },
-            [&](int k, int jj) { return o0 + jj < other ? Xs[(r0 + k) + (o0 + jj) * xl] : 0.0; }, bs);
+            [&](int ii, int k) { const int r = x0 + ii < lo + rest ? x0 + ii : lo + rest - 1; const double v = Ls[(r0 + k) + (size_t)r * n]; return x0 + ii < lo + rest ? v : 0.0; },
+            [&](int k, int jj) { const int c = o0 + jj < other ? o0 + jj : other - 1; const double v = Xs[(r0 + k) + c * xl]; return o0 + jj < other ? v : 0.0; }, bs);
       else  // X(:, rest) -= X_b L(rest, b)^T, rows of X along 'other'
         acc = mfma_tile16(
-            [&](int ii, int k) { return o0 + ii < other ? Xs[(o0 + ii) + (r0 + k) * xl] : 0.0; },
-            [&](int k, int jj) { return x0 + jj < lo + rest ? Ls[(x0 + jj) + (size_t)(r0 + k) * n] : 0.0; }, bs);
+            [&](int ii, int k) { const int r = o0 + ii < other ? o0 + ii : other - 1; const double v = Xs[r + (r0 + k) * xl]; return o0 + ii < other ? v : 0.0; },
+            [&](int k, int jj) { const int c = x0 + jj < lo + rest ? x0 + jj : lo + rest - 1; const double v = Ls[c + (size_t)(r0 + k) * n]; return x0 + jj < lo + rest ? v : 0.0; }, bs);
       const int jj = lane & 15;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -590,9 +599,21 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
     }
     __syncthreads();
   }
+  tp[3] = clock64();
   for (int e = threadIdx.x; e < xr * xc; e += kBlock) {
     const int i = e % xr, j = e / xr;
     X[i + (size_t)j * ldx] = (T)Xs[i + j * xl];
+  }
+  if (prof) {
+    __syncthreads();
+    tp[4] = clock64();
+    if (threadIdx.x == 0) {
+      atomicAdd(prof + 0, (double)(tp[1] - tp[0]));
+      atomicAdd(prof + 1, (double)(tp[2] - tp[1]));
+      atomicAdd(prof + 2, (double)(tp[3] - tp[2]));
+      atomicAdd(prof + 3, (double)(tp[4] - tp[3]));
+      atomicAdd(prof + 4, 1.0);
+    }
   }
 }
 
@@ -1289,6 +1310,7 @@ int64_t rkrk_workspace(int k1, int k2) {
 }
 
 double* g_fadd_prof = nullptr;
+double* g_trsm_prof = nullptr;
 
 namespace {
 __global__ void k_fill_int(int* p, size_t n, int v) {
@@ -1316,7 +1338,7 @@ void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile*
       hipLaunchKernelGGL(k_potrf<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
       break;
     case T_TRSM:
-      hipLaunchKernelGGL(k_trsm<T>, dim3(count * kTrsmSplit), dim3(kBlock), 0, s, tasks, flags + 1);
+      hipLaunchKernelGGL(k_trsm<T>, dim3(count * kTrsmSplit), dim3(kBlock), 0, s, tasks, flags + 1, g_trsm_prof);
       break;
     case T_FADD:
       hipLaunchKernelGGL(k_fadd<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags, g_fadd_prof);
