@@ -76,6 +76,8 @@ struct Unit {
 
 struct SegLaunch {
   int type, first, count, tfirst, tcount;
+  int wave;      // dependency wave: launches of one wave are independent
+  bool overlap;  // may run beside the other launches of its wave (its workspace does not overlap theirs)
 };
 
 // One executed segment: its tasks and GEMM tiles in device memory and the
@@ -123,7 +125,29 @@ struct FactorState {
     for (auto& g : solve_graphs) g.second.release();
     solve_graphs.clear();
   }
-  ~FactorState() { drop_graphs(); }
+  // side streams (default priority) and events for running the independent
+  // launches of one wave side by side
+  static constexpr int kSideStreams = 2;
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t fork = nullptr, join[kSideStreams] = {};
+  void ensure_side() {
+    if (fork) return;
+    for (int k = 0; k < kSideStreams; ++k) {
+      HMX_CHECK(hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking));
+      HMX_CHECK(hipEventCreateWithFlags(&join[k], hipEventDisableTiming));
+    }
+    HMX_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  }
+  ~FactorState() {
+    drop_graphs();
+    if (fork) {
+      (void)hipEventDestroy(fork);
+      for (int k = 0; k < kSideStreams; ++k) {
+        (void)hipEventDestroy(join[k]);
+        (void)hipStreamDestroy(side[k]);
+      }
+    }
+  }
 };
 
 namespace {
@@ -148,11 +172,37 @@ void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool cap
     for (auto& e : ev) HMX_CHECK(hipEventCreate(&e));
     HMX_CHECK(hipEventRecord(ev[0], s));
   }
-  for (size_t q = 0; q < P.launches.size(); ++q) {
-    const SegLaunch& L = P.launches[q];
-    launch_task_kernel<T>(L.type, dt + L.first, L.count, dtl + L.tfirst, L.tcount,
-                          fs.ws.as<double>(), fs.flags.as<int>(), s);
-    if (trace >= 0) HMX_CHECK(hipEventRecord(ev[q + 1], s));
+  const bool side_by_side = !capturing && trace < 0;
+  if (side_by_side) fs.ensure_side();
+  for (size_t q = 0; q < P.launches.size();) {
+    // the launches of one wave that may overlap go round-robin over the
+    // matrix's stream and the side streams, joined before the next wave
+    size_t e = q + 1;
+    if (side_by_side && P.launches[q].overlap)
+      while (e < P.launches.size() && P.launches[e].wave == P.launches[q].wave && P.launches[e].overlap) ++e;
+    if (e - q == 1) {
+      const SegLaunch& L = P.launches[q];
+      launch_task_kernel<T>(L.type, dt + L.first, L.count, dtl + L.tfirst, L.tcount,
+                            fs.ws.as<double>(), fs.flags.as<int>(), s);
+      if (trace >= 0) HMX_CHECK(hipEventRecord(ev[q + 1], s));
+      q = e;
+      continue;
+    }
+    const int used = (int)std::min<size_t>(e - q - 1, FactorState::kSideStreams);
+    HMX_CHECK(hipEventRecord(fs.fork, s));
+    for (int k = 0; k < used; ++k) HMX_CHECK(hipStreamWaitEvent(fs.side[k], fs.fork, 0));
+    for (size_t j = q; j < e; ++j) {
+      const SegLaunch& L = P.launches[j];
+      const int slot = (int)((j - q) % (used + 1));
+      hipStream_t st = slot == 0 ? s : fs.side[slot - 1];
+      launch_task_kernel<T>(L.type, dt + L.first, L.count, dtl + L.tfirst, L.tcount,
+                            fs.ws.as<double>(), fs.flags.as<int>(), st);
+    }
+    for (int k = 0; k < used; ++k) {
+      HMX_CHECK(hipEventRecord(fs.join[k], fs.side[k]));
+      HMX_CHECK(hipStreamWaitEvent(s, fs.join[k], 0));
+    }
+    q = e;
   }
   if (trace >= 0) {
     HMX_CHECK(hipStreamSynchronize(s));
@@ -1327,10 +1377,11 @@ class Exec {
     std::vector<SegLaunch>& launches = plan->launches;
     int64_t ws_max = 0;
     const int64_t ws_budget = (int64_t)(4ll << 30) / 8;  // doubles per launch
+    int64_t wave_ws = 0;  // workspace taken by the overlapping launches of the current wave
     size_t i = 0;
     while (i < n) {
       const int w = waves_[order[i]], ty = tasks_[order[i]].type;
-      SegLaunch L{ty, (int)i, 0, (int)tiles.size(), 0};
+      SegLaunch L{ty, (int)i, 0, (int)tiles.size(), 0, w, false};
       int64_t ws = 0;
       while (i < n && waves_[order[i]] == w && tasks_[order[i]].type == ty) {
         FTask t = tasks_[order[i]];
@@ -1348,7 +1399,19 @@ class Exec {
         ++i;
       }
       L.tcount = (int)tiles.size() - L.tfirst;
-      ws_max = std::max(ws_max, ws);
+      // workspace after the other launches of the wave, so that they can run
+      // side by side; a launch that would exceed the budget runs alone
+      int64_t base = 0;
+      if (!launches.empty() && launches.back().wave == w && launches.back().overlap) base = wave_ws;
+      if (base + ws <= ws_budget) {
+        L.overlap = true;
+        if (base > 0)
+          for (int q = L.first; q < L.first + L.count; ++q) sorted[q].ws += base;
+        wave_ws = base + ws;
+      } else {
+        wave_ws = 0;
+      }
+      ws_max = std::max(ws_max, L.overlap ? wave_ws : ws);
       launches.push_back(L);
     }
     upload(plan->tasks, sorted.data(), n * sizeof(FTask), s);
