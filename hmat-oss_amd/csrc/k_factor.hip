@@ -813,33 +813,56 @@ __device__ void wy_apply_out(const double* V, int ldv, int rows, int K, const do
     const double* wc[kC];
 #pragma unroll
     for (int b = 0; b < kC; ++b) wc[b] = w.w + (size_t)(c0 + b < nk ? c0 + b : nk - 1) * K;
-    for (int i = threadIdx.x; i < rows; i += kBlock) {
-      double acc[kC];
+    // two rows per thread at a time (the second one kBlock further), so that
+    // 16 loads of V are in flight per round; rows past the panel read a
+    // clamped row and are not stored
+    for (int i0 = threadIdx.x; i0 < rows; i0 += 2 * kBlock) {
+      const int ia = i0, ib = i0 + kBlock < rows ? i0 + kBlock : rows - 1;
+      const bool has_b = i0 + kBlock < rows;
+      double acc[2][kC];
 #pragma unroll
-      for (int b = 0; b < kC; ++b) acc[b] = 0.0;
-      const int lmax = i < K ? i : K;  // entries strictly below the unit diagonal
-      const double* vi = V + i;
+      for (int b = 0; b < kC; ++b) acc[0][b] = acc[1][b] = 0.0;
+      // entries strictly below the unit diagonal: l < min(i, K)
+      const int la = ia < K ? ia : K, lb = ib < K ? ib : K, lmax = la > lb ? la : lb;
+      const double* va = V + ia;
+      const double* vb = V + ib;
       int l = 0;
       for (; l + 8 <= lmax; l += 8) {
-        double v[8];
+        double xa[8], xb[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = vi[(size_t)(l + q) * ldv];
+        for (int q = 0; q < 8; ++q) {
+          xa[q] = va[(size_t)(l + q) * ldv];
+          xb[q] = vb[(size_t)(l + q) * ldv];
+        }
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
+        for (int q = 0; q < 8; ++q) {
+          const double ya = l + q < la ? xa[q] : 0.0, yb = l + q < lb ? xb[q] : 0.0;
 #pragma unroll
-          for (int b = 0; b < kC; ++b) acc[b] += v[q] * wc[b][l + q];
+          for (int b = 0; b < kC; ++b) {
+            acc[0][b] += ya * wc[b][l + q];
+            acc[1][b] += yb * wc[b][l + q];
+          }
+        }
       }
       for (; l < lmax; ++l) {
-        const double v = vi[(size_t)l * ldv];
+        const double ya = l < la ? va[(size_t)l * ldv] : 0.0, yb = l < lb ? vb[(size_t)l * ldv] : 0.0;
 #pragma unroll
-        for (int b = 0; b < kC; ++b) acc[b] += v * wc[b][l];
+        for (int b = 0; b < kC; ++b) {
+          acc[0][b] += ya * wc[b][l];
+          acc[1][b] += yb * wc[b][l];
+        }
       }
 #pragma unroll
-      for (int b = 0; b < kC; ++b) {
-        if (c0 + b >= nk) break;
-        double y = -acc[b];
-        if (i < K) y += src[i + (size_t)w.order[c0 + b] * K] * w.scale[c0 + b] - wc[b][i];
-        out[i + (size_t)(c0 + b) * ldo] = (T)y;
+      for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !has_b) break;
+        const int i = h == 0 ? ia : ib;
+#pragma unroll
+        for (int b = 0; b < kC; ++b) {
+          if (c0 + b >= nk) break;
+          double y = -acc[h][b];
+          if (i < K) y += src[i + (size_t)w.order[c0 + b] * K] * w.scale[c0 + b] - wc[b][i];
+          out[i + (size_t)(c0 + b) * ldo] = (T)y;
+        }
       }
     }
   }
