@@ -703,6 +703,10 @@ __device__ int dense_to_rk(double* G, int gr, int gc, bool tr, double* V, double
   return k;
 }
 
+// Widest core SVD done by one wavefront; wider cores use the workgroup (a
+// barrier per round, but a quarter of the rows per lane).
+constexpr int kWaveJacobiMax = 16;
+
 // Scratch of the QR recompression (all K x K blocks ld K).
 struct QrScratch {
   double *taua, *taub;  // K each
@@ -1119,7 +1123,7 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
   int sweeps = 0;
   // the core SVD runs on one wavefront when its scratch is in LDS
   const bool core_lds = mode == 5 || mode == 6 || cr == sh;
-  qr_finish<T>(Va, lda, m, Vb, ldb, n, K, w, core_lds && K <= 32, fabs(t.eps), tg, overflow, &flag,
+  qr_finish<T>(Va, lda, m, Vb, ldb, n, K, w, core_lds && K <= kWaveJacobiMax, fabs(t.eps), tg, overflow, &flag,
                &sweeps);
   if (pr && threadIdx.x == 0) {
     pr[0] = mode; pr[1] = m; pr[2] = n; pr[3] = K;
@@ -1218,7 +1222,7 @@ __global__ void __launch_bounds__(kBlock) k_rkrk(const FTask* tasks, double* wsb
     }
     __syncthreads();
   }
-  if (G == sh && gc <= 32) {
+  if (G == sh && gc <= kWaveJacobiMax) {
     if (threadIdx.x < kWave) wave_jacobi_svd(G, gr, gr, gc, V, gc);
     __syncthreads();
   } else {
