@@ -172,8 +172,12 @@ __device__ inline void team_sum_n(double (&v)[kC], double* red) {
   }
 }
 
-template <int kLanes, int R>
-__device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, double* tau) {
+// Element type of a panel known to be in global memory: loads and stores then
+// use a uniform base and 32-bit offsets instead of 64-bit flat addresses.
+typedef __attribute__((address_space(1))) double gdouble;
+
+template <int kLanes, int R, typename E = double>
+__device__ inline void team_house_qr_rows(E* A, int lda, int m, int k, double* tau) {
   __shared__ double red[4 * kWaves];
   const int tid = team_tid<kLanes>();
   const int steps = m < k ? m : k;
@@ -186,7 +190,7 @@ __device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, doub
     rc[r] = row[r] < m ? row[r] : m - 1;
   }
   for (int j = 0; j < steps; ++j) {
-    double* vcol = A + (size_t)j * lda;
+    E* vcol = A + (size_t)j * lda;
     double v[R];
     double s1[1] = {0.0};
 #pragma unroll
@@ -222,7 +226,7 @@ __device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, doub
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int cc = c0 + c < k ? c0 + c : k - 1;
-          const double* ac = A + (size_t)cc * lda;
+          const E* ac = A + (size_t)cc * lda;
           aj[c] = ac[j];  // read by all before thread 0 rewrites it
 #pragma unroll
           for (int r = 0; r < R; ++r) a[c][r] = ac[rc[r]];
@@ -238,7 +242,7 @@ __device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, doub
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           if (c0 + c >= k) continue;
-          double* ac = A + (size_t)(c0 + c) * lda;
+          E* ac = A + (size_t)(c0 + c) * lda;
           const double f = t * (d[c] + aj[c]);
 #pragma unroll
           for (int r = 0; r < R; ++r)
@@ -268,15 +272,29 @@ __device__ inline void team_house_qr_rows(double* A, int lda, int m, int k, doub
     team_house_qr_rows<kLanes, 2 * kQrRows>(A, lda, m, k, tau);
 }
 
-// Householder QR of two panels at once, one per half of the workgroup
-// (rows <= 16 * 128 each, k columns each, k <= min(m, n)).  Both halves
-// take the same k steps, so their workgroup barriers pair up.
+// The rows QR of a panel in global memory (m <= 16 kLanes).
+template <int kLanes>
+__device__ inline void team_house_qr_rows_global(double* A, int lda, int m, int k, double* tau) {
+  gdouble* G = (gdouble*)A;
+  if (m <= 2 * kLanes)
+    team_house_qr_rows<kLanes, 2, gdouble>(G, lda, m, k, tau);
+  else if (m <= 4 * kLanes)
+    team_house_qr_rows<kLanes, 4, gdouble>(G, lda, m, k, tau);
+  else if (m <= 8 * kLanes)
+    team_house_qr_rows<kLanes, kQrRows, gdouble>(G, lda, m, k, tau);
+  else
+    team_house_qr_rows<kLanes, 2 * kQrRows, gdouble>(G, lda, m, k, tau);
+}
+
+// Householder QR of two global panels at once, one per half of the
+// workgroup (rows <= 16 * 128 each, k columns each, k <= min(m, n)).  Both
+// halves take the same k steps, so their workgroup barriers pair up.
 __device__ inline void house_qr_pair(double* A, int lda, int m, double* tau_a, double* B, int ldb, int n,
                                      double* tau_b, int k) {
   if (threadIdx.x < kBlock / 2)
-    team_house_qr_rows<kBlock / 2>(A, lda, m, k, tau_a);
+    team_house_qr_rows_global<kBlock / 2>(A, lda, m, k, tau_a);
   else
-    team_house_qr_rows<kBlock / 2>(B, ldb, n, k, tau_b);
+    team_house_qr_rows_global<kBlock / 2>(B, ldb, n, k, tau_b);
   __syncthreads();
 }
 
