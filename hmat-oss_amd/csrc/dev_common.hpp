@@ -60,21 +60,37 @@ template <typename D, typename S, int kS = kStage>
 __device__ inline void stage_in(D* dst, int ldd, const S* src, long si, long sj, int m, int n,
                                 int mv, int nv, double scale = 1.0) {
   const int total = m * n;
+  if (total <= 0) return;
+  // element e = i + j m of thread t is t + kBlock u: the row and column step by
+  // kBlock mod m and kBlock / m (one division per thread, not per element);
+  // the loads read a clamped entry and select, so they issue back to back
+  const int dq = kBlock / m, dr = kBlock % m;
+  const bool any = mv > 0 && nv > 0;
+  const int im = (mv < m ? mv : m) - 1, jm = (nv < n ? nv : n) - 1;
+  int i = threadIdx.x % m, j = threadIdx.x / m;
   for (int base = threadIdx.x; base < total; base += kS * kBlock) {
     D v[kS];
+    int pi[kS], pj[kS];
 #pragma unroll
     for (int u = 0; u < kS; ++u) {
       const int e = base + u * kBlock;
-      v[u] = (D)0;
-      if (e < total) {
-        const int i = e % m, j = e / m;
-        if (i < mv && j < nv) v[u] = (D)(scale * (double)src[i * si + j * sj]);
+      pi[u] = i;
+      pj[u] = j;
+      const bool ok = any && e < total && i < mv && j < nv;
+      const int ic = any ? (i < im ? i : im) : 0, jc = any ? (j < jm ? j : jm) : 0;
+      const S x = any ? src[ic * si + jc * sj] : (S)0;
+      v[u] = ok ? (D)(scale * (double)x) : (D)0;
+      i += dr;
+      j += dq;
+      if (i >= m) {
+        i -= m;
+        ++j;
       }
     }
 #pragma unroll
     for (int u = 0; u < kS; ++u) {
       const int e = base + u * kBlock;
-      if (e < total) dst[e % m + (e / m) * ldd] = v[u];
+      if (e < total) dst[pi[u] + pj[u] * ldd] = v[u];
     }
   }
 }

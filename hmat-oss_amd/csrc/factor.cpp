@@ -1615,8 +1615,8 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st) {
       HMX_CHECK(hipMemcpy(h, g_trsm_prof, sizeof(h), hipMemcpyDeviceToHost));
       g_trsm_prof = nullptr;
       const double nwg = h[4] > 0 ? h[4] : 1.0;
-      fprintf(stderr, "trsm per workgroup (cycles): staging %.0f, inverses %.0f, steps %.0f, write %.0f; %.0f workgroups, %.0f fallbacks\n",
-              h[0] / nwg, h[1] / nwg, h[2] / nwg, h[3] / nwg, h[4], h[7]);
+      fprintf(stderr, "trsm per workgroup (cycles): staging %.0f (task read %.0f, L %.0f), inverses %.0f, steps %.0f, write %.0f; %.0f workgroups, %.0f fallbacks\n",
+              h[0] / nwg, h[5] / nwg, h[6] / nwg, h[1] / nwg, h[2] / nwg, h[3] / nwg, h[4], h[7]);
     }
     if (getenv("HMX_TRACE_LAUNCHES")) {
       HMX_CHECK(hipStreamSynchronize(stream));

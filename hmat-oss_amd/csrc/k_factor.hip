@@ -473,13 +473,20 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
   const int ldl = t.lda;
   T* X = (T*)t.C;
   const int ldx = t.ldc;
+  tp[5] = clock64();
   if (transa)
     stage_in(Ls, n, Lg, (long)ldl, 1L, n, n);
   else
     stage_in(Ls, n, Lg, 1L, (long)ldl, n, n);
+  __syncthreads();
+  const long tl = clock64();
   stage_in(Xs, xl, X, 1L, (long)ldx, xr, xc);
   __syncthreads();
   tp[1] = clock64();
+  if (prof && threadIdx.x == 0) {
+    atomicAdd(prof + 5, (double)(tp[5] - tp[0]));
+    atomicAdd(prof + 6, (double)(tl - tp[5]));
+  }
   if ((t.mode & kTrsmPiv) && mode != 0) {
     const int* piv = (const int*)t.B;
     for (int c = threadIdx.x; c < other; c += kBlock)
