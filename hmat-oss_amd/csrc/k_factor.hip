@@ -151,8 +151,15 @@ __device__ void potrf_body(T* L, int ld, int n, int* info) {
     const int je = jb + kNb < n ? jb + kNb : n;
     for (int j = jb; j < je; ++j) {
       for (int i = j + threadIdx.x; i < n; i += kBlock) {
+        // the panel's earlier columns: a fixed 16 terms with a mask, so that
+        // their loads issue together (column indices clamped into the matrix)
         T s = L[i + (size_t)j * ld];
-        for (int l = jb; l < j; ++l) s -= L[i + (size_t)l * ld] * L[j + (size_t)l * ld];
+#pragma unroll
+        for (int q = 0; q < kNb; ++q) {
+          const int l = jb + q < n ? jb + q : n - 1;
+          const T a = L[i + (size_t)l * ld], b = L[j + (size_t)l * ld];
+          s -= (jb + q < j) ? a * b : (T)0;
+        }
         L[i + (size_t)j * ld] = s;
       }
       __syncthreads();
@@ -171,7 +178,12 @@ __device__ void potrf_body(T* L, int ld, int n, int* info) {
       const int i = je + e % r, c = je + e / r;
       if (c > i) continue;
       T s = 0;
-      for (int l = jb; l < je; ++l) s += L[i + (size_t)l * ld] * L[c + (size_t)l * ld];
+#pragma unroll
+      for (int q = 0; q < kNb; ++q) {
+        const int l = jb + q < n ? jb + q : n - 1;
+        const T a = L[i + (size_t)l * ld], b = L[c + (size_t)l * ld];
+        s += (jb + q < je) ? a * b : (T)0;
+      }
       L[i + (size_t)c * ld] -= s;
     }
     __syncthreads();
@@ -1301,15 +1313,19 @@ __global__ void __launch_bounds__(kBlock) k_acafull(const FTask* tasks, const in
       const int r = e % m, c = e / m;
       D[r + (size_t)c * ld] -= va[r] * vb[c];
     }
-    // estimate (the reference sums l < nu - 1)
+    // estimate (the reference sums l < nu - 1): each wavefront takes the terms
+    // l = w, w + 4, ... with wavefront sums, then one workgroup sum
     double cross = 0.0;
-    for (int l = 0; l + 1 < nu; ++l) {
-      double da = 0.0, db = 0.0;
-      for (int r = threadIdx.x; r < m; r += kBlock) da += (double)va[r] * (double)OA[r + (size_t)l * out.la];
-      for (int c = threadIdx.x; c < n; c += kBlock) db += (double)vb[c] * (double)OB[c + (size_t)l * out.lb];
-      da = block_sum(da, red);
-      db = block_sum(db, red);
-      cross += da * db;
+    {
+      const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+      double part = 0.0;
+      for (int l = wv; l + 1 < nu; l += kWaves) {
+        double da = 0.0, db = 0.0;
+        for (int r = lane; r < m; r += kWave) da += (double)va[r] * (double)OA[r + (size_t)l * out.la];
+        for (int c = lane; c < n; c += kWave) db += (double)vb[c] * (double)OB[c + (size_t)l * out.lb];
+        part += wave_sum(da) * wave_sum(db);
+      }
+      cross = block_sum(lane == 0 ? part : 0.0, red);
     }
     double an = 0.0, bn = 0.0;
     for (int r = threadIdx.x; r < m; r += kBlock) an += (double)va[r] * (double)va[r];
