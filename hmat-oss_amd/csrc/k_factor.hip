@@ -1278,6 +1278,7 @@ __global__ void __launch_bounds__(kBlock) k_rkrk(const FTask* tasks, double* wsb
 
 // ---------------------------------------------------------------------------
 // ACAFULL: acaFull (compression.cpp:270-355) on a dense temp (modified)
+constexpr int kAcaLds = 16384;  // entries of the LDS copy (a 128 x 128 leaf product)
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_acafull(const FTask* tasks, const int* fail) {
@@ -1287,8 +1288,18 @@ __global__ void __launch_bounds__(kBlock) k_acafull(const FTask* tasks, const in
   if (*fail) return;
   const FTask& t = tasks[blockIdx.x];
   RkRef out = t.tgt;
+  const int m = t.m.s, n = t.n.s;
+  // the dense block is a temporary: work on an LDS copy (in the storage
+  // precision) when it fits, on the block itself otherwise
+  __shared__ T dl[kAcaLds];
   T* D = (T*)t.A;
-  const int m = t.m.s, n = t.n.s, ld = t.lda;
+  int ld = t.lda;
+  if ((int64_t)m * n <= kAcaLds) {
+    stage_in(dl, m, (const T*)t.A, 1L, (long)t.lda, m, n);
+    __syncthreads();
+    D = dl;
+    ld = m;
+  }
   int maxk = m < n ? m : n;
   if (maxk > out.cap) maxk = out.cap;
   T* OA = (T*)out.a;
