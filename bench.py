@@ -26,6 +26,11 @@ import numpy as np  # noqa: E402
 
 METRIC = "H-LLt factor+solve s @N=200k, resid<=1e-10: FP32+IR vs FP64; H-gemv GB/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes of one FP64 H-gemv of the N=200,000, height-16 workload, from the
+# FETCH_SIZE and WRITE_SIZE counters of its four kernels, scaled by what the
+# same counters read for a 2 GiB copy (x2 and x1):
+# profiles/h_gemv_fp64_n200k_hbm_traffic.md
+HGEMV_TRAFFIC_BYTES_N200K = 2.3852e9
 
 
 def log(*a):
@@ -112,6 +117,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--gemv-only", action="store_true",
                     help="assemble and time the FP64 H-gemv only (for profiling its kernels)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy 2 GiB device to device and exit: the known-size launch that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -119,6 +127,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        src = torch.ones(2**28, dtype=torch.float64, device="cuda")  # 2 GiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8, "check": float(dst[-1])}), flush=True)
+        return
+
     import hmat_amd
     from oracle.hmat_oracle import cylinder_points  # input generator only
 
@@ -258,7 +276,9 @@ def main():
                    "speedup_vs_fp64": (fp64["factor_solve_s"] / mean_s) if fp64 else None,
                    "hgemv_ms": gemv_ms, "hgemv_GBps": gbs},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "H-gemv (FP64)"},
+                     "frac": gbs / HBM_PEAK_GBS, "kernel": "H-gemv (FP64)",
+                     "traffic": HGEMV_TRAFFIC_BYTES_N200K
+                     if (n, args.height) == (200000, 16.0) else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
