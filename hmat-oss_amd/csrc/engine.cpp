@@ -219,7 +219,7 @@ void Matrix::build_gemv_plan(int nrhs) {
   GemvPlan& P = plan_;
   const auto& D = s64.hdesc;
   const int nleaf = (int)D.size();
-  const int chunk = 1024;
+  const int chunk = kZChunk;
   std::vector<ZItem> zi;
   std::vector<ZReduce> zr;
   std::vector<int64_t> zoff_side0(nleaf, -1), zoff_side1(nleaf, -1);

@@ -54,6 +54,62 @@ __global__ void __launch_bounds__(kBlock) k_zpass(const LeafDesc* desc, const in
   }
 }
 
+// One right-hand side.  A chunk of at most 64 kR64 rows: a lane keeps its
+// entries of x (rows lane + 64 v) in registers, and a wavefront issues the
+// loads of kZChunk / (64 kR64) panel columns together (indices clamped to the
+// chunk and the rank) before it adds them up, column by column in row order.
+template <typename T, int kR64>
+__device__ void zpass1_chunk(const T* P, int ld, const T* xs, int rows, int k, double* z) {
+  constexpr int kC = kZChunk / kWave / kR64;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int rm = rows - 1;
+  double xr[kR64];
+#pragma unroll
+  for (int v = 0; v < kR64; ++v) {
+    const int i = lane + kWave * v;
+    const double t = (double)xs[i < rm ? i : rm];
+    xr[v] = i < rows ? t : 0.0;
+  }
+  for (int l0 = w * kC; l0 < k; l0 += kWaves * kC) {
+    double p[kC][kR64];
+#pragma unroll
+    for (int c = 0; c < kC; ++c) {
+      const T* col = P + (size_t)(l0 + c < k ? l0 + c : k - 1) * ld;
+#pragma unroll
+      for (int v = 0; v < kR64; ++v) {
+        const int i = lane + kWave * v;
+        p[c][v] = (double)col[i < rm ? i : rm];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kC; ++c) {
+      double sum = 0.0;
+#pragma unroll
+      for (int v = 0; v < kR64; ++v) sum += p[c][v] * xr[v];
+      sum = wave_sum(sum);
+      if (lane == 0 && l0 + c < k) z[l0 + c] = sum;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_zpass1(const LeafDesc* desc, const int* ranks,
+                                                   const T* arena, const ZItem* items, const T* x,
+                                                   double* zpart) {
+  const ZItem it = items[blockIdx.x];
+  const LeafDesc d = desc[it.leaf];
+  const int k = ranks[it.leaf];
+  const int ld = it.side == 0 ? d.n : d.m;
+  const T* P = arena + (it.side == 0 ? d.b : d.a) + it.row0;
+  const T* xs = x + (it.side == 0 ? d.c0 : d.r0) + it.row0;
+  double* z = zpart + it.zslot;
+  if (it.rows <= 64) zpass1_chunk<T, 1>(P, ld, xs, it.rows, k, z);
+  else if (it.rows <= 128) zpass1_chunk<T, 2>(P, ld, xs, it.rows, k, z);
+  else if (it.rows <= 256) zpass1_chunk<T, 4>(P, ld, xs, it.rows, k, z);
+  else if (it.rows <= 512) zpass1_chunk<T, 8>(P, ld, xs, it.rows, k, z);
+  else zpass1_chunk<T, 16>(P, ld, xs, it.rows, k, z);
+}
+
 __global__ void k_zreduce(const ZReduce* red, const int* ranks, const double* zpart,
                           const ZItem* items, int nrhs, const LeafDesc* desc, double* z) {
   const ZReduce rd = red[blockIdx.x];
@@ -79,8 +135,10 @@ __global__ void k_zreduce(const ZReduce* red, const int* ranks, const double* zp
 //    to HBM serves four slices.
 // The partial sums of the row owners are added in a fixed order at the end.
 // kR right-hand sides per pass.
-constexpr int kYTile = 5120;   // doubles of the leaf tile (40 KiB)
-constexpr int kYVec = 256 * 8; // doubles of each staged vector block (x_I, x_J)
+// doubles of the leaf tile: 30 KiB with one right-hand side (the whole
+// workgroup then takes 39 KiB of LDS, four per CU), 40 KiB with more
+template <int kR>
+constexpr int y_tile() { return kR == 1 ? 3840 : 5120; }
 constexpr int kYMeta = 128;    // contribution records described in LDS at a time
 
 template <typename T, int kR>
@@ -89,9 +147,10 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
                                                   const YContrib* contrib, const double* z,
                                                   double* slots, const T* x, int ldx, int nrhs,
                                                   double* acc_out, int ldacc) {
+  constexpr int kYTile = y_tile<kR>();
   __shared__ double tile[kYTile];
-  __shared__ double xI[kYVec];
-  __shared__ double xJ[kYVec];
+  __shared__ double xI[256 * kR];  // x_I, x_J: 256 rows per right-hand side
+  __shared__ double xJ[256 * kR];
   __shared__ const T* mp[kYMeta];      // full: leaf data; rk: first row of the slice
   __shared__ const double* mz[kYMeta]; // rk: z; full mirrored: slot (as double*)
   __shared__ int mkind[kYMeta], mk[kYMeta], mld[kYMeta], mcap[kYMeta], mc0[kYMeta], mn[kYMeta];
@@ -399,8 +458,12 @@ template <typename T>
 void launch_gemv_zpass(const LeafDesc* desc, const int* ranks, const T* arena, const ZItem* items,
                        int count, const T* x, int ldx, int nrhs, double* zpart, hipStream_t s) {
   if (count == 0) return;
-  hipLaunchKernelGGL(k_zpass<T>, dim3(count), dim3(kBlock), 0, s, desc, ranks, arena, items, x,
-                     ldx, nrhs, zpart);
+  if (nrhs == 1)
+    hipLaunchKernelGGL(k_zpass1<T>, dim3(count), dim3(kBlock), 0, s, desc, ranks, arena, items, x,
+                       zpart);
+  else
+    hipLaunchKernelGGL(k_zpass<T>, dim3(count), dim3(kBlock), 0, s, desc, ranks, arena, items, x,
+                       ldx, nrhs, zpart);
   HMX_CHECK(hipGetLastError());
 }
 

@@ -58,7 +58,9 @@ void launch_aca(const LeafDesc* desc, const AcaTask* tasks, int count, T* arena,
                 double aca_eps, double trunc_eps, int partial, hipStream_t s);
 
 // ---- H-gemv -----------------------------------------------------------------
-// Work items of the z pass: z_part = P^T x over a row range of a panel.
+// Work items of the z pass: z_part = P^T x over a row range of a panel of at
+// most kZChunk rows.
+constexpr int kZChunk = 1024;
 struct ZItem {
   int leaf;
   int side;       // 0: B panel against x[cols]  (z),  1: A panel against x[rows] (z')
