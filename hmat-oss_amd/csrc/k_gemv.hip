@@ -204,21 +204,31 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
         double* slot = const_cast<double*>(mz[ci]);
         const int ldt = m | 1;
         const int cw_max = kYTile / ldt < 256 ? kYTile / ldt : 256;
+        const int ti0 = tid % m, tj0 = tid / m, tdr = kBlock % m, tdq = kBlock / m;
         for (int j0 = 0; j0 < n; j0 += cw_max) {
           const int cw = n - j0 < cw_max ? n - j0 : cw_max;
           __syncthreads();
           const T* src = M + (size_t)j0 * m;
-          for (int base = tid; base < m * cw; base += 8 * kBlock) {
-            double v[8];
+          // m cw <= kYTile: every load of the chunk is issued in one round
+          // (indices clamped), then element e = i + m j goes to i + ldt j,
+          // with (i, j) stepped by kBlock without a division
+          const int tot = m * cw;
+          constexpr int kTileLoads = (kYTile + kBlock - 1) / kBlock;
+          double v[kTileLoads];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const int e = base + u * kBlock;
-              v[u] = e < m * cw ? (double)src[e] : 0.0;
-            }
+          for (int u = 0; u < kTileLoads; ++u) {
+            const int e = tid + u * kBlock;
+            v[u] = (double)src[e < tot ? e : tot - 1];
+          }
+          int ti = ti0, tj = tj0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const int e = base + u * kBlock;
-              if (e < m * cw) tile[e % m + (e / m) * ldt] = v[u];
+          for (int u = 0; u < kTileLoads; ++u) {
+            if (tid + u * kBlock < tot) tile[ti + tj * ldt] = v[u];
+            ti += tdr;
+            tj += tdq;
+            if (ti >= m) {
+              ti -= m;
+              ++tj;
             }
           }
           for (int e = tid; e < cw * kR; e += kBlock) {
