@@ -205,21 +205,30 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
         const int ldt = m | 1;
         const int cw_max = kYTile / ldt < 256 ? kYTile / ldt : 256;
         const int ti0 = tid % m, tj0 = tid / m, tdr = kBlock % m, tdq = kBlock / m;
-        for (int j0 = 0; j0 < n; j0 += cw_max) {
-          const int cw = n - j0 < cw_max ? n - j0 : cw_max;
-          __syncthreads();
+        // m cw <= kYTile: every load of a column chunk is issued in one round
+        // (indices clamped) into registers; with one right-hand side the next
+        // chunk's loads are in flight while the current one is used from LDS
+        // (with more, the registers go to the accumulators)
+        constexpr int kTileLoads = (kYTile + kBlock - 1) / kBlock;
+        constexpr bool kPrefetch = kR == 1;
+        double v[kTileLoads];
+        auto load_chunk = [&](int j0) {
+          const int tot0 = m * (n - j0 < cw_max ? n - j0 : cw_max);
           const T* src = M + (size_t)j0 * m;
-          // m cw <= kYTile: every load of the chunk is issued in one round
-          // (indices clamped), then element e = i + m j goes to i + ldt j,
-          // with (i, j) stepped by kBlock without a division
-          const int tot = m * cw;
-          constexpr int kTileLoads = (kYTile + kBlock - 1) / kBlock;
-          double v[kTileLoads];
 #pragma unroll
           for (int u = 0; u < kTileLoads; ++u) {
             const int e = tid + u * kBlock;
-            v[u] = (double)src[e < tot ? e : tot - 1];
+            v[u] = (double)src[e < tot0 ? e : tot0 - 1];
           }
+        };
+        if (kPrefetch) load_chunk(0);
+        for (int j0 = 0; j0 < n; j0 += cw_max) {
+          const int cw = n - j0 < cw_max ? n - j0 : cw_max;
+          __syncthreads();
+          if (!kPrefetch) load_chunk(j0);
+          // element e = i + m j of the chunk goes to i + ldt j, with (i, j)
+          // stepped by kBlock without a division
+          const int tot = m * cw;
           int ti = ti0, tj = tj0;
 #pragma unroll
           for (int u = 0; u < kTileLoads; ++u) {
@@ -236,6 +245,7 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
             xJ[j + r * 256] = r < nr ? (double)x[c0 + j0 + j + (size_t)(rc + r) * ldx] : 0.0;
           }
           __syncthreads();
+          if (kPrefetch && j0 + cw_max < n) load_chunk(j0 + cw_max);
           if (lo) {
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
