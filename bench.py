@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE counters of its four kernels, scaled by what the
 # same counters read for a 2 GiB copy (x2 and x1):
 # profiles/h_gemv_fp64_n200k_hbm_traffic.md
-HGEMV_TRAFFIC_BYTES_N200K = 2.3849e9
+HGEMV_TRAFFIC_BYTES_N200K = 2.3843e9
 
 
 def log(*a):
