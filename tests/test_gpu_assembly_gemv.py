@@ -91,12 +91,17 @@ def test_aca_partial_and_svd_methods():
         h.close()
 
 
+# N=6000 on a cylinder of height 8 with leaf 64 has Rk panels of up to 1500
+# rows (oracle partition, eta 2), so the one-RHS z pass sees chunks in every
+# row class (<= 64, 128, 256, 512 and 1024 rows), including full 1024-row
+# chunks and the 476-row remainders of the 1500-row panels
 @pytest.mark.gpu
-@pytest.mark.parametrize("nrhs", [1, 3, 9])
-def test_gemv_device_matches_dense(nrhs):
+@pytest.mark.parametrize("nrhs,n,height,leaf", [(1, 4000, 1.6, 128), (3, 4000, 1.6, 128),
+                                                (9, 4000, 1.6, 128), (1, 6000, 8.0, 64)])
+def test_gemv_device_matches_dense(nrhs, n, height, leaf):
     import torch
-    say(f"gemv device nrhs={nrhs}")
-    pts, t, h = build(4000, 1.6, 5)
+    say(f"gemv device nrhs={nrhs} n={n} leaf={leaf}")
+    pts, t, h = build(n, height, 5, leaf=leaf)
     D = h.to_dense()
     rng = np.random.default_rng(nrhs)
     x = rng.standard_normal((h.n, nrhs))
