@@ -165,9 +165,12 @@ void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
   upload(dsvd, svd.data(), svd.size() * sizeof(int), stream);
   launch_assemble_full<double>(s64.desc.as<LeafDesc>(), dfull.as<int>(), (int)full.size(),
                                s64.data(), rp, cp, kp, stream);
+  // compress() follows the SVD with rk->truncate(lowRankEpsilon) (compression.cpp:759): the
+  // QR + core SVD of an SVD factorisation has the same singular values, so the two cuts
+  // together keep sigma_i > max(eps, lowrank_eps) * sigma_0
   launch_svd_compress<double>(s64.desc.as<LeafDesc>(), dsvd.as<int>(), (int)svd.size(), s64.data(),
-                              s64.ranks.as<int>(), overflow_.as<int>(), rp, cp, kp, comp.eps,
-                              stream);
+                              s64.ranks.as<int>(), overflow_.as<int>(), rp, cp, kp,
+                              std::max(comp.eps, lowrank_eps), stream);
   // ACA blocks in batches bounded by a workspace budget.
   const int64_t budget = (int64_t)4 << 27;  // doubles (4 GiB)
   const int partial = comp.method == 2;

@@ -24,6 +24,12 @@
 #include "engine.hpp"
 #include "factor_kernels.hpp"
 
+// Built with -DHMX_DIAGNOSTICS=1, HMX_TRACE_LAUNCHES=<seconds> times every
+// batched launch of the factorisation on the host (off in the library).
+#ifndef HMX_DIAGNOSTICS
+#define HMX_DIAGNOSTICS 0
+#endif
+
 namespace hmx {
 
 namespace {
@@ -151,9 +157,11 @@ struct FactorState {
 };
 
 namespace {
-// launch time per task type under HMX_TRACE_LAUNCHES (diagnostics)
+#if HMX_DIAGNOSTICS
+// launch time per task type (diagnostics build only)
 double g_type_seconds[32];
 long g_type_launches[32];
+#endif
 
 template <typename T>
 void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool capturing = false) {
@@ -162,8 +170,9 @@ void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool cap
   if (P.ranks_used) launch_fill_int(fs.ranks.as<int>(), (size_t)P.ranks_used, 0, s);
   const FTask* dt = P.tasks.as<FTask>();
   const GemmTile* dtl = P.tiles.as<GemmTile>();
+#if HMX_DIAGNOSTICS
   // HMX_TRACE_LAUNCHES=<seconds>: time every launch with events and report
-  // the launches slower than that on stderr (diagnostics; not for capture)
+  // the launches slower than that on stderr (not for capture)
   static const double trace_env = getenv("HMX_TRACE_LAUNCHES") ? atof(getenv("HMX_TRACE_LAUNCHES")) : -1.0;
   const double trace = capturing ? -1.0 : trace_env;
   std::vector<hipEvent_t> ev;
@@ -172,6 +181,9 @@ void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool cap
     for (auto& e : ev) HMX_CHECK(hipEventCreate(&e));
     HMX_CHECK(hipEventRecord(ev[0], s));
   }
+#else
+  constexpr double trace = -1.0;
+#endif
   const bool side_by_side = !capturing && trace < 0;
   if (side_by_side) fs.ensure_side();
   for (size_t q = 0; q < P.launches.size();) {
@@ -184,7 +196,9 @@ void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool cap
       const SegLaunch& L = P.launches[q];
       launch_task_kernel<T>(L.type, dt + L.first, L.count, dtl + L.tfirst, L.tcount,
                             fs.ws.as<double>(), fs.flags.as<int>(), s);
+#if HMX_DIAGNOSTICS
       if (trace >= 0) HMX_CHECK(hipEventRecord(ev[q + 1], s));
+#endif
       q = e;
       continue;
     }
@@ -204,6 +218,7 @@ void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool cap
     }
     q = e;
   }
+#if HMX_DIAGNOSTICS
   if (trace >= 0) {
     HMX_CHECK(hipStreamSynchronize(s));
     for (size_t q = 0; q < P.launches.size(); ++q) {
@@ -219,6 +234,7 @@ void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool cap
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
   }
+#endif
 }
 }  // namespace
 
@@ -381,9 +397,7 @@ class Exec {
   }
 
   void emit(FTask& t, int64_t ws_doubles = 0) {
-    // HMX_SERIAL_TASKS: one task per wave (diagnostics: rules out scheduling races)
-    static const bool serial = getenv("HMX_SERIAL_TASKS") != nullptr;
-    int wave = serial ? max_wave_ + 1 : 1;
+    int wave = 1;
     for (uint64_t k : r_) {
       auto it = units_.find(k);
       if (it != units_.end()) wave = std::max(wave, it->second.lw + 1);
@@ -583,9 +597,10 @@ class Exec {
   }
 
   // Rk target += alpha * dense part (at offsets), eval + truncated SVD.
-  void fadd_dense(const RV& tgt, double alpha, const DV& part, int roff, int coff) {
+  void fadd_dense(const RV& tgt, double alpha, const DV& part, int roff, int coff, bool replace = false) {
     FTask t{};
     t.type = T_FADD_DENSE;
+    t.mode = replace ? 1 : 0;
     t.eps = eps_;
     t.tgt = ref(tgt);
     t.alpha = alpha;
@@ -808,7 +823,33 @@ class Exec {
       hgemv(a, ta, 1.0, X, out, dim(bc->size));
       return out;
     }
-    throw std::runtime_error("multiply_full: full x H products are not supported");
+    if (a->kind == Kind::Full && !b->is_leaf()) {
+      // multiplyFullH h_matrix.cpp:1623-1646: op(M) op(H) = (op(H)^T op(M)^T)^T,
+      // the transposed product by H-gemv, then one transposing copy
+      DV outT = temp_dense(bc->size, ar->size, bc->offset);
+      zero(outT);
+      DV X = full_of(a);
+      X.trans = (ta == 'N');  // op(M)^T
+      hgemv(b, tb == 'N' ? 'T' : 'N', 1.0, X, outT, dim(ar->size));
+      DV out = temp_dense(ar->size, bc->size, ar->offset);
+      FTask t{};
+      t.type = T_AXPY;
+      t.mode = 1;
+      t.ta = 1;
+      t.m = dim(ar->size);
+      t.n = dim(bc->size);
+      t.alpha = 1.0;
+      t.A = outT.p;
+      t.lda = outT.ld;
+      t.C = out.p;
+      t.ldc = out.ld;
+      rd(outT);
+      wr(out);
+      emit(t);
+      release();  // outT
+      return out;
+    }
+    throw std::runtime_error("multiply_full: unsupported operand combination");
   }
 
   // ---- axpy into the tree (HMatrix::axpy h_matrix.cpp:976-1073) ---------------
@@ -879,6 +920,84 @@ class Exec {
       wr(C);
       emit(t);
     }
+  }
+
+  // ---- dense evaluation of an H block and its write-back ----------------------------
+  DV sub_of(const DV& D, int ro, int co, int m, int n) const {
+    DV s = D;
+    s.p = (char*)D.p + sizeof(T) * ((size_t)ro + (size_t)co * D.ld);
+    s.m = m;
+    s.n = n;
+    if (D.row0 >= 0) s.row0 = D.row0 + ro;
+    return s;
+  }
+  template <typename F>
+  static void for_each_leaf(const Block* b, F&& f) {
+    if (b->is_leaf()) {
+      f(b);
+      return;
+    }
+    for (int q = 0; q < b->nrow * b->ncol; ++q)
+      if (b->child[q]) for_each_leaf(b->child[q], f);
+  }
+  // HMatrix::evalPart: a dense temporary holding every leaf of b (released by the caller)
+  DV eval_block(const Block* b) {
+    DV D = temp_dense(b->m(), b->n(), b->rows->offset);
+    zero(D);
+    for_each_leaf(b, [&](const Block* lf) {
+      DV sub = sub_of(D, lf->rows->offset - b->rows->offset, lf->cols->offset - b->cols->offset,
+                      lf->m(), lf->n());
+      if (lf->kind == Kind::Full) {
+        DV F = full_of(lf);
+        FTask t{};
+        t.type = T_AXPY;
+        t.mode = 1;
+        t.m = dim(lf->m());
+        t.n = dim(lf->n());
+        t.alpha = 1.0;
+        t.A = F.p;
+        t.lda = F.ld;
+        t.C = sub.p;
+        t.ldc = sub.ld;
+        rd(F);
+        wr(sub);
+        emit(t);
+      } else {
+        RV r = rk_of(lf);
+        DV A = dv_panel(r.a, r.la, r.m, r.cap, r.bufa, r.row0);
+        DV B = dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0);
+        B.trans = true;
+        gemm(sub, true, 1.0, A, B, dim(lf->m()), dim(lf->n()), dim(r.cap, r.rank));
+      }
+    });
+    return D;
+  }
+  // HMatrix::clear + HMatrix::axpy(1, FullMatrix) h_matrix.cpp:2201-2215, 1034-1073:
+  // full leaves take their part of D, Rk leaves the truncated SVD of it
+  // (RkMatrix::formattedAddParts with a full part on an empty Rk, rk_matrix.cpp:984-1010).
+  void clear_and_set(const Block* b, const DV& D) {
+    for_each_leaf(b, [&](const Block* lf) {
+      DV sub = sub_of(D, lf->rows->offset - b->rows->offset, lf->cols->offset - b->cols->offset,
+                      lf->m(), lf->n());
+      if (lf->kind == Kind::Full) {
+        DV C = full_of(lf);
+        FTask t{};
+        t.type = T_AXPY;
+        t.mode = 1;
+        t.m = dim(lf->m());
+        t.n = dim(lf->n());
+        t.alpha = 1.0;
+        t.A = sub.p;
+        t.lda = sub.ld;
+        t.C = C.p;
+        t.ldc = C.ld;
+        rd(sub);
+        wr(C);
+        emit(t);
+      } else {
+        fadd_dense(rk_of(lf), 1.0, sub, 0, 0, true);
+      }
+    });
   }
 
   // ---- RkMatrix::gemmRk rk_matrix.cpp:1352-1415 -------------------------------
@@ -1011,8 +1130,10 @@ class Exec {
       gemm_rk(tg, ta, tb, alpha, a, b);
       return;
     }
-    if (!a->is_leaf() && !b->is_leaf())
-      throw std::runtime_error("H-gemm: full leaf target with two H operands is not supported");
+    if (!a->is_leaf() && !b->is_leaf()) {
+      full_hh_gemm(c, ta, tb, alpha, a, b);
+      return;
+    }
     DV C = full_of(c);
     if (a->kind == Kind::Rk || b->kind == Kind::Rk) {
       int owned = 0;
@@ -1047,6 +1168,27 @@ class Exec {
     wr(C);
     emit(t);
     release();
+  }
+
+  // fullHHGemm h_matrix.cpp:1449-1472: a full leaf target and two H operands;
+  // recurse on the operands' children (the target's clusters are leaves, so
+  // the children keep its rows / columns) until one operand is a leaf.
+  void full_hh_gemm(const Block* c, char ta, char tb, double alpha, const Block* a, const Block* b) {
+    if (!a->is_leaf() && !b->is_leaf()) {
+      const int ni = ta == 'N' ? a->nrow : a->ncol;
+      const int nj = tb == 'N' ? b->ncol : b->nrow;
+      const int nk = ta == 'N' ? a->ncol : a->nrow;
+      for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nj; ++j)
+          for (int k = 0; k < nk; ++k) {
+            char tA = ta, tB = tb;
+            const Block* ca = child_for_gemm(a, tA, i, k);
+            const Block* cb = child_for_gemm(b, tB, k, j);
+            if (ca && cb) full_hh_gemm(c, tA, tB, alpha, ca, cb);
+          }
+      return;
+    }
+    hgemm(c, ta, tb, alpha, a, b);
   }
 
   // ---- triangular solves -----------------------------------------------------------
@@ -1119,7 +1261,14 @@ class Exec {
       }
       return;
     }
-    if (!b->is_leaf()) throw std::runtime_error("solveUpperTriangularRight: H right-hand side with a leaf diagonal");
+    if (!b->is_leaf()) {
+      // h_matrix.cpp:2386-2397: evaluate b, solve the dense block, clear b and add it back
+      DV D = eval_block(b);
+      trsm(0, L, D, dim(D.m));
+      clear_and_set(b, D);
+      release();
+      return;
+    }
     if (b->kind == Kind::Full) {
       if (!L->is_leaf()) throw std::runtime_error("solveUpperTriangularRight: full block with an H diagonal");
       DV X = full_of(b);
@@ -1228,8 +1377,16 @@ class Exec {
       return;
     }
     if (!b->is_leaf()) {
-      if (b->nrow != 1) throw std::runtime_error("solveLowerTriangularLeft: H right-hand side with a leaf diagonal");
-      for (int k = 0; k < b->ncol; ++k) lu_solve_lower_left_h(L, b->get(0, k));
+      if (b->nrow == 1) {
+        // h_matrix.cpp:2305-2311: one block row, solve column block by column block
+        for (int k = 0; k < b->ncol; ++k) lu_solve_lower_left_h(L, b->get(0, k));
+        return;
+      }
+      // h_matrix.cpp:2313-2320: evaluate b, solve, clear b and add it back
+      DV D = eval_block(b);
+      lu_solve_lower_left(L, D, dim(D.n));
+      clear_and_set(b, D);
+      release();
       return;
     }
     if (b->kind == Kind::Full) {
@@ -1265,8 +1422,11 @@ class Exec {
       return;
     }
     if (!b->is_leaf()) {
-      if (b->ncol != 1) throw std::runtime_error("solveUpperTriangularRight: H right-hand side with a leaf diagonal");
-      for (int k = 0; k < b->nrow; ++k) lu_solve_upper_right_h(U, b->get(k, 0));
+      // h_matrix.cpp:2386-2397: evaluate b, solve, clear b and add it back
+      DV D = eval_block(b);
+      trsm(0 | kTrsmTransA, U, D, dim(D.m));
+      clear_and_set(b, D);
+      release();
       return;
     }
     if (b->kind == Kind::Full) {
@@ -1425,36 +1585,8 @@ class Exec {
     plan->ranks_used = fs_.rank_used;
     plan->ntasks = (long)n;
     plan->waves = W;
-    static const bool prof = getenv("HMX_PROF_FADD") != nullptr;
-    DevBuf pbuf;
-    if (prof) {
-      // task ids for the per-task phase clocks
-      for (size_t q = 0; q < n; ++q) sorted[q].pad = (int)q;
-      upload(plan->tasks, sorted.data(), n * sizeof(FTask), s);
-      pbuf.alloc(n * 8 * sizeof(double));
-      HMX_CHECK(hipMemsetAsync(pbuf.p, 0, n * 8 * sizeof(double), s));
-      g_fadd_prof = pbuf.as<double>();
-    }
-    static const bool trace_seg = getenv("HMX_TRACE_LAUNCHES") != nullptr;
-    const double te0 = trace_seg ? now_s() : 0.0;
     execute_plan<T>(*plan, fs_, s);
     HMX_CHECK(hipStreamSynchronize(s));
-    if (trace_seg)
-      fprintf(stderr, "segment: %zu tasks, %zu launches, execute %.4f s\n", n, plan->launches.size(),
-              now_s() - te0);
-    if (prof) {
-      g_fadd_prof = nullptr;
-      std::vector<double> h(n * 8);
-      HMX_CHECK(hipMemcpy(h.data(), pbuf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-      FILE* f = fopen(getenv("HMX_PROF_FADD"), "a");
-      if (f) {
-        for (size_t q = 0; q < n; ++q)
-          if (sorted[q].type == T_FADD)
-            fprintf(f, "%d %.12g %.12g %.12g %.12g %.12g %.12g %.12g %.12g\n", waves_[order[q]], h[8 * q], h[8 * q + 1], h[8 * q + 2],
-                    h[8 * q + 3], h[8 * q + 4], h[8 * q + 5], h[8 * q + 6], h[8 * q + 7]);
-        fclose(f);
-      }
-    }
     if (record_) record_->push_back(plan);
     M_.stats.tasks += (long)n;
     M_.stats.waves += W;
@@ -1475,14 +1607,17 @@ class Exec {
   void record_into(PlanList* p) { record_ = p; }
   void set_lu(bool v) { lu_mode_ = v; }
   void report(double total) const {
-    if (getenv("HMX_TRACE_LAUNCHES"))
-    {
+#if HMX_DIAGNOSTICS
+    if (getenv("HMX_TRACE_LAUNCHES")) {
       fprintf(stderr, "factorisation: total %.4f s, segments %.4f s, temp allocations %.4f s, %zu units\n",
               total, t_seg_, t_alloc_, units_.size());
       for (int q = 0; q < 32; ++q)
         if (g_type_launches[q])
           fprintf(stderr, "  task type %d: %ld launches, %.4f s\n", q, g_type_launches[q], g_type_seconds[q]);
     }
+#else
+    (void)total;
+#endif
   }
 
  private:
@@ -1601,30 +1736,7 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st) {
       ex.solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
       ex.solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
     }
-    DevBuf tprof;
-    if (getenv("HMX_TRACE_LAUNCHES")) {
-      for (int q = 0; q < 32; ++q) g_type_seconds[q] = 0.0, g_type_launches[q] = 0;
-      tprof.ensure(8 * sizeof(double));
-      HMX_CHECK(hipMemset(tprof.p, 0, 8 * sizeof(double)));
-      g_trsm_prof = tprof.as<double>();
-    }
     ex.flush(false);
-    if (g_trsm_prof) {
-      double h[8];
-      HMX_CHECK(hipStreamSynchronize(stream));
-      HMX_CHECK(hipMemcpy(h, g_trsm_prof, sizeof(h), hipMemcpyDeviceToHost));
-      g_trsm_prof = nullptr;
-      const double nwg = h[4] > 0 ? h[4] : 1.0;
-      fprintf(stderr, "trsm per workgroup (cycles): staging %.0f (task read %.0f, L %.0f), inverses %.0f, steps %.0f, write %.0f; %.0f workgroups, %.0f fallbacks\n",
-              h[0] / nwg, h[5] / nwg, h[6] / nwg, h[1] / nwg, h[2] / nwg, h[3] / nwg, h[4], h[7]);
-    }
-    if (getenv("HMX_TRACE_LAUNCHES")) {
-      HMX_CHECK(hipStreamSynchronize(stream));
-      fprintf(stderr, "solve recording (nrhs %d): %.4f s\n", nrhs, now_s() - t0);
-      for (int q = 0; q < 32; ++q)
-        if (g_type_launches[q])
-          fprintf(stderr, "  task type %d: %ld launches, %.4f s\n", q, g_type_launches[q], g_type_seconds[q]);
-    }
     it = fs.solve_plans.emplace(key, std::move(plans)).first;
   } else {
     HMX_CHECK(hipMemcpy2DAsync(rhs.p, (size_t)n * sizeof(T), b, (size_t)ldb * sizeof(T),

@@ -16,14 +16,6 @@ int64_t fadd_workspace(int m, int n, int kmax_concat);
 int64_t fadd_dense_workspace(int m, int n);
 int64_t rkrk_workspace(int k1, int k2);
 
-// When non-null, k_fadd records 8 doubles per task at g_fadd_prof[8 * task.pad]
-// (path, m, n, K, total clocks, set-up, QR, SVD + output; the SVD field
-// carries 1e-3 x the Jacobi sweep count) for performance analysis.
-extern double* g_fadd_prof;
-// When non-null, k_trsm adds its phase cycles (staging, block inverses, block
-// steps, write-back), its workgroup count and its fallbacks at [0..8) (diagnostics).
-extern double* g_trsm_prof;
-
 // p[0..n) = v on stream s (a kernel, so it may be captured in a graph).
 void launch_fill_int(int* p, size_t n, int v, hipStream_t s);
 

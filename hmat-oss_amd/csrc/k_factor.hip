@@ -448,11 +448,9 @@ constexpr int kTrsmRegs = 8;  // block-step results per thread: 16 x 8 x 256 cov
 constexpr int kTrsmSplit = 2;
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* fail, double* prof) {
+__global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* fail) {
   __shared__ double sh[kShDoubles];
   if (*fail) return;
-  long tp[6];
-  tp[0] = clock64();
   FTask t = tasks[blockIdx.x / kTrsmSplit];
   {
     const int all = eff(t.m), half = blockIdx.x % kTrsmSplit;
@@ -475,7 +473,6 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
   const int64_t need = (int64_t)n * n + (int64_t)nb * kNb * kNb + (int64_t)xl * xc;
   if (need > kShDoubles || other > kNb * kTrsmRegs) {
     trsm_rowwise<T>(t, sh);
-    if (prof && threadIdx.x == 0) atomicAdd(prof + 7, 1.0);
     return;
   }
   double* Ls = sh;                          // n x n, ld n (L, or U^T with transa)
@@ -485,20 +482,13 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
   const int ldl = t.lda;
   T* X = (T*)t.C;
   const int ldx = t.ldc;
-  tp[5] = clock64();
   if (transa)
     stage_in(Ls, n, Lg, (long)ldl, 1L, n, n);
   else
     stage_in(Ls, n, Lg, 1L, (long)ldl, n, n);
   __syncthreads();
-  const long tl = clock64();
   stage_in(Xs, xl, X, 1L, (long)ldx, xr, xc);
   __syncthreads();
-  tp[1] = clock64();
-  if (prof && threadIdx.x == 0) {
-    atomicAdd(prof + 5, (double)(tp[5] - tp[0]));
-    atomicAdd(prof + 6, (double)(tl - tp[5]));
-  }
   if ((t.mode & kTrsmPiv) && mode != 0) {
     const int* piv = (const int*)t.B;
     for (int c = threadIdx.x; c < other; c += kBlock)
@@ -540,7 +530,6 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
       if (r < bs) inv[r + c * kNb] = x[r];
   }
   __syncthreads();
-  tp[2] = clock64();
   // Every block step is two small products on the matrix cores (16 x 16
   // tiles, inner dimension the block's bs <= 16 columns of L): the block of
   // X times the inverted diagonal block, held in registers across a barrier,
@@ -618,21 +607,9 @@ __global__ void __launch_bounds__(kBlock) k_trsm(const FTask* tasks, const int* 
     }
     __syncthreads();
   }
-  tp[3] = clock64();
   for (int e = threadIdx.x; e < xr * xc; e += kBlock) {
     const int i = e % xr, j = e / xr;
     X[i + (size_t)j * ldx] = (T)Xs[i + j * xl];
-  }
-  if (prof) {
-    __syncthreads();
-    tp[4] = clock64();
-    if (threadIdx.x == 0) {
-      atomicAdd(prof + 0, (double)(tp[1] - tp[0]));
-      atomicAdd(prof + 1, (double)(tp[2] - tp[1]));
-      atomicAdd(prof + 2, (double)(tp[3] - tp[2]));
-      atomicAdd(prof + 3, (double)(tp[4] - tp[3]));
-      atomicAdd(prof + 4, 1.0);
-    }
   }
 }
 
@@ -679,7 +656,10 @@ __global__ void k_axpy(const FTask* tasks, const int* fail) {
   for (int e = threadIdx.x; e < m * n; e += blockDim.x) {
     const int i = e % m, j = e / m;
     const T v = t.ta ? A[j + (size_t)i * t.lda] : A[i + (size_t)j * t.lda];
-    C[i + (size_t)j * t.ldc] += al * v;
+    if (t.mode)
+      C[i + (size_t)j * t.ldc] = al * v;
+    else
+      C[i + (size_t)j * t.ldc] += al * v;
   }
 }
 
@@ -945,14 +925,11 @@ __device__ void qr_finish(const double* Va, int lda, int m, const double* Vb, in
 // FADD: RkMatrix::formattedAddParts (rk_matrix.cpp:846-981)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsbase, int* overflow,
-                                                 double* prof) {
+__global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsbase, int* overflow) {
   __shared__ double sh[kShDoubles];
   __shared__ int flag;
   if (overflow[1]) return;
   const FTask& t = tasks[blockIdx.x];
-  const long t0 = wall_clock64();
-  double* pr = prof ? prof + (size_t)8 * t.pad : nullptr;
   const RkRef tg = t.tgt;
   const int m = tg.m, n = tg.n;
   const int kt = rank_of(tg);
@@ -991,9 +968,6 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     }
     __syncthreads();
     if (threadIdx.x == 0) *tg.rank = k;
-    if (pr && threadIdx.x == 0) {
-      pr[0] = 0; pr[1] = m; pr[2] = n; pr[3] = K; pr[4] = (double)(wall_clock64() - t0);
-    }
     return;
   }
   double* gws = wsbase + t.ws;
@@ -1026,13 +1000,8 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     }
     __syncthreads();
     double* V = G + (size_t)m * n;
-    const long t1 = wall_clock64();
     dense_to_rk<T>(G, gr, gc, tr, V, V + (size_t)gc * gc, (int*)(V + (size_t)gc * gc + gc),
                    fabs(t.eps), tg, overflow, &flag);
-    if (pr && threadIdx.x == 0) {
-      pr[0] = 1; pr[1] = m; pr[2] = n; pr[3] = K; pr[4] = (double)(wall_clock64() - t0);
-      pr[5] = (double)(t1 - t0); pr[6] = (G == sh);
-    }
     return;
   }
   // QR path (truncate, rk_matrix.cpp:490-585) on the concatenated panels
@@ -1072,7 +1041,6 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     mode = cr == sh ? 3 : 4;
   }
   const QrScratch w = carve_qr(cr, K);
-  const long t1 = wall_clock64();
   if (mode == 5) {
     concat_side<T>(Va, lda, m, false, t, kt);
     concat_side<T>(Vb, ldb, n, true, t, kt);
@@ -1138,17 +1106,11 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     core_product(Va, lda, Vb, ldb, K, w.g);
   }
   __syncthreads();
-  const long t2 = wall_clock64();
   int sweeps = 0;
   // the core SVD runs on one wavefront when its scratch is in LDS
   const bool core_lds = mode == 5 || mode == 6 || cr == sh;
   qr_finish<T>(Va, lda, m, Vb, ldb, n, K, w, core_lds && K <= kWaveJacobiMax, fabs(t.eps), tg, overflow, &flag,
                &sweeps);
-  if (pr && threadIdx.x == 0) {
-    pr[0] = mode; pr[1] = m; pr[2] = n; pr[3] = K;
-    pr[4] = (double)(wall_clock64() - t0); pr[5] = (double)(t1 - t0);
-    pr[6] = (double)(t2 - t1); pr[7] = (double)(wall_clock64() - t2) + 1e-3 * sweeps;
-  }
 }
 
 // FADD_DENSE: RkMatrix::formattedAddParts with a full part (rk_matrix.cpp:984-1010)
@@ -1159,7 +1121,8 @@ __global__ void __launch_bounds__(kBlock) k_fadd_dense(const FTask* tasks, doubl
   if (overflow[1]) return;
   const FTask& t = tasks[blockIdx.x];
   const RkRef tg = t.tgt;
-  const int m = tg.m, n = tg.n, kt = rank_of(tg);
+  // mode 1 replaces the target (its current panels are ignored)
+  const int m = tg.m, n = tg.n, kt = t.mode ? 0 : rank_of(tg);
   const T* TA = (const T*)tg.a;
   const T* TB = (const T*)tg.b;
   const T* P = (const T*)t.A;
@@ -1370,9 +1333,6 @@ int64_t rkrk_workspace(int k1, int k2) {
   return (a * b + b * b + 2 * b + 1 + (a + b) * b + 127) / 64 * 64;
 }
 
-double* g_fadd_prof = nullptr;
-double* g_trsm_prof = nullptr;
-
 namespace {
 __global__ void k_fill_int(int* p, size_t n, int v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1399,10 +1359,10 @@ void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile*
       hipLaunchKernelGGL(k_potrf<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
       break;
     case T_TRSM:
-      hipLaunchKernelGGL(k_trsm<T>, dim3(count * kTrsmSplit), dim3(kBlock), 0, s, tasks, flags + 1, g_trsm_prof);
+      hipLaunchKernelGGL(k_trsm<T>, dim3(count * kTrsmSplit), dim3(kBlock), 0, s, tasks, flags + 1);
       break;
     case T_FADD:
-      hipLaunchKernelGGL(k_fadd<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags, g_fadd_prof);
+      hipLaunchKernelGGL(k_fadd<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags);
       break;
     case T_FADD_DENSE:
       hipLaunchKernelGGL(k_fadd_dense<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags);

@@ -300,17 +300,24 @@ class HMatrix:
 
     # host-pointer API entries (user numbering unless stated)
     def gemv(self, x, alpha=1.0, beta=0.0, y=None, permute=True):
-        x = np.asfortranarray(x, dtype=np.float64 if self.value_type else np.float32)
+        dt = np.float64 if self.value_type else np.float32
+        x = np.asfortranarray(x, dtype=dt)
         nrhs = 1 if x.ndim == 1 else x.shape[1]
+        if x.shape[0] != self.n:
+            raise ValueError(f"gemv: x has {x.shape[0]} rows, the matrix {self.n} columns")
         if y is None:
             y = np.zeros_like(x)
-        y = np.asfortranarray(y)
+        elif y.shape != x.shape:
+            raise ValueError(f"gemv: y has shape {y.shape}, expected {x.shape}")
+        out = np.asfortranarray(y, dtype=dt)   # a converted copy when y's dtype/layout differs
         T = C.c_double if self.value_type else C.c_float
         a, b = T(alpha), T(beta)
         fn = self.iface.gemv if permute else self.iface.gemm_scalar
         _check(fn(b"N", C.cast(C.byref(a), _vp), self.handle, x.ctypes.data,
-                  C.cast(C.byref(b), _vp), y.ctypes.data, nrhs), "gemv")
-        return y
+                  C.cast(C.byref(b), _vp), out.ctypes.data, nrhs), "gemv")
+        if out is not y and isinstance(y, np.ndarray):
+            y[...] = out
+        return out
 
     def solve(self, b, permute=True):
         x = np.array(b, dtype=np.float64 if self.value_type else np.float32, order="F", copy=True)

@@ -19,35 +19,49 @@ def say(*a):
 
 
 def build(n, height, seed, leaf=128, kind=hmat_amd.KERNEL_EXPONENTIAL, length=0.5, nugget=1e-2,
-          method="aca_plus", vt=hmat_amd.HMAT_DOUBLE_PRECISION):
+          method="aca_plus", vt=hmat_amd.HMAT_DOUBLE_PRECISION, eps=EPS, lowrank_eps=EPS):
     pts = ho.cylinder_points(n, 1.0, height, seed)
     t = hmat_amd.ClusterTree(pts, leaf)
-    h = hmat_amd.HMatrix(t, value_type=vt)
-    h.assemble(kind=kind, length=length, nugget=nugget, eps=EPS, method=method)
+    h = hmat_amd.HMatrix(t, value_type=vt, lowrank_eps=lowrank_eps)
+    h.assemble(kind=kind, length=length, nugget=nugget, eps=eps, method=method)
     return pts, t, h
 
 
-def oracle_build(pts, leaf=128, length=0.5, nugget=1e-2, kernel=ho.ExpKernel):
+def oracle_build(pts, leaf=128, length=0.5, nugget=1e-2, kernel=ho.ExpKernel, eps=EPS,
+                 lowrank_eps=EPS):
     root, idx = ho.build_cluster_tree(pts, leaf)
     bt = ho.build_block_tree(root, root, 2.0, True)
     k = kernel(pts, length, nugget)
-    ho.assemble(bt, idx, k)
+    ho.assemble(bt, idx, k, eps=eps, lowrank_eps=lowrank_eps)
     return bt, idx, k
 
 
+def _near_cut(block, rank_a, rank_b, cut):
+    """True when a singular value between the two ranks lies within 1e-8 of
+    the cut (a tie that rounding may decide either way)."""
+    s = np.linalg.svd(block, compute_uv=False)
+    lo, hi = min(rank_a, rank_b), max(rank_a, rank_b)
+    return all(abs(s[i] / s[0] - cut) <= 1e-8 * cut for i in range(lo, hi))
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,height,seed,leaf", [(3000, 1.2, 3, 128), (6000, 2.4, 11, 64)])
-def test_assembly_leaves_match_oracle(n, height, seed, leaf):
-    say(f"assembly parity n={n} leaf={leaf} seed={seed}")
-    pts, t, h = build(n, height, seed, leaf)
-    bt, idx, k = oracle_build(pts, leaf)
+@pytest.mark.parametrize("n,height,seed,leaf,eps,lowrank_eps",
+                         [(3000, 1.2, 3, 128, EPS, EPS), (6000, 2.4, 11, 64, EPS, EPS),
+                          (3000, 1.2, 3, 128, 1e-6, 1e-3)])
+def test_assembly_leaves_match_oracle(n, height, seed, leaf, eps, lowrank_eps):
+    say(f"assembly parity n={n} leaf={leaf} seed={seed} eps={eps} lowrank_eps={lowrank_eps}")
+    pts, t, h = build(n, height, seed, leaf, eps=eps, lowrank_eps=lowrank_eps)
+    bt, idx, k = oracle_build(pts, leaf, eps=eps, lowrank_eps=lowrank_eps)
     assert np.array_equal(t.indices, idx)
     oleaves = ho.leaves(bt)
     gleaves = list(h.leaves())
     assert len(oleaves) == len(gleaves)
+    tol = max(eps, lowrank_eps)
     worst_full = 0.0
     worst_rk = 0.0
-    rank_diff = []
+    worst_vs_oracle = 0.0
+    mismatch = {"svd": 0, "aca": 0}
+    count = {"svd": 0, "aca": 0}
     for ob, (m, nn, r0, c0, kind, rank, a, b) in zip(oleaves, gleaves):
         assert (ob.rows.offset, ob.m, ob.cols.offset, ob.n) == (r0, m, c0, nn)
         exact = k.block(idx[r0:r0 + m], idx[c0:c0 + nn])
@@ -59,14 +73,25 @@ def test_assembly_leaves_match_oracle(n, height, seed, leaf):
             g = a @ b.T if rank > 0 else np.zeros((m, nn))
             o = ob.rk.eval(m, nn, np.float64)
             nrm = np.linalg.norm(exact)
-            # both are eps-approximations of the same block
+            # both are tol-approximations of the same block
             worst_rk = max(worst_rk, np.linalg.norm(g - exact) / nrm)
-            assert np.linalg.norm(g - o) <= 4 * EPS * nrm
-            rank_diff.append(abs(rank - ob.rk.rank))
-    say("worst full", worst_full, "worst rk", worst_rk, "max rank diff", max(rank_diff))
+            worst_vs_oracle = max(worst_vs_oracle, np.linalg.norm(g - o) / nrm)
+            cls = "svd" if max(m, nn) < 100 else "aca"
+            count[cls] += 1
+            if rank != ob.rk.rank:
+                mismatch[cls] += 1
+                if cls == "svd":
+                    # SVD leaves: the same singular values cut by the same rule
+                    assert _near_cut(exact, rank, ob.rk.rank, tol), (r0, c0, rank, ob.rk.rank)
+                else:
+                    assert abs(rank - ob.rk.rank) <= 1, (r0, c0, rank, ob.rk.rank)
+    say("worst full", worst_full, "worst rk", worst_rk, "vs oracle", worst_vs_oracle,
+        "rank mismatches", mismatch, "of", count)
     assert worst_full < 1e-14
-    assert worst_rk < 4 * EPS
-    assert max(rank_diff) <= 2
+    assert worst_rk < 3 * tol
+    assert worst_vs_oracle < 3 * tol
+    # ACA pivot walks agree with the oracle's on all but a few leaves
+    assert mismatch["aca"] <= max(1, count["aca"] // 100)
     inf = h.info()
     oinf = ho.info(bt)
     assert inf["full_count"] == oinf["full_count"] and inf["rk_count"] == oinf["rk_count"]

@@ -1,0 +1,98 @@
+"""GPU parity of H-LLt and H-LU on partitions whose leaves sit at two depths
+(N=2050 with leaf 128, N=1030 with leaf 64).  These reach the recursion
+branches of h_matrix.cpp:2386-2397 (H right-hand side on a leaf diagonal),
+:1623 (Full x H) and :1449-1472 (fullHHGemm); the engine's factor, solve and
+logdet are compared with the oracle's on the same inputs."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import hmat_oracle as ho
+
+hmat_amd = pytest.importorskip("hmat_amd")
+
+EPS = 1e-4
+UNEVEN = [(2050, 1.0, 5, 128), (1030, 0.6, 5, 64)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,height,seed,leaf", UNEVEN)
+def test_llt_uneven_matches_oracle(n, height, seed, leaf):
+    say(f"llt uneven n={n} leaf={leaf}")
+    pts = ho.cylinder_points(n, 1.0, height, seed)
+    root, idx = ho.build_cluster_tree(pts, leaf)
+    assert len({c.depth for c in ho.cluster_leaves(root)}) > 1
+    t = hmat_amd.ClusterTree(pts, leaf)
+    assert np.array_equal(t.indices, idx)
+    h = hmat_amd.HMatrix(t)
+    h.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    H = h.to_dense()
+    h.factorize(hmat_amd.FACT_LLT)
+    L = np.tril(h.to_dense(mirror=False))
+    rec = np.linalg.norm(L @ L.T - H) / np.linalg.norm(H)
+    bt = ho.build_block_tree(root, root, 2.0, True)
+    k = ho.ExpKernel(pts, 0.5, 1e-2)
+    ho.assemble(bt, idx, k)
+    Ho = ho.to_dense(bt, n, n)
+    ho.llt(bt, EPS)
+    Lo = ho.factor_dense(bt, n)
+    rec_o = np.linalg.norm(Lo @ Lo.T - Ho) / np.linalg.norm(Ho)
+    ld_g, ld_o = h.logdet(), ho.logdet(bt)
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal((n, 2))
+    x = h.solve(b)
+    xo = b[idx].copy()
+    ho.solve_llt(bt, xo)
+    xou = np.empty_like(xo)
+    xou[idx] = xo
+    A = k.block(np.arange(n), np.arange(n))
+    res = np.linalg.norm(A @ x - b) / np.linalg.norm(b)
+    res_o = np.linalg.norm(A @ xou - b) / np.linalg.norm(b)
+    rel = np.linalg.norm(x - xou) / np.linalg.norm(xou)
+    say(f"rec gpu {rec:.3e} oracle {rec_o:.3e}; logdet gpu {ld_g:.12g} oracle {ld_o:.12g} "
+        f"rel {abs(ld_g - ld_o) / abs(ld_o):.3e}; residual gpu {res:.3e} oracle {res_o:.3e}; "
+        f"solution rel diff {rel:.3e}")
+    assert rec <= 3e-5
+    assert abs(ld_g - ld_o) <= 1e-6 * abs(ld_o)
+    assert rel <= 1e-4
+    assert res <= 1.5 * res_o + 1e-12
+    h.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,height,seed,leaf", UNEVEN)
+def test_lu_uneven_matches_oracle(n, height, seed, leaf):
+    say(f"lu uneven n={n} leaf={leaf}")
+    nugget, shift, eps = 0.2, 0.3, 1e-8
+    pts = ho.cylinder_points(n, 1.0, height, seed)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    h = hmat_amd.HMatrix(t, lower_symmetric=False, lowrank_eps=eps)
+    h.assemble(kind=hmat_amd.KERNEL_SHIFTED_EXPONENTIAL, length=0.5, nugget=nugget, eps=eps,
+               shift=shift)
+    h.factorize(hmat_amd.FACT_LU)
+    root, idx = ho.build_cluster_tree(pts, leaf)
+    bt = ho.build_block_tree(root, root, 2.0, False)
+    k = ho.ShiftedExpKernel(pts, 0.5, nugget, shift)
+    ho.assemble(bt, idx, k, eps=eps, lowrank_eps=eps)
+    ho.lu(bt, eps)
+    rng = np.random.default_rng(seed)
+    b = rng.standard_normal((n, 2))
+    x = h.solve(b)
+    xo = b[idx].copy()
+    ho.solve_lu(bt, xo)
+    xou = np.empty_like(xo)
+    xou[idx] = xo
+    A = k.block(np.arange(n), np.arange(n))
+    res = np.linalg.norm(A @ x - b) / np.linalg.norm(b)
+    res_o = np.linalg.norm(A @ xou - b) / np.linalg.norm(b)
+    rel = np.linalg.norm(x - xou) / np.linalg.norm(xou)
+    say(f"residual gpu {res:.3e} oracle {res_o:.3e}; solution rel diff {rel:.3e}")
+    assert rel <= 1e-5
+    assert res <= 1.5 * res_o + 1e-12
+    h.close()
