@@ -496,6 +496,24 @@ int hmx_gemv_device(hmat_matrix_t* h, int precision, double alpha, const void* x
   });
 }
 
+int hmx_gemv_device_noalloc(hmat_matrix_t* h, int precision, double alpha, const void* x,
+                            double beta, void* y, int nrhs) {
+  return guard([&] {
+    Matrix* m = M(h);
+    if (!m->assembled || !m->stream)
+      throw std::runtime_error("hmx_gemv_device_noalloc: matrix not assembled");
+    size_t elems = 0;
+    double* scratch = hmx::noalloc_gemv_scratch(&elems);
+    if (precision == 0)
+      m->gemv_device_scratch<double>(alpha, (const double*)x, beta, (double*)y, nrhs, m->s64,
+                                     scratch, elems);
+    else
+      m->gemv_device_scratch<float>(alpha, (const float*)x, beta, (float*)y, nrhs, m->s32,
+                                    scratch, elems);
+    if (!m->user_stream) HMX_CHECK(hipStreamSynchronize(m->stream));
+  });
+}
+
 int hmx_solve_device(hmat_matrix_t* h, int precision, void* b, int nrhs) {
   return guard([&] {
     Matrix* m = M(h);

@@ -211,15 +211,21 @@ void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
     s64.valid = false;
   }
   assembled = true;
+  plan_.built = false;
   plan_.nrhs = 0;
+  build_gemv_plan();
   stats.t_assemble = now() - t0;
 }
 
 // ---------------------------------------------------------------------------
 // H-gemv plan (host) and driver
 // ---------------------------------------------------------------------------
-void Matrix::build_gemv_plan(int nrhs) {
+// The item lists hold offsets in units of one right-hand side (the offsets for
+// nrhs right-hand sides are nrhs times these; the kernels scale them), so one
+// plan serves every nrhs and only the scratch buffers depend on it.
+void Matrix::build_gemv_plan() {
   GemvPlan& P = plan_;
+  const int nrhs = 1;
   const auto& D = s64.hdesc;
   const int nleaf = (int)D.size();
   const int chunk = kZChunk;
@@ -290,36 +296,80 @@ void Matrix::build_gemv_plan(int nrhs) {
   upload(P.yitems, yi.data(), yi.size() * sizeof(YItem), stream);
   upload(P.contrib, contrib.data(), contrib.size() * sizeof(YContrib), stream);
   upload(P.slotlist, slist.data(), slist.size() * sizeof(int64_t), stream);
-  P.zpart.ensure(std::max<int64_t>(zpart, 1) * sizeof(double));
-  P.z.ensure(std::max<int64_t>(zbuf, 1) * sizeof(double));
-  P.slots.ensure(std::max<int64_t>(slot, 1) * sizeof(double));
-  P.acc.ensure((size_t)rows->n * nrhs * sizeof(double));
+  P.zpart1 = zpart;
+  P.z1 = zbuf;
+  P.slot1 = slot;
   P.nz = (int)zi.size();
   P.nzred = (int)zr.size();
   P.ny = (int)yi.size();
-  P.nrhs = nrhs;
+  P.built = true;
+  P.nrhs = 0;
   HMX_CHECK(hipStreamSynchronize(stream));
+}
+
+size_t Matrix::gemv_scratch_elems(int nrhs) const {
+  const GemvPlan& P = plan_;
+  return (size_t)(P.zpart1 + P.z1 + P.slot1 + rows->n) * (size_t)nrhs;
+}
+
+template <typename T>
+void Matrix::gemv_launch(double alpha, const T* x, double beta, T* y, int nrhs, int ldx, int ldy,
+                         const Store<T>& st, double* zpart, double* z, double* slots,
+                         double* acc) {
+  const GemvPlan& P = plan_;
+  const LeafDesc* desc = st.desc.template as<LeafDesc>();
+  const int* ranks = st.ranks.template as<int>();
+  launch_gemv_zpass<T>(desc, ranks, st.data(), P.zitems.as<ZItem>(), P.nz, x, ldx, nrhs, zpart,
+                       stream);
+  launch_gemv_zreduce(P.zred.as<ZReduce>(), P.nzred, ranks, zpart, P.zitems.as<ZItem>(), nrhs,
+                      desc, z, stream);
+  launch_gemv_ypass<T>(desc, ranks, st.data(), P.yitems.as<YItem>(), P.ny,
+                       P.contrib.as<YContrib>(), z, slots, x, ldx, nrhs, acc, rows->n, stream);
+  launch_gemv_yfinal<T>(P.yitems.as<YItem>(), P.ny, P.slotlist.as<int64_t>(), slots, acc,
+                        rows->n, y, ldy, nrhs, alpha, beta, stream);
 }
 
 template <typename T>
 void Matrix::gemv_device(double alpha, const T* x, double beta, T* y, int nrhs, int ldx, int ldy,
                          const Store<T>& st) {
   if (!st.valid) throw std::runtime_error("gemv: matrix data not available in this precision");
-  if (plan_.nrhs != nrhs) build_gemv_plan(nrhs);
+  if (nrhs < 1) throw std::runtime_error("gemv: nrhs must be positive");
+  if (!plan_.built) build_gemv_plan();
   GemvPlan& P = plan_;
-  const LeafDesc* desc = st.desc.template as<LeafDesc>();
-  const int* ranks = st.ranks.template as<int>();
-  launch_gemv_zpass<T>(desc, ranks, st.data(), P.zitems.as<ZItem>(), P.nz, x, ldx, nrhs,
-                       P.zpart.as<double>(), stream);
-  launch_gemv_zreduce(P.zred.as<ZReduce>(), P.nzred, ranks, P.zpart.as<double>(),
-                      P.zitems.as<ZItem>(), nrhs, desc, P.z.as<double>(), stream);
-  launch_gemv_ypass<T>(desc, ranks, st.data(), P.yitems.as<YItem>(), P.ny,
-                       P.contrib.as<YContrib>(), P.z.as<double>(), P.slots.as<double>(), x, ldx,
-                       nrhs, P.acc.as<double>(), rows->n, stream);
-  launch_gemv_yfinal<T>(P.yitems.as<YItem>(), P.ny, P.slotlist.as<int64_t>(),
-                        P.slots.as<double>(), P.acc.as<double>(), rows->n, y, ldy, nrhs, alpha,
-                        beta, stream);
+  if (P.nrhs < nrhs) {
+    HMX_CHECK(hipStreamSynchronize(stream));
+    P.zpart.ensure(std::max<int64_t>(P.zpart1 * nrhs, 1) * sizeof(double));
+    P.z.ensure(std::max<int64_t>(P.z1 * nrhs, 1) * sizeof(double));
+    P.slots.ensure(std::max<int64_t>(P.slot1 * nrhs, 1) * sizeof(double));
+    P.acc.ensure((size_t)rows->n * nrhs * sizeof(double));
+    P.nrhs = nrhs;
+  }
+  gemv_launch<T>(alpha, x, beta, y, nrhs, ldx, ldy, st, P.zpart.as<double>(), P.z.as<double>(),
+                 P.slots.as<double>(), P.acc.as<double>());
 }
+
+template <typename T>
+void Matrix::gemv_device_scratch(double alpha, const T* x, double beta, T* y, int nrhs,
+                                 const Store<T>& st, double* scratch, size_t scratch_elems) {
+  if (!st.valid) throw std::runtime_error("gemv: matrix data not available in this precision");
+  if (!plan_.built) throw std::runtime_error("gemv: no H-gemv plan (matrix not assembled)");
+  if (nrhs < 1) throw std::runtime_error("gemv: nrhs must be positive");
+  const size_t need = gemv_scratch_elems(nrhs);
+  if (need > scratch_elems)
+    throw std::runtime_error("gemv: needs " + std::to_string(need * sizeof(double)) +
+                             " bytes of scratch, the allocation-free entry point has " +
+                             std::to_string(scratch_elems * sizeof(double)));
+  const GemvPlan& P = plan_;
+  double* zpart = scratch;
+  double* z = zpart + P.zpart1 * nrhs;
+  double* slots = z + P.z1 * nrhs;
+  double* acc = slots + P.slot1 * nrhs;
+  gemv_launch<T>(alpha, x, beta, y, nrhs, n_cols(), n_rows(), st, zpart, z, slots, acc);
+}
+template void Matrix::gemv_device_scratch<double>(double, const double*, double, double*, int,
+                                                  const Store<double>&, double*, size_t);
+template void Matrix::gemv_device_scratch<float>(double, const float*, double, float*, int,
+                                                 const Store<float>&, double*, size_t);
 template void Matrix::gemv_device<double>(double, const double*, double, double*, int, int, int,
                                           const Store<double>&);
 template void Matrix::gemv_device<float>(double, const float*, double, float*, int, int, int,

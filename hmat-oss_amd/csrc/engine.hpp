@@ -61,12 +61,15 @@ struct KernelSpec {
   std::vector<double> points;  // user numbering
 };
 
+// H-gemv plan: item lists with offsets per right-hand side (built once after
+// assembly), plus scratch buffers sized for the largest nrhs seen so far.
 struct GemvPlan {
-  int nrhs = 0;
+  bool built = false;
+  int nrhs = 0;       // scratch buffers below hold this many right-hand sides
   DevBuf zitems, zred, yitems, contrib, slotlist;
   int nz = 0, nzred = 0, ny = 0;
+  int64_t zpart1 = 0, z1 = 0, slot1 = 0;  // scratch doubles per right-hand side
   DevBuf zpart, z, slots, acc;
-  size_t zpart_elems = 0, z_elems = 0, slot_elems = 0;
 };
 
 struct Stats {
@@ -112,6 +115,12 @@ class Matrix {
   template <typename T>
   void gemv_device(double alpha, const T* x, double beta, T* y, int nrhs, int ldx, int ldy,
                    const Store<T>& st);
+  // The same product with every temporary in caller-provided device scratch
+  // (scratch_elems doubles); allocates nothing.  x has ld = N cols, y ld = N rows.
+  template <typename T>
+  void gemv_device_scratch(double alpha, const T* x, double beta, T* y, int nrhs,
+                           const Store<T>& st, double* scratch, size_t scratch_elems);
+  size_t gemv_scratch_elems(int nrhs) const;
   void gemv_host(char trans, const void* alpha, void* x, const void* beta, void* y, int nrhs,
                  bool permute);
 
@@ -136,7 +145,10 @@ class Matrix {
   void build_storage();
   template <typename T>
   void init_store(Store<T>& st);
-  void build_gemv_plan(int nrhs);
+  void build_gemv_plan();
+  template <typename T>
+  void gemv_launch(double alpha, const T* x, double beta, T* y, int nrhs, int ldx, int ldy,
+                   const Store<T>& st, double* zpart, double* z, double* slots, double* acc);
   GemvPlan plan_;
   DevBuf overflow_;
   std::shared_ptr<struct FactorState> fstate_;

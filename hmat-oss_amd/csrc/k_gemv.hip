@@ -9,6 +9,10 @@
 // Every stored leaf is streamed once per pass that needs it; accumulation is
 // in FP64 whatever the storage type.
 #include <algorithm>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -29,7 +33,7 @@ __global__ void __launch_bounds__(kBlock) k_zpass(const LeafDesc* desc, const in
   const int ld = it.side == 0 ? d.n : d.m;
   const T* P = arena + (it.side == 0 ? d.b : d.a) + it.row0;
   const T* xs = x + (it.side == 0 ? d.c0 : d.r0) + it.row0;
-  double* z = zpart + it.zslot;
+  double* z = zpart + it.zslot * nrhs;
   for (int l = w; l < k; l += kWaves) {
     const T* col = P + (size_t)l * ld;
     for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk) {
@@ -118,8 +122,8 @@ __global__ void k_zreduce(const ZReduce* red, const int* ranks, const double* zp
   for (int e = threadIdx.x; e < k * nrhs; e += blockDim.x) {
     const int l = e % k, r = e / k;
     double s = 0.0;
-    for (int q = rd.beg; q < rd.end; ++q) s += zpart[items[q].zslot + l + (size_t)r * cap];
-    z[rd.z + l + (size_t)r * cap] = s;
+    for (int q = rd.beg; q < rd.end; ++q) s += zpart[items[q].zslot * nrhs + l + (size_t)r * cap];
+    z[rd.z * nrhs + l + (size_t)r * cap] = s;
   }
 }
 
@@ -182,14 +186,14 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
         mkind[tid] = c.kind;
         if (c.kind == 0) {
           mp[tid] = arena + d.a;
-          mz[tid] = c.z >= 0 ? slots + c.z : nullptr;
+          mz[tid] = c.z >= 0 ? slots + c.z * nrhs : nullptr;
           mk[tid] = d.m;
           mn[tid] = d.n;
           mc0[tid] = d.c0;
         } else {
           const bool isA = c.kind == 2;
           mp[tid] = arena + (isA ? d.a : d.b) + c.row0;
-          mz[tid] = z + c.z;
+          mz[tid] = z + c.z * nrhs;
           mk[tid] = ranks[c.leaf];
           mld[tid] = isA ? d.m : d.n;
           mcap[tid] = d.cap;
@@ -363,7 +367,7 @@ __global__ void k_yfinal(const YItem* items, const int64_t* slot_list, const dou
   for (int e = threadIdx.x; e < it.len * nrhs; e += blockDim.x) {
     const int i = e % it.len, r = e / it.len;
     double s = acc[it.r0 + i + (size_t)r * ldacc];
-    for (int q = it.sbeg; q < it.send; ++q) s += slots[slot_list[q] + i + (size_t)r * it.len];
+    for (int q = it.sbeg; q < it.send; ++q) s += slots[slot_list[q] * nrhs + i + (size_t)r * it.len];
     T* py = y + it.r0 + i + (size_t)r * ldy;
     const double old = (beta == 0.0) ? 0.0 : beta * (double)(*py);
     *py = (T)(old + alpha * s);
@@ -531,5 +535,33 @@ void launch_gemv_yfinal(const YItem* items, int count, const int64_t* slot_list,
                                       hipStream_t);
 HMX_INST(double)
 HMX_INST(float)
+
+// Device scratch of the allocation-free H-gemv entry point: a fixed array in
+// the code object, so each GPU that loads the library holds its own copy (at
+// its own address) until the process ends.  Every call on one GPU shares it
+// and leaves its contents behind; the H-gemv passes write every scratch entry
+// they read, so nothing needs clearing between calls.  Host code must not use
+// the array's name as a device address: noalloc_gemv_scratch() returns the
+// device address looked up with hipGetSymbolAddress, once per device.
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+double* noalloc_gemv_scratch(size_t* elems) {
+  static std::mutex mu;
+  static std::vector<double*> ptr;
+  int dev = 0;
+  HMX_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)ptr.size() <= dev) ptr.resize(dev + 1, nullptr);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") +
+                               hipGetErrorString(e));
+    ptr[dev] = static_cast<double*>(p);
+  }
+  *elems = kNoAllocScratchBytes / sizeof(double);
+  return ptr[dev];
+}
 
 }  // namespace hmx

@@ -65,7 +65,8 @@ struct ZItem {
   int leaf;
   int side;       // 0: B panel against x[cols]  (z),  1: A panel against x[rows] (z')
   int row0, rows; // local row range inside the panel
-  int64_t zslot;  // output slot (cap * nrhs doubles)
+  int64_t zslot;  // output slot offset per right-hand side (the slot holds cap * nrhs doubles
+                  // at nrhs * zslot)
 };
 // y pass: one work item per destination leaf cluster.  Contributions:
 //   kind 0: full leaf with rows == destination: acc += M x[cols]; if the leaf is
@@ -84,8 +85,14 @@ struct YContrib {
   int kind;
   int row0;
   int pad;
-  int64_t z;       // z / z' offset (doubles) for rk, slot offset (doubles) for mirrored full, else -1
+  int64_t z;       // z / z' offset for rk, slot offset for mirrored full (doubles per
+                   // right-hand side: the data sits at nrhs * z), else -1
 };
+
+// Fixed device scratch of the allocation-free H-gemv (64 MiB per GPU); returns
+// its device address on the current device (looked up once per device).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+double* noalloc_gemv_scratch(size_t* elems);
 
 template <typename T>
 void launch_gemv_zpass(const LeafDesc* desc, const int* ranks, const T* arena, const ZItem* items,

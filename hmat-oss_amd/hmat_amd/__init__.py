@@ -122,6 +122,8 @@ _EXPORTS = {
     "hmat_mi355x_kernel_compute": (None, [_vp, C.c_int, C.c_int, _vp]),
     "hmx_last_error": (C.c_char_p, []),
     "hmx_gemv_device": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_double, _vp, C.c_int]),
+    "hmx_gemv_device_noalloc": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_double, _vp,
+                                          C.c_int]),
     "hmx_solve_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int]),
     "hmx_factorize_mixed": (C.c_int, [_vp, C.c_int]),
     "hmx_solve_refined_device": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int,
@@ -335,6 +337,12 @@ class HMatrix:
     def gemv_device(self, x_ptr, y_ptr, nrhs, alpha=1.0, beta=0.0, precision=0):
         _check(load().hmx_gemv_device(self.handle, precision, alpha, x_ptr, beta, y_ptr, nrhs),
                "hmx_gemv_device")
+
+    def gemv_device_noalloc(self, x_ptr, y_ptr, nrhs, alpha=1.0, beta=0.0, precision=0):
+        """hmx_gemv_device with the temporaries in the library's fixed device
+        scratch (allocates nothing; see include/hmat/hmat_mi355x.h)."""
+        _check(load().hmx_gemv_device_noalloc(self.handle, precision, alpha, x_ptr, beta, y_ptr,
+                                              nrhs), "hmx_gemv_device_noalloc")
 
     def solve_device(self, b_ptr, nrhs, precision=0):
         _check(load().hmx_solve_device(self.handle, precision, b_ptr, nrhs), "hmx_solve_device")

@@ -61,6 +61,17 @@ HMAT_API const char *hmx_last_error(void);
 HMAT_API int hmx_gemv_device(hmat_matrix_t *m, int precision, double alpha, const void *x,
                              double beta, void *y, int nrhs);
 
+/* No counterpart in hmat-oss.  The same product as hmx_gemv_device for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): every temporary lives
+ * in a fixed 64 MiB device array of the library, one per GPU, held until the
+ * process ends; the call allocates nothing.  A matrix/nrhs whose temporaries
+ * exceed 64 MiB is rejected with an error (hmx_last_error() names the bytes
+ * needed).  Calls that overlap on one GPU (two streams, two threads) race on
+ * the shared array; serialise them.  The matrix must be assembled. */
+HMAT_API int hmx_gemv_device_noalloc(hmat_matrix_t *m, int precision, double alpha,
+                                     const void *x, double beta, void *y, int nrhs);
+
 /* Solves with the factor in place on device data (hmat numbering). */
 HMAT_API int hmx_solve_device(hmat_matrix_t *m, int precision, void *b, int nrhs);
 
