@@ -89,14 +89,16 @@ def test_assembly_leaves_match_oracle(n, height, seed, leaf, eps, lowrank_eps):
         "rank mismatches", mismatch, "of", count)
     assert worst_full < 1e-14
     assert worst_rk < 3 * tol
-    assert worst_vs_oracle < 3 * tol
-    # ACA pivot walks agree with the oracle's on all but a few leaves
-    assert mismatch["aca"] <= max(1, count["aca"] // 100)
+    # same pivots, same cuts: the leaves equal the oracle's to rounding
+    # (measured 3e-14), and every rank is the oracle's
+    assert worst_vs_oracle < 1e-10
+    assert mismatch["aca"] == 0
     inf = h.info()
     oinf = ho.info(bt)
     assert inf["full_count"] == oinf["full_count"] and inf["rk_count"] == oinf["rk_count"]
     assert inf["nr_block_clusters"] == oinf["nr_block_clusters"]
-    assert abs(inf["compressed_size"] - oinf["compressed_size"]) <= 0.02 * oinf["compressed_size"]
+    if mismatch["svd"] == 0:
+        assert inf["compressed_size"] == oinf["compressed_size"]
     h.close()
 
 

@@ -35,18 +35,23 @@ def test_llt_fp64_matches_oracle(n, height, seed, leaf):
     say("stats", st)
     L = dense_lower_factor(h)
     rec = np.linalg.norm(L @ L.T - H) / np.linalg.norm(H)
-    say("||LL^T - H|| / ||H||", rec)
-    assert rec < 20 * EPS
     # oracle on the same inputs
     root, idx = ho.build_cluster_tree(pts, leaf)
     bt = ho.build_block_tree(root, root, 2.0, True)
     k = ho.ExpKernel(pts, 0.5, 1e-2)
     ho.assemble(bt, idx, k)
+    Ho = ho.to_dense(bt, n, n)
     ho.llt(bt, EPS)
+    Lo = ho.factor_dense(bt, n)
+    rec_o = np.linalg.norm(Lo @ Lo.T - Ho) / np.linalg.norm(Ho)
+    say("||LL^T - H|| / ||H|| gpu", rec, "oracle", rec_o)
+    # the same recompressions at the same eps: the factorisation error is the
+    # oracle's to a few digits (and about eps)
+    assert abs(rec - rec_o) <= 1e-2 * rec_o and rec < 10 * EPS
     ld_o = ho.logdet(bt)
     ld_g = h.logdet()
-    say("logdet gpu", ld_g, "oracle", ld_o)
-    assert abs(ld_g - ld_o) <= 1e-5 * abs(ld_o)
+    say("logdet gpu", ld_g, "oracle", ld_o, "rel", abs(ld_g - ld_o) / abs(ld_o))
+    assert abs(ld_g - ld_o) <= 1e-8 * abs(ld_o)       # measured 1.5e-9 / 2.2e-9
     sign, ld_dense = np.linalg.slogdet(H)
     assert abs(ld_g - 0.5 * ld_dense) <= 1e-4 * abs(ld_dense)
     # solve: user numbering through solve_systems
@@ -62,12 +67,17 @@ def test_llt_fp64_matches_oracle(n, height, seed, leaf):
     res_o = np.linalg.norm(A @ xou - b) / np.linalg.norm(b)
     rel = np.linalg.norm(x - xou) / np.linalg.norm(xou)
     say("residual gpu", res, "oracle", res_o, "solution rel diff", rel)
-    assert res < 5 * max(res_o, 1e-6)
-    assert rel < 50 * EPS * max(1.0, np.linalg.norm(xou) / np.linalg.norm(b))
-    # solve against the H-matrix itself: H x = b holds to the factor accuracy
+    assert abs(res - res_o) <= 1e-3 * res_o                # agree to 5 digits measured
+    assert rel <= 1e-5                                     # measured 2.8e-8 / 1.3e-6
+    # solve against the H-matrix itself: H x = b holds to the factor accuracy,
+    # the same as the oracle's factor against its own H
     xh = h.solve(b[idx], permute=False)
     res_h = np.linalg.norm(H @ xh - b[idx]) / np.linalg.norm(b)
-    say("residual vs H", res_h)
+    xoh = b[idx].copy()
+    ho.solve_llt(bt, xoh)
+    res_oh = np.linalg.norm(Ho @ xoh - b[idx]) / np.linalg.norm(b)
+    say("residual vs H gpu", res_h, "oracle", res_oh)
+    assert abs(res_h - res_oh) <= 1e-2 * res_oh
     h.close()
 
 
@@ -84,14 +94,27 @@ def test_llt_fp32_type():
     L = dense_lower_factor(h, precision=1)
     rec = np.linalg.norm(L @ L.T - H) / np.linalg.norm(H)
     say("fp32 ||LL^T - H|| / ||H||", rec)
-    assert rec < 50 * EPS
+    assert rec < 10 * EPS                                   # measured 2.0e-5
     b = np.random.default_rng(0).standard_normal((n, 1)).astype(np.float32)
     x = h.solve(b)
     k = ho.ExpKernel(pts, 0.5, 1e-2)
     A = k.block(np.arange(n), np.arange(n))
     res = np.linalg.norm(A @ x.astype(np.float64) - b) / np.linalg.norm(b)
-    say("fp32 residual", res)
-    assert res < 1e-2
+    # the oracle's FP64 H-LLt solve on the same inputs: FP32 storage changes
+    # the solution by far less than the compression does
+    root, idx = ho.build_cluster_tree(pts, 128)
+    bt = ho.build_block_tree(root, root, 2.0, True)
+    ho.assemble(bt, idx, k)
+    ho.llt(bt, EPS)
+    xo = b[idx].astype(np.float64)
+    ho.solve_llt(bt, xo)
+    xou = np.empty_like(xo)
+    xou[idx] = xo
+    res_o = np.linalg.norm(A @ xou - b) / np.linalg.norm(b)
+    rel = np.linalg.norm(x - xou) / np.linalg.norm(xou)
+    say("fp32 residual", res, "oracle fp64", res_o, "solution rel diff", rel)
+    assert res <= 1.1 * res_o
+    assert rel <= 1e-4
     h.close()
 
 
@@ -210,12 +233,12 @@ def test_lu_fp64_matches_oracle(n, height, seed, leaf, nugget, shift):
     res_o = np.linalg.norm(A @ xou - b) / np.linalg.norm(b)
     rel = np.linalg.norm(x - xou) / np.linalg.norm(xou)
     say("residual gpu", res, "oracle", res_o, "solution rel diff", rel)
-    assert res < 5 * max(res_o, 1e-6)
-    assert rel < 100 * EPS
+    assert abs(res - res_o) <= 1e-2 * res_o               # measured 2e-5 relative
+    assert rel <= 5e-5                                     # measured 3.2e-6 / 8.0e-6
     xh = h.solve(b[idx], permute=False)
     res_h = np.linalg.norm(H @ xh - b[idx]) / np.linalg.norm(b)
     say("residual vs H", res_h)
-    assert res_h < 5 * max(res_o, 1e-3)
+    assert res_h <= res_o                                  # H is closer to itself than A
     h.close()
 
 

@@ -58,10 +58,10 @@ def test_llt_uneven_matches_oracle(n, height, seed, leaf):
     say(f"rec gpu {rec:.3e} oracle {rec_o:.3e}; logdet gpu {ld_g:.12g} oracle {ld_o:.12g} "
         f"rel {abs(ld_g - ld_o) / abs(ld_o):.3e}; residual gpu {res:.3e} oracle {res_o:.3e}; "
         f"solution rel diff {rel:.3e}")
-    assert rec <= 3e-5
-    assert abs(ld_g - ld_o) <= 1e-6 * abs(ld_o)
-    assert rel <= 1e-4
-    assert res <= 1.5 * res_o + 1e-12
+    assert abs(rec - rec_o) <= 1e-2 * rec_o                 # equal to 4 digits measured
+    assert abs(ld_g - ld_o) <= 1e-8 * abs(ld_o)             # measured 1.0e-9 / 7.2e-10
+    assert rel <= 1e-6                                      # measured 2.3e-8 / 1.0e-8
+    assert abs(res - res_o) <= 1e-3 * res_o
     h.close()
 
 
@@ -93,6 +93,6 @@ def test_lu_uneven_matches_oracle(n, height, seed, leaf):
     res_o = np.linalg.norm(A @ xou - b) / np.linalg.norm(b)
     rel = np.linalg.norm(x - xou) / np.linalg.norm(xou)
     say(f"residual gpu {res:.3e} oracle {res_o:.3e}; solution rel diff {rel:.3e}")
-    assert rel <= 1e-5
-    assert res <= 1.5 * res_o + 1e-12
+    assert rel <= 1e-5                                      # measured 1.5e-6 / 1.5e-7
+    assert res <= 1.2 * res_o                               # measured 0.94 / 0.88 of res_o
     h.close()
