@@ -1707,6 +1707,13 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st) {
   if (!fstate_) fstate_ = std::make_shared<FactorState>();
   FactorState& fs = *fstate_;
   const int n = rows->n;
+  // The persistent-sweep solve (sweep.cpp, k_sweep.hip) did not complete on the
+  // device and its cause is not found: it is switched off, and asking for it
+  // (HMX_SOLVE_SWEEP=1) is an error.  The recorded task plans below solve.
+  const char* sweep_env = getenv("HMX_SOLVE_SWEEP");
+  if (sweep_env && atoi(sweep_env))
+    throw std::runtime_error("solve: the persistent-sweep solve is disabled (it did not complete "
+                             "on the device; cause not found)");
   const auto key = std::make_pair((int)sizeof(T), nrhs);
   const size_t bytes = (size_t)n * nrhs * sizeof(T);
   auto it = fs.solve_plans.find(key);
