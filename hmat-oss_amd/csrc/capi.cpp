@@ -96,7 +96,7 @@ int assemble_generic(hmat_matrix_t* h, hmat_assemble_context_t* ctx) {
     spec.length = k->length;
     spec.nugget = k->nugget;
     spec.shift = k->kind == HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL ? k->shift : 0.0;
-    if (k->kind < 0 || k->kind > HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL)
+    if (k->kind < 0 || k->kind > HMAT_MI355X_KERNEL_GAUSSIAN_DLENGTH)
       throw std::runtime_error("hmat_mi355x_kernel_t: unknown kernel kind");
     spec.dim = k->dimension;
     spec.n = k->n;
@@ -134,11 +134,73 @@ int solve_dense(hmat_matrix_t* h, void* b, int nrhs) {
   return guard([&] { M(h)->solve_host(b, nrhs, false); });
 }
 
+// solve_generic c_wrapping.hpp:500-540: lower / upper select one triangular
+// sweep of an LLt factor (solveLower(b, transpose))
 int solve_generic(hmat_matrix_t* h, const struct hmat_solve_context_t* ctx) {
   return guard([&] {
     if (ctx->nr_rhs == 0) throw std::runtime_error("solve_generic: H-matrix right-hand sides are not supported");
-    if (ctx->lower || ctx->upper) throw std::runtime_error("solve_generic: lower/upper solves are not supported");
-    M(h)->solve_host(ctx->values, ctx->nr_rhs, ctx->no_permutation == 0);
+    if (ctx->lower && ctx->upper)
+      throw std::runtime_error("lower and upper cannot both be true in hmat_solve_context_t");
+    M(h)->solve_host(ctx->values, ctx->nr_rhs, ctx->no_permutation == 0,
+                     ctx->lower ? 1 : (ctx->upper ? 2 : 0));
+  });
+}
+
+// solve_lower_triangular(_dense) c_wrapping.hpp:793-820
+int solve_lower_triangular(hmat_matrix_t* h, int transpose, void* b, int nrhs) {
+  return guard([&] { M(h)->solve_host(b, nrhs, true, transpose ? 2 : 1); });
+}
+int solve_lower_triangular_dense(hmat_matrix_t* h, int transpose, void* b, int nrhs) {
+  return guard([&] { M(h)->solve_host(b, nrhs, false, transpose ? 2 : 1); });
+}
+
+// copy / copy_struct c_wrapping.hpp:98-108 (the copy shares the cluster trees)
+hmat_matrix_t* copy_matrix(hmat_matrix_t* h) {
+  hmat_matrix_t* r = nullptr;
+  guard([&] { r = reinterpret_cast<hmat_matrix_t*>(M(h)->clone(false)); });
+  return r;
+}
+hmat_matrix_t* copy_struct(hmat_matrix_t* h) {
+  hmat_matrix_t* r = nullptr;
+  guard([&] { r = reinterpret_cast<hmat_matrix_t*>(M(h)->clone(true)); });
+  return r;
+}
+
+// norm c_wrapping.hpp:412 (HMatrix::norm, Frobenius)
+double norm_matrix(hmat_matrix_t* h) {
+  double r = NAN;
+  guard([&] { r = M(h)->norm(); });
+  return r;
+}
+
+template <typename T>
+int scale_matrix(void* alpha, hmat_matrix_t* h) {
+  return guard([&] { M(h)->scale((double)*(const T*)alpha); });
+}
+template <typename T>
+int add_identity(hmat_matrix_t* h, void* alpha) {
+  return guard([&] { M(h)->add_identity((double)*(const T*)alpha); });
+}
+int truncate_matrix(hmat_matrix_t* h) {
+  return guard([&] { M(h)->truncate_all(); });
+}
+// axpy c_wrapping.hpp:218-230: y <- y + a x (same partition)
+template <typename T>
+int axpy_matrix(void* a, hmat_matrix_t* x, hmat_matrix_t* y) {
+  return guard([&] { M(y)->axpy_from((double)*(const T*)a, *M(x)); });
+}
+
+// extract_diagonal c_wrapping.hpp:760-773: the diagonal, in user numbering
+template <typename T>
+int extract_diagonal(hmat_matrix_t* h, void* diag, int size) {
+  return guard([&] {
+    Matrix* m = M(h);
+    const int n = m->n_rows();
+    if (size < n) throw std::runtime_error("extract_diagonal: the output holds fewer than N values");
+    std::vector<double> hd(n);
+    m->extract_diagonal(hd.data());
+    T* out = static_cast<T*>(diag);
+    for (int i = 0; i < n; ++i) out[m->cols->indices[i]] = (T)hd[i];
   });
 }
 
@@ -227,18 +289,13 @@ int unsupported(const char* what) {
   fprintf(stderr, "%s\n", g_last_error.c_str());
   return 1;
 }
-hmat_matrix_t* u_copy(hmat_matrix_t*) { unsupported("copy"); return nullptr; }
-double u_norm(hmat_matrix_t*) { unsupported("norm"); return NAN; }
 int u_inverse(hmat_matrix_t*) { return unsupported("inverse"); }
 hmat_matrix_t* u_get_child(hmat_matrix_t*, int, int) { unsupported("get_child"); return nullptr; }
 int u_h(hmat_matrix_t*) { return unsupported("operation"); }
 int u_solve_mat(hmat_matrix_t*, hmat_matrix_t*) { return unsupported("solve_mat"); }
-int u_scale(void*, hmat_matrix_t*) { return unsupported("scale"); }
 int u_gemm(char, char, const void*, hmat_matrix_t*, hmat_matrix_t*, const void*, hmat_matrix_t*) { return unsupported("gemm"); }
-int u_axpy(void*, hmat_matrix_t*, hmat_matrix_t*) { return unsupported("axpy"); }
 int u_trsm(char, char, char, char, int, int, void*, hmat_matrix_t*, int, void*) { return unsupported("trsm"); }
 int u_full_gemm(char, char, int, int, void*, void*, void*, hmat_matrix_t*, void*) { return unsupported("full_gemm"); }
-int u_add_identity(hmat_matrix_t*, void*) { return unsupported("add_identity"); }
 int u_profile(hmat_matrix_t*, hmat_profile_t*) { return unsupported("get_profile"); }
 int u_ratio(hmat_matrix_t*, hmat_FPCompressionRatio_t*) { return unsupported("get_ratio"); }
 hmat_fp_settings_t u_get_fp(hmat_matrix_t*) { unsupported("GetFPCompressionSettings"); return hmat_fp_settings_t{}; }
@@ -246,8 +303,6 @@ int u_set_fp(hmat_matrix_t*, hmat_fp_settings_t) { return unsupported("SetFPComp
 int u_set_fp2(hmat_matrix_t*, float, int, hmat_FPcompress_t, bool, bool) { return unsupported("SetFPCompressionSettingsParams"); }
 int u_dump(hmat_matrix_t*, char*) { return unsupported("dump_info"); }
 int u_set_ct(hmat_matrix_t*, const hmat_cluster_tree_t*, const hmat_cluster_tree_t*) { return unsupported("set_cluster_trees"); }
-int u_diag(hmat_matrix_t*, void*, int) { return unsupported("extract_diagonal"); }
-int u_slt(hmat_matrix_t*, int, void*, int) { return unsupported("solve_lower_triangular"); }
 int u_values(struct hmat_get_values_context_t*) { return unsupported("get_block/get_values"); }
 int u_walk(hmat_matrix_t*, hmat_procedure_t*) { return unsupported("walk"); }
 int u_leaf(hmat_matrix_t*, const hmat_leaf_procedure_t*) { return unsupported("apply_on_leaf"); }
@@ -263,9 +318,9 @@ void fill(hmat_interface_t* i) {
   i->own_cluster_trees = own_cluster_trees;
   i->set_low_rank_epsilon = set_low_rank_epsilon;
   i->assemble_generic = assemble_generic;
-  i->copy = u_copy;
-  i->copy_struct = u_copy;
-  i->norm = u_norm;
+  i->copy = copy_matrix;
+  i->copy_struct = copy_struct;
+  i->norm = norm_matrix;
   i->logdet = logdet;
   i->inverse = u_inverse;
   i->factorize_generic = factorize_generic;
@@ -277,18 +332,18 @@ void fill(hmat_interface_t* i) {
   i->solve_systems = solve_systems;
   i->solve_dense = solve_dense;
   i->transpose = u_h;
-  i->scale = u_scale;
-  i->truncate = u_h;
+  i->scale = scale_matrix<T>;
+  i->truncate = truncate_matrix;
   i->vector_reorder = vector_order<T, false>;
   i->vector_restore = vector_order<T, true>;
   i->gemm = u_gemm;
-  i->axpy = u_axpy;
+  i->axpy = axpy_matrix<T>;
   i->trsm = u_trsm;
   i->gemv = gemv;
   i->gemm_scalar = gemm_scalar;
   i->full_gemm = u_full_gemm;
   i->gemm_dense = gemm_dense;
-  i->add_identity = u_add_identity;
+  i->add_identity = add_identity<T>;
   i->init = init_engine;
   i->finalize = finalize_engine;
   i->get_info = get_info;
@@ -302,9 +357,9 @@ void fill(hmat_interface_t* i) {
   i->dump_info = u_dump;
   i->get_cluster_trees = get_cluster_trees;
   i->set_cluster_trees = u_set_ct;
-  i->extract_diagonal = u_diag;
-  i->solve_lower_triangular = u_slt;
-  i->solve_lower_triangular_dense = u_slt;
+  i->extract_diagonal = extract_diagonal<T>;
+  i->solve_lower_triangular = solve_lower_triangular;
+  i->solve_lower_triangular_dense = solve_lower_triangular_dense;
   i->get_block = u_values;
   i->get_values = u_values;
   i->walk = u_walk;

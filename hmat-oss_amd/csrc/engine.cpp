@@ -46,7 +46,7 @@ Store<float>& Matrix::store<float>() {
 
 Matrix::Matrix(const ClusterTree* r, const ClusterTree* c, bool sym,
                const AdmissibilityParams& adm, int vt, int rank_cap)
-    : value_type(vt), rows(r), cols(c), lower_sym(sym), rk_cap_max(rank_cap) {
+    : adm(adm), value_type(vt), rows(r), cols(c), lower_sym(sym), rk_cap_max(rank_cap) {
   if (sym && r != c) throw std::runtime_error("lower-symmetric storage needs rows tree == cols tree");
   bt = build_block_tree(*r, *c, adm, sym);
   build_storage();

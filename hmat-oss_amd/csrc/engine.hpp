@@ -82,6 +82,7 @@ class Matrix {
  public:
   Matrix(const ClusterTree* rows, const ClusterTree* cols, bool lower_sym,
          const AdmissibilityParams& adm, int value_type, int rank_cap);
+  AdmissibilityParams adm;
   ~Matrix();
 
   int value_type;        // hmat_value_t
@@ -126,14 +127,25 @@ class Matrix {
 
   // factorisation / solve (factor.cpp)
   void factorize(int algo, bool fp32);
+  // part 0: the whole solve; 1: the lower-triangular sweep only (L x = b);
+  // 2: the upper one only (L^T x = b) -- LLt factors
   template <typename T>
-  void solve_device(T* b, int nrhs, int ldb, const Store<T>& st);
-  void solve_host(void* b, int nrhs, bool permute);
+  void solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part = 0);
+  void solve_host(void* b, int nrhs, bool permute, int part = 0);
   double logdet();
   // Iterative refinement (FP32 factor, FP64 residual with the FP64 matrix).
   int solve_refined(double* b_dev, int nrhs, int ldb, double tol, int max_iter, double* resid_out);
 
   void info(hmat_info_t& out) const;
+
+  // whole-matrix operations (ops.cpp; truncate_all / axpy_from in factor.cpp)
+  Matrix* clone(bool structure_only);       // same trees and partition; null leaves if asked
+  double norm();                            // Frobenius norm of the whole matrix
+  void scale(double alpha);                 // A <- alpha A
+  void add_identity(double alpha);          // A <- A + alpha I
+  void extract_diagonal(double* diag);      // diagonal in hmat numbering (n_rows doubles)
+  void truncate_all();                      // recompress every Rk leaf at lowrank_eps
+  void axpy_from(double alpha, Matrix& x);  // A <- A + alpha X, X of the same partition
 
   int n_rows() const { return rows->n; }
   int n_cols() const { return cols->n; }

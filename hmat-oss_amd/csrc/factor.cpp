@@ -1494,6 +1494,62 @@ class Exec {
     }
   }
 
+  // ---- whole-matrix updates -----------------------------------------------------------
+  // RkMatrix::truncate (rk_matrix.cpp:490-585) of every Rk leaf at eps: a
+  // formatted-add task whose target is recompressed alone (mode 1)
+  void truncate_leaves() {
+    for (const Block* b : M_.bt->leaves) {
+      if (b->kind != Kind::Rk) continue;
+      RV tg = rk_of(b);
+      FTask t{};
+      t.type = T_FADD;
+      t.mode = 1;
+      t.eps = eps_;
+      t.tgt = ref(tg);
+      t.nparts = 0;
+      rd(tg);
+      wr(tg);
+      emit(t, fadd_workspace(tg.m, tg.n, tg.cap));
+      maybe_flush();
+    }
+  }
+
+  // HMatrix::axpy(alpha, X) (h_matrix.cpp:930-1073) for X of the same
+  // partition, stored in xs: Rk leaves by formatted addition + truncation,
+  // full leaves by a dense axpy.  X's leaves are read-only dependency units.
+  void axpy_matrix(double alpha, Store<T>& xs) {
+    constexpr int64_t kXBase = int64_t(1) << 36;
+    const int* xr = xs.ranks.template as<int>();
+    for (size_t l = 0; l < M_.bt->leaves.size(); ++l) {
+      const Block* b = M_.bt->leaves[l];
+      const LeafDesc& L = xs.hdesc[l];
+      if (L.kind == 0) {
+        DV f;
+        f.p = xs.data() + L.a;
+        f.ld = L.m;
+        f.m = L.m;
+        f.n = L.n;
+        f.buf = kXBase + (int64_t)l;
+        axpy_full(b, alpha, f, b->rows->offset, b->cols->offset);
+      } else {
+        RV r;
+        r.a = xs.data() + L.a;
+        r.b = xs.data() + L.b;
+        r.la = L.m;
+        r.lb = L.n;
+        r.m = L.m;
+        r.n = L.n;
+        r.cap = L.cap;
+        r.rank = const_cast<int*>(xr) + l;
+        r.bufa = r.bufb = kXBase + (int64_t)l;
+        r.row0 = b->rows->offset;
+        r.col0 = b->cols->offset;
+        axpy_rk(b, alpha, r);
+      }
+      maybe_flush();
+    }
+  }
+
   // ---- execution -----------------------------------------------------------------------
   void maybe_flush() {
     if (live_ != 0) return;
@@ -1698,7 +1754,7 @@ void Matrix::factorize(int algo, bool fp32) {
 }
 
 template <typename T>
-void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st) {
+void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part) {
   if (factorization == -1) throw std::runtime_error("solve: matrix not factorised");
   if ((factor32 ? 1 : 0) != (sizeof(T) == 4 ? 1 : 0))
     throw std::runtime_error("solve: the factor is stored in the other precision");
@@ -1714,7 +1770,9 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st) {
   if (sweep_env && atoi(sweep_env))
     throw std::runtime_error("solve: the persistent-sweep solve is disabled (it did not complete "
                              "on the device; cause not found)");
-  const auto key = std::make_pair((int)sizeof(T), nrhs);
+  if (part != 0 && factorization != hmat_factorization_llt)
+    throw std::runtime_error("solve_lower_triangular: provided for LLt factors only");
+  const auto key = std::make_pair((int)sizeof(T) + 16 * part, nrhs);
   const size_t bytes = (size_t)n * nrhs * sizeof(T);
   auto it = fs.solve_plans.find(key);
   if (it == fs.solve_plans.end()) fs.solve_rhs.erase(key);
@@ -1739,9 +1797,10 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st) {
       ex.lu_solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
       ex.lu_solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
     } else {
-      // HMatrix::solveLlt h_matrix.cpp:3010-3018
-      ex.solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
-      ex.solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
+      // HMatrix::solveLlt h_matrix.cpp:3010-3018 (part 1: L x = b only, part 2:
+      // L^T x = b only, as solveLower(b, transpose) of the solve_lower_triangular slot)
+      if (part != 2) ex.solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
+      if (part != 1) ex.solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
     }
     ex.flush(false);
     it = fs.solve_plans.emplace(key, std::move(plans)).first;
@@ -1780,10 +1839,10 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st) {
   if (!user_stream) HMX_CHECK(hipStreamSynchronize(stream));
   stats.t_solve = now_s() - t0;
 }
-template void Matrix::solve_device<double>(double*, int, int, const Store<double>&);
-template void Matrix::solve_device<float>(float*, int, int, const Store<float>&);
+template void Matrix::solve_device<double>(double*, int, int, const Store<double>&, int);
+template void Matrix::solve_device<float>(float*, int, int, const Store<float>&, int);
 
-void Matrix::solve_host(void* b, int nrhs, bool permute) {
+void Matrix::solve_host(void* b, int nrhs, bool permute, int part) {
   const int n = rows->n;
   auto run = [&](auto* tag) {
     using T = std::remove_pointer_t<decltype(tag)>;
@@ -1797,7 +1856,7 @@ void Matrix::solve_host(void* b, int nrhs, bool permute) {
     }
     DevBuf d;
     upload(d, h.data(), h.size() * sizeof(T), stream);
-    solve_device<T>(d.as<T>(), nrhs, n, store<T>());
+    solve_device<T>(d.as<T>(), nrhs, n, store<T>(), part);
     HMX_CHECK(hipMemcpyAsync(h.data(), d.p, h.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
     HMX_CHECK(hipStreamSynchronize(stream));
     if (permute) {
@@ -1815,6 +1874,55 @@ void Matrix::solve_host(void* b, int nrhs, bool permute) {
     run((float*)nullptr);
   } else {
     throw std::runtime_error("solve: complex types are not supported by this engine");
+  }
+}
+
+void Matrix::truncate_all() {
+  if (!assembled) throw std::runtime_error("truncate: matrix not assembled");
+  if (factorization != -1) throw std::runtime_error("truncate: matrix has been factorised");
+  device_init();
+  if (!fstate_) fstate_ = std::make_shared<FactorState>();
+  if (s64.valid) {
+    Exec<double> ex(*this, s64, *fstate_);
+    ex.truncate_leaves();
+    ex.flush(false);
+    ex.check_flags();
+  } else if (s32.valid) {
+    Exec<float> ex(*this, s32, *fstate_);
+    ex.truncate_leaves();
+    ex.flush(false);
+    ex.check_flags();
+  }
+}
+
+void Matrix::axpy_from(double alpha, Matrix& x) {
+  if (!assembled || !x.assembled) throw std::runtime_error("axpy: matrix not assembled");
+  if (factorization != -1 || x.factorization != -1)
+    throw std::runtime_error("axpy: factorised matrices are not supported");
+  if (x.rows != rows || x.cols != cols || x.lower_sym != lower_sym || x.value_type != value_type ||
+      x.s64.hdesc.size() != s64.hdesc.size())
+    throw std::runtime_error("axpy: the matrices must share cluster trees, storage and partition");
+  for (size_t l = 0; l < s64.hdesc.size(); ++l) {
+    const LeafDesc &a = s64.hdesc[l], &b = x.s64.hdesc[l];
+    if (a.m != b.m || a.n != b.n || a.r0 != b.r0 || a.c0 != b.c0 || a.kind != b.kind || a.cap != b.cap)
+      throw std::runtime_error("axpy: the matrices must share cluster trees, storage and partition");
+  }
+  device_init();
+  x.device_init();
+  HMX_CHECK(hipStreamSynchronize(x.stream));
+  if (!fstate_) fstate_ = std::make_shared<FactorState>();
+  if (s64.valid && x.s64.valid) {
+    Exec<double> ex(*this, s64, *fstate_);
+    ex.axpy_matrix(alpha, x.s64);
+    ex.flush(false);
+    ex.check_flags();
+  } else if (s32.valid && x.s32.valid) {
+    Exec<float> ex(*this, s32, *fstate_);
+    ex.axpy_matrix(alpha, x.s32);
+    ex.flush(false);
+    ex.check_flags();
+  } else {
+    throw std::runtime_error("axpy: no data in a common precision");
   }
 }
 

@@ -15,7 +15,8 @@ enum TaskType : int {
   T_GEMM = 0,     // C = beta C + alpha op(A) op(B)
   T_POTRF,        // A <- chol(A) lower, upper triangle zeroed
   T_TRSM,         // mode: 0 X <- X L^{-T}, 1 X <- L^{-1} X, 2 X <- L^{-T} X
-  T_FADD,         // Rk target (+)= sum_i alpha_i part_i, then truncate
+  T_FADD,         // Rk target (+)= sum_i alpha_i part_i, then truncate (mode 1: truncate
+                  // the target alone, no parts)
   T_FADD_DENSE,   // Rk target += alpha * dense part, eval + SVD (mode 1: target := alpha * part)
   T_RKRK,         // Rk product core: (A1 B1^T)(A2 B2^T)
   T_ACAFULL,      // dense -> Rk by full-pivot ACA

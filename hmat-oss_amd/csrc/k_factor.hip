@@ -942,7 +942,8 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
       last = q;
     }
   }
-  if (np == 0) return;
+  // mode 1 recompresses the target alone (RkMatrix::truncate)
+  if (np == 0 && !(t.mode == 1 && kt > 0)) return;
   T* TA = (T*)tg.a;
   T* TB = (T*)tg.b;
   if (kt == 0 && np == 1) {

@@ -1,0 +1,122 @@
+// Whole-matrix leaf operations behind the remaining hmat_interface_t slots:
+// squared Frobenius norms of leaves (HMatrix::normSqr h_matrix.cpp:624-646,
+// RkMatrix::normSqr, FullMatrix::normSqr), scaling (HMatrix::scale), adding a
+// multiple of the identity (HMatrix::addIdentity) and reading the diagonal
+// (HMatrix::extractDiagonal).  One workgroup per leaf of a list.
+#include "kernels.hpp"
+
+namespace hmx {
+
+namespace {
+
+// out[i] = ||leaf||_F^2 for the i-th listed leaf: full, sum of squares;
+// Rk, sum over p, q of (A_p . A_q)(B_p . B_q) (the trace of (A^T A)(B^T B)).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_leaf_norm2(const LeafDesc* desc, const int* ranks,
+                                                       const T* arena, const int* list,
+                                                       double* out) {
+  __shared__ double red[kWaves];
+  const int l = list[blockIdx.x];
+  const LeafDesc L = desc[l];
+  if (L.kind == 0) {
+    const T* a = arena + L.a;
+    const long cnt = (long)L.m * L.n;
+    double s = 0.0;
+    for (long e = threadIdx.x; e < cnt; e += kBlock) {
+      const double v = (double)a[e];
+      s += v * v;
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+    return;
+  }
+  const int k = ranks[l];
+  const T* A = arena + L.a;
+  const T* B = arena + L.b;
+  double tot = 0.0;
+  for (int p = 0; p < k; ++p)
+    for (int q = p; q < k; ++q) {
+      double sa = 0.0, sb = 0.0;
+      for (int i = threadIdx.x; i < L.m; i += kBlock)
+        sa += (double)A[i + (long)p * L.m] * (double)A[i + (long)q * L.m];
+      for (int i = threadIdx.x; i < L.n; i += kBlock)
+        sb += (double)B[i + (long)p * L.n] * (double)B[i + (long)q * L.n];
+      sa = block_sum(sa, red);
+      sb = block_sum(sb, red);
+      tot += (p == q ? 1.0 : 2.0) * sa * sb;
+    }
+  if (threadIdx.x == 0) out[blockIdx.x] = tot;
+}
+
+// Full leaf: every entry times alpha; Rk leaf: its A panel times alpha.
+template <typename T>
+__global__ void k_leaf_scale(const LeafDesc* desc, const int* list, T* arena, double alpha) {
+  const LeafDesc L = desc[list[blockIdx.x]];
+  T* a = arena + L.a;
+  const long cnt = (long)L.m * (L.kind == 0 ? L.n : L.cap);
+  for (long e = threadIdx.x; e < cnt; e += blockDim.x) a[e] = (T)(alpha * (double)a[e]);
+}
+
+// Diagonal full leaves: a_ii += alpha.
+template <typename T>
+__global__ void k_leaf_add_diag(const LeafDesc* desc, const int* list, T* arena, double alpha) {
+  const LeafDesc L = desc[list[blockIdx.x]];
+  T* a = arena + L.a;
+  for (int i = threadIdx.x; i < L.m; i += blockDim.x) a[i + (long)i * L.m] += (T)alpha;
+}
+
+// Diagonal full leaves: out[r0 + i] = a_ii (hmat numbering).
+template <typename T>
+__global__ void k_leaf_get_diag(const LeafDesc* desc, const int* list, const T* arena,
+                                double* out) {
+  const LeafDesc L = desc[list[blockIdx.x]];
+  const T* a = arena + L.a;
+  for (int i = threadIdx.x; i < L.m; i += blockDim.x) out[L.r0 + i] = (double)a[i + (long)i * L.m];
+}
+
+}  // namespace
+
+template <typename T>
+void launch_leaf_norm2(const LeafDesc* desc, const int* ranks, const T* arena, const int* list,
+                       int count, double* out, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_leaf_norm2<T>, dim3(count), dim3(kBlock), 0, s, desc, ranks, arena, list, out);
+  HMX_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_leaf_scale(const LeafDesc* desc, const int* list, int count, T* arena, double alpha,
+                       hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_leaf_scale<T>, dim3(count), dim3(256), 0, s, desc, list, arena, alpha);
+  HMX_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_leaf_add_diag(const LeafDesc* desc, const int* list, int count, T* arena,
+                          double alpha, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_leaf_add_diag<T>, dim3(count), dim3(256), 0, s, desc, list, arena, alpha);
+  HMX_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_leaf_get_diag(const LeafDesc* desc, const int* list, int count, const T* arena,
+                          double* out, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_leaf_get_diag<T>, dim3(count), dim3(256), 0, s, desc, list, arena, out);
+  HMX_CHECK(hipGetLastError());
+}
+
+#define HMX_OPS_INST(T)                                                                          \
+  template void launch_leaf_norm2<T>(const LeafDesc*, const int*, const T*, const int*, int,     \
+                                     double*, hipStream_t);                                      \
+  template void launch_leaf_scale<T>(const LeafDesc*, const int*, int, T*, double, hipStream_t); \
+  template void launch_leaf_add_diag<T>(const LeafDesc*, const int*, int, T*, double,            \
+                                        hipStream_t);                                            \
+  template void launch_leaf_get_diag<T>(const LeafDesc*, const int*, int, const T*, double*,     \
+                                        hipStream_t);
+HMX_OPS_INST(double)
+HMX_OPS_INST(float)
+
+}  // namespace hmx

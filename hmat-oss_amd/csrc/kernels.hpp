@@ -89,6 +89,20 @@ struct YContrib {
                    // right-hand side: the data sits at nrhs * z), else -1
 };
 
+// ---- whole-matrix leaf operations (k_ops.hip); one workgroup per listed leaf
+template <typename T>
+void launch_leaf_norm2(const LeafDesc* desc, const int* ranks, const T* arena, const int* list,
+                       int count, double* out, hipStream_t s);
+template <typename T>
+void launch_leaf_scale(const LeafDesc* desc, const int* list, int count, T* arena, double alpha,
+                       hipStream_t s);
+template <typename T>
+void launch_leaf_add_diag(const LeafDesc* desc, const int* list, int count, T* arena,
+                          double alpha, hipStream_t s);
+template <typename T>
+void launch_leaf_get_diag(const LeafDesc* desc, const int* list, int count, const T* arena,
+                          double* out, hipStream_t s);
+
 // Fixed device scratch of the allocation-free H-gemv (64 MiB per GPU); returns
 // its device address on the current device (looked up once per device).
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;

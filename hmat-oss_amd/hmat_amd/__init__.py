@@ -22,6 +22,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libhmat_mi355x.so")
 HMAT_SIMPLE_PRECISION, HMAT_DOUBLE_PRECISION = 0, 1
 FACT_NONE, FACT_LU, FACT_LDLT, FACT_LLT = -1, 0, 1, 2
 KERNEL_EXPONENTIAL, KERNEL_GAUSSIAN, KERNEL_SHIFTED_EXPONENTIAL = 0, 1, 2
+# derivatives with respect to the correlation length (for the GP likelihood gradient)
+KERNEL_EXPONENTIAL_DLENGTH, KERNEL_GAUSSIAN_DLENGTH = 3, 4
 
 _vp = C.c_void_p
 
@@ -71,24 +73,30 @@ _INFO = C.CFUNCTYPE(C.c_int, _vp, C.POINTER(hmat_info_t))
 _VOID_EPS = C.CFUNCTYPE(None, _vp, C.c_double)
 _OWN = C.CFUNCTYPE(None, _vp, C.c_int, C.c_int)
 _INIT = C.CFUNCTYPE(C.c_int)
+_COPY = C.CFUNCTYPE(_vp, _vp)
+_NORM = C.CFUNCTYPE(C.c_double, _vp)
+_SCALAR_H = C.CFUNCTYPE(C.c_int, _vp, _vp)
+_AXPY = C.CFUNCTYPE(C.c_int, _vp, _vp, _vp)
+_DIAG = C.CFUNCTYPE(C.c_int, _vp, _vp, C.c_int)
+_SOLVE_TRI = C.CFUNCTYPE(C.c_int, _vp, C.c_int, _vp, C.c_int)
 
 # slot order of hmat_interface_t (hmat.h:717-1103); unused slots are plain pointers
 _SLOTS = [
     ("create_empty_hmatrix_admissibility", _CREATE), ("own_cluster_trees", _OWN),
-    ("set_low_rank_epsilon", _VOID_EPS), ("assemble_generic", _ASSEMBLE), ("copy", _vp),
-    ("copy_struct", _vp), ("norm", _vp), ("logdet", _LOGDET), ("inverse", _vp),
+    ("set_low_rank_epsilon", _VOID_EPS), ("assemble_generic", _ASSEMBLE), ("copy", _COPY),
+    ("copy_struct", _COPY), ("norm", _NORM), ("logdet", _LOGDET), ("inverse", _vp),
     ("factorize_generic", _FACTOR), ("destroy", _P_INT), ("get_child", _vp),
     ("destroy_child", _vp), ("solve_generic", _SOLVE_GEN), ("solve_mat", _vp),
-    ("solve_systems", _SOLVE), ("solve_dense", _SOLVE), ("transpose", _vp), ("scale", _vp),
-    ("truncate", _vp), ("vector_reorder", _REORDER), ("vector_restore", _REORDER),
-    ("gemm", _vp), ("axpy", _vp), ("trsm", _vp), ("gemv", _GEMV), ("gemm_scalar", _GEMV),
-    ("full_gemm", _vp), ("gemm_dense", _GEMM_DENSE), ("add_identity", _vp), ("init", _INIT),
+    ("solve_systems", _SOLVE), ("solve_dense", _SOLVE), ("transpose", _vp), ("scale", _SCALAR_H),
+    ("truncate", _P_INT), ("vector_reorder", _REORDER), ("vector_restore", _REORDER),
+    ("gemm", _vp), ("axpy", _AXPY), ("trsm", _vp), ("gemv", _GEMV), ("gemm_scalar", _GEMV),
+    ("full_gemm", _vp), ("gemm_dense", _GEMM_DENSE), ("add_identity", _SCALAR_H), ("init", _INIT),
     ("finalize", _INIT), ("get_info", _INFO), ("get_profile", _vp), ("get_ratio", _vp),
     ("FPcompress", _vp), ("FPdecompress", _vp), ("GetFPCompressionSettings", _vp),
     ("SetFPCompressionSettings", _vp), ("SetFPCompressionSettingsParams", _vp),
     ("dump_info", _vp), ("get_cluster_trees", _vp), ("set_cluster_trees", _vp),
-    ("extract_diagonal", _vp), ("solve_lower_triangular", _vp),
-    ("solve_lower_triangular_dense", _vp), ("get_block", _vp), ("get_values", _vp),
+    ("extract_diagonal", _DIAG), ("solve_lower_triangular", _SOLVE_TRI),
+    ("solve_lower_triangular_dense", _SOLVE_TRI), ("get_block", _vp), ("get_values", _vp),
     ("walk", _vp), ("apply_on_leaf", _vp), ("value_type", C.c_int), ("internal", _vp),
     ("read_struct", _vp), ("write_struct", _vp), ("read_data", _vp), ("write_data", _vp),
     ("set_progressbar", _vp), ("extract_diagonal_block", _vp),
@@ -326,6 +334,57 @@ class HMatrix:
         nrhs = 1 if x.ndim == 1 else x.shape[1]
         fn = self.iface.solve_systems if permute else self.iface.solve_dense
         _check(fn(self.handle, x.ctypes.data, nrhs), "solve")
+        return x
+
+    # ---- whole-matrix slots ----------------------------------------------------
+    def _scalar(self, v):
+        return (C.c_double if self.value_type else C.c_float)(v)
+
+    def copy(self, structure_only=False):
+        """hmat_interface_t.copy / copy_struct: a new matrix sharing this one's
+        cluster trees (a null matrix of the same structure when structure_only)."""
+        fn = self.iface.copy_struct if structure_only else self.iface.copy
+        h = fn(self.handle)
+        if not h:
+            raise EngineError(f"copy failed: {load().hmx_last_error().decode()}")
+        c = object.__new__(HMatrix)
+        c.tree, c.iface, c.value_type = self.tree, self.iface, self.value_type
+        c.handle, c.lower_symmetric, c._kernel = h, self.lower_symmetric, self._kernel
+        return c
+
+    def norm(self):
+        r = self.iface.norm(self.handle)
+        if r != r:
+            raise EngineError(f"norm failed: {load().hmx_last_error().decode()}")
+        return r
+
+    def scale(self, alpha):
+        a = self._scalar(alpha)
+        _check(self.iface.scale(C.cast(C.byref(a), _vp), self.handle), "scale")
+
+    def add_identity(self, alpha):
+        a = self._scalar(alpha)
+        _check(self.iface.add_identity(self.handle, C.cast(C.byref(a), _vp)), "add_identity")
+
+    def truncate(self):
+        _check(self.iface.truncate(self.handle), "truncate")
+
+    def axpy(self, alpha, x):
+        """self <- self + alpha x (x of the same partition)."""
+        a = self._scalar(alpha)
+        _check(self.iface.axpy(C.cast(C.byref(a), _vp), x.handle, self.handle), "axpy")
+
+    def extract_diagonal(self):
+        d = np.zeros(self.n, dtype=np.float64 if self.value_type else np.float32)
+        _check(self.iface.extract_diagonal(self.handle, d.ctypes.data, self.n), "extract_diagonal")
+        return d
+
+    def solve_lower(self, b, transpose=False, permute=True):
+        """L x = b (or L^T x = b) with the LLt factor, x overwriting a copy of b."""
+        x = np.array(b, dtype=np.float64 if self.value_type else np.float32, order="F", copy=True)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        fn = self.iface.solve_lower_triangular if permute else self.iface.solve_lower_triangular_dense
+        _check(fn(self.handle, 1 if transpose else 0, x.ctypes.data, nrhs), "solve_lower_triangular")
         return x
 
     def logdet(self):
