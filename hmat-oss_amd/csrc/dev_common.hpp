@@ -24,7 +24,13 @@ constexpr int kBlock = 256;  // threads per workgroup for the one-workgroup-per-
 constexpr int kWaves = kBlock / kWave;
 
 // Built-in interaction kernels (user numbering for the nugget).
-enum KernelKind : int { kExponential = 0, kGaussian = 1, kShiftedExponential = 2 };
+enum KernelKind : int {
+  kExponential = 0,
+  kGaussian = 1,
+  kShiftedExponential = 2,
+  kExponentialDLength = 3,  // d/dl of exp(-r / l): (r / l^2) exp(-r / l)
+  kGaussianDLength = 4      // d/dl of exp(-r^2 / l^2): (2 r^2 / l^3) exp(-r^2 / l^2)
+};
 
 struct KernelParams {
   int kind;
@@ -40,6 +46,12 @@ __host__ __device__ inline double kernel_value(const KernelParams& k, double dx,
   double v;
   if (k.kind == kGaussian) {
     v = exp(-r2 / (k.length * k.length));
+  } else if (k.kind == kExponentialDLength) {
+    const double r = sqrt(r2);
+    v = r / (k.length * k.length) * exp(-r / k.length);
+  } else if (k.kind == kGaussianDLength) {
+    const double l2 = k.length * k.length;
+    v = 2.0 * r2 / (l2 * k.length) * exp(-r2 / l2);
   } else {
     v = exp(-sqrt(r2) / k.length);
   }

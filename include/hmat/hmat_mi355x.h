@@ -32,8 +32,12 @@ HMAT_API void hmat_init_mi355x_interface(hmat_interface_t *i, hmat_value_t type)
 typedef enum {
   HMAT_MI355X_KERNEL_EXPONENTIAL = 0, /* exp(-r / length) + nugget * delta_ij */
   HMAT_MI355X_KERNEL_GAUSSIAN = 1,    /* exp(-r^2 / length^2) + nugget * delta_ij */
-  HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL = 2
+  HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL = 2,
   /* exp(-|x_i - x_j - shift e_z| / length) + nugget * delta_ij: non-symmetric */
+  HMAT_MI355X_KERNEL_EXPONENTIAL_DLENGTH = 3,
+  /* d/dlength of the exponential: (r / length^2) exp(-r / length) + nugget * delta_ij */
+  HMAT_MI355X_KERNEL_GAUSSIAN_DLENGTH = 4
+  /* d/dlength of the Gaussian: (2 r^2 / length^3) exp(-r^2 / length^2) + nugget * delta_ij */
 } hmat_mi355x_kernel_kind_t;
 
 typedef struct {
