@@ -104,6 +104,53 @@ def time_gemv(h, n, reps):
     return gemv_ms, nbytes / (gemv_ms * 1e-3) / 1e9, nbytes
 
 
+def batch_bench(count, threads, seed=3):
+    """Many independent systems (the batching workload): count systems with N
+    log-uniform in [1,000, 50,000], each on its own seeded cylinder (radius
+    U[0.5, 2], height 4 r), Gaussian covariance exp(-r^2/l^2) + 1e-2 I with
+    l ~ U[0.1, 1], leaf size drawn from {16, 32, 64, 128, 256}, ACA+ eps 1e-4,
+    one N(0,1) right-hand side; assembly + H-LLt + solve per system, in FP64
+    and in FP32 + refinement to 1e-10, run through hmx_run_batch."""
+    import hmat_amd
+    from oracle.hmat_oracle import cylinder_points
+    rng = np.random.default_rng(seed)
+    probs = []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(5e4)))))
+        r, l = rng.uniform(0.5, 2.0), rng.uniform(0.1, 1.0)
+        leaf = int(rng.choice([16, 32, 64, 128, 256]))
+        pts = cylinder_points(n, r, 4 * r, 1000 + i)
+        probs.append(dict(tree=hmat_amd.ClusterTree(pts, leaf), length=l, nugget=1e-2, eps=1e-4,
+                          kind=hmat_amd.KERNEL_GAUSSIAN,
+                          b=np.random.default_rng(i).standard_normal((n, 1))))
+    out = {}
+    log(f"batch: {count} systems built")
+    for mode, mixed in (("fp64", False), ("fp32_ir", True)):
+        t = time.time()
+        secs, status = [], []
+        chunk = 32   # one hmx_run_batch call per chunk, so that progress shows
+        for c0 in range(0, count, chunk):
+            _, s_, st_ = hmat_amd.run_batch(probs[c0:c0 + chunk], mixed=mixed, tol=1e-10,
+                                            threads=threads)
+            secs += s_
+            status += st_
+            log(f"batch {mode}: {c0 + len(s_)}/{count} systems, {time.time() - t:.1f}s")
+        wall = time.time() - t
+        ok = [s for s, st in zip(secs, status) if st == 0]
+        out[mode] = {"systems_per_s": len(ok) / wall, "wall_s": wall, "failed": count - len(ok),
+                     "p50_s": float(np.percentile(ok, 50)) if ok else None,
+                     "p99_s": float(np.percentile(ok, 99)) if ok else None}
+        log(f"batch {mode}: {len(ok)}/{count} systems in {wall:.1f}s "
+            f"({len(ok) / wall:.2f}/s), p50 {out[mode]['p50_s']}, p99 {out[mode]['p99_s']}")
+    total_n = int(sum(p["tree"].n for p in probs))
+    print(json.dumps({"metric": "independent SPD systems per second (assembly + H-LLt + solve)",
+                      "value": out["fp32_ir"]["systems_per_s"], "unit": "systems/s", "n_gpus": 1,
+                      "higher_is_better": True, "threads": threads,
+                      "config": {"workload": f"{count} systems, N log-uniform in [1e3, 5e4] "
+                                             f"(total {total_n}), Gaussian kernel, leaves 16-256",
+                                 **out}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +164,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--gemv-only", action="store_true",
                     help="assemble and time the FP64 H-gemv only (for profiling its kernels)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="run this many independent systems through hmx_run_batch instead "
+                         "(the batching workload; prints its own JSON line)")
+    ap.add_argument("--batch-threads", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 2 GiB device to device and exit: the known-size launch that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -135,6 +186,10 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": src.numel() * 8, "check": float(dst[-1])}), flush=True)
+        return
+
+    if args.batch:
+        batch_bench(args.batch, args.batch_threads)
         return
 
     import hmat_amd

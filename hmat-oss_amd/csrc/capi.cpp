@@ -6,8 +6,13 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <functional>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "engine.hpp"
 #include "hmat/hmat.h"
@@ -589,6 +594,63 @@ int hmx_solve_refined_device(hmat_matrix_t* h, double* b, int nrhs, double tol, 
   int it = -1;
   const int rc = guard([&] { it = M(h)->solve_refined(b, nrhs, M(h)->n_rows(), tol, max_iter, resid); });
   return rc ? -1 : it;
+}
+
+int hmx_run_batch(hmat_matrix_t** ms, hmat_assemble_context_t** ctx, int count, int algo,
+                  int mixed, double tol, double** b, const int* nrhs, int threads,
+                  double* seconds, int* status) {
+  return guard([&] {
+    if (count < 0 || threads < 1) throw std::runtime_error("hmx_run_batch: bad count or thread count");
+    // several host threads submit at once: each system keeps to its own stream
+    // and to its share of the device memory for temporaries
+    size_t freeb = 0, total = 0;
+    HMX_CHECK(hipMemGetInfo(&freeb, &total));
+    const int workers = std::min(threads, std::max(count, 1));
+    const size_t share = std::min((size_t)(0.5 * (double)freeb / workers), (size_t)96 << 30);
+    for (int i = 0; i < count; ++i) {
+      M(ms[i])->single_stream = true;
+      M(ms[i])->temp_budget_limit = share;
+    }
+    std::atomic<int> next{0};
+    auto work = [&]() {
+      for (int i = next++; i < count; i = next++) {
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = assemble_generic(ms[i], ctx[i]);
+        if (rc == 0)
+          rc = guard([&] {
+            Matrix* m = M(ms[i]);
+            const int n = m->n_rows();
+            if (!mixed) {
+              m->factorize(algo, false);
+              m->solve_host(b[i], nrhs[i], true);
+              m->release_temporaries();
+              return;
+            }
+            // FP32 factor, refinement against the FP64 matrix on device vectors
+            m->factorize(algo, true);
+            const size_t cnt = (size_t)n * nrhs[i];
+            std::vector<double> hb(cnt);
+            for (int r = 0; r < nrhs[i]; ++r)
+              for (int q = 0; q < n; ++q) hb[q + (size_t)r * n] = b[i][m->rows->indices[q] + (size_t)r * n];
+            hmx::DevBuf d;
+            hmx::upload(d, hb.data(), cnt * sizeof(double), m->stream);
+            double res = 0.0;
+            m->solve_refined(d.as<double>(), nrhs[i], n, tol, 200, &res);
+            if (!(res <= tol)) throw std::runtime_error("hmx_run_batch: refinement did not reach tol");
+            HMX_CHECK(hipMemcpy(hb.data(), d.p, cnt * sizeof(double), hipMemcpyDeviceToHost));
+            for (int r = 0; r < nrhs[i]; ++r)
+              for (int q = 0; q < n; ++q) b[i][m->rows->indices[q] + (size_t)r * n] = hb[q + (size_t)r * n];
+            m->release_temporaries();
+          });
+        seconds[i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        status[i] = rc;
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads && t < count; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+  });
 }
 
 int hmx_partition(hmat_matrix_t* h, int* out) {

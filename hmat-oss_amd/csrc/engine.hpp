@@ -96,6 +96,17 @@ class Matrix {
   hipStream_t stream = nullptr;       // stream every launch goes to
   hipStream_t own_stream = nullptr;   // the engine's own stream
   bool user_stream = false;
+  // every launch on `stream` alone, no side streams or cross-stream events:
+  // set for matrices driven from several host threads at once, whose streams
+  // share the few hardware queues (a wait queued ahead of the event it waits
+  // for, in another thread's stream on the same queue, would never end)
+  bool single_stream = false;
+  // upper bound on the factorisation's temporary memory (0: half the free
+  // device memory, at most 96 GB); matrices factorised side by side share it
+  size_t temp_budget_limit = 0;
+  // frees the factorisation's temporaries and the recorded solve plans that
+  // use them (the factor itself stays)
+  void release_temporaries();
 
   Store<double> s64;
   Store<float> s32;

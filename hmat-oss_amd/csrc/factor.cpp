@@ -164,7 +164,8 @@ long g_type_launches[32];
 #endif
 
 template <typename T>
-void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool capturing = false) {
+void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool capturing = false,
+                  bool single_stream = false) {
   fs.ws.ensure((size_t)std::max<int64_t>(P.ws, 1) * sizeof(double));
   // a fill kernel rather than a memset node, so that the plan can be captured
   if (P.ranks_used) launch_fill_int(fs.ranks.as<int>(), (size_t)P.ranks_used, 0, s);
@@ -184,7 +185,7 @@ void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool cap
 #else
   constexpr double trace = -1.0;
 #endif
-  const bool side_by_side = !capturing && trace < 0;
+  const bool side_by_side = !capturing && trace < 0 && !single_stream;
   if (side_by_side) fs.ensure_side();
   for (size_t q = 0; q < P.launches.size();) {
     // the launches of one wave that may overlap go round-robin over the
@@ -252,6 +253,7 @@ class Exec {
       size_t freeb = 0, total = 0;
       HMX_CHECK(hipMemGetInfo(&freeb, &total));
       fs_.temp_budget = std::min((size_t)(0.5 * (double)freeb), (size_t)96 << 30);
+      if (M.temp_budget_limit) fs_.temp_budget = std::min(fs_.temp_budget, M.temp_budget_limit);
     }
     if (fs_.rank_cap < (1 << 22)) {
       fs_.ranks.alloc(sizeof(int) << 22);
@@ -1641,7 +1643,7 @@ class Exec {
     plan->ranks_used = fs_.rank_used;
     plan->ntasks = (long)n;
     plan->waves = W;
-    execute_plan<T>(*plan, fs_, s);
+    execute_plan<T>(*plan, fs_, s, false, M_.single_stream);
     HMX_CHECK(hipStreamSynchronize(s));
     if (record_) record_->push_back(plan);
     M_.stats.tasks += (long)n;
@@ -1808,10 +1810,20 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part)
     HMX_CHECK(hipMemcpy2DAsync(rhs.p, (size_t)n * sizeof(T), b, (size_t)ldb * sizeof(T),
                                (size_t)n * sizeof(T), nrhs, hipMemcpyDeviceToDevice, stream));
     // replays run as one HIP graph: the substitution is a long chain of small
-    // batched launches whose cost is mostly launch overhead
+    // batched launches whose cost is mostly launch overhead.  Matrices driven
+    // from several host threads replay eagerly (a capture in one thread makes
+    // other threads' synchronous calls fail).
     int64_t wsmax = 1;
     for (const auto& p : it->second) wsmax = std::max(wsmax, p->ws);
     fs.ws.ensure((size_t)wsmax * sizeof(double));
+    if (single_stream) {
+      for (const auto& p : it->second) execute_plan<T>(*p, fs, stream, false, true);
+      HMX_CHECK(hipMemcpy2DAsync(b, (size_t)ldb * sizeof(T), rhs.p, (size_t)n * sizeof(T),
+                                 (size_t)n * sizeof(T), nrhs, hipMemcpyDeviceToDevice, stream));
+      if (!user_stream) HMX_CHECK(hipStreamSynchronize(stream));
+      stats.t_solve = now_s() - t0;
+      return;
+    }
     FactorState::SolveGraph& g = fs.solve_graphs[key];
     if (!g.exec || g.ws != fs.ws.p || g.rhs != rhs.p || g.ranks != fs.ranks.p ||
         g.flags != fs.flags.p) {
@@ -1875,6 +1887,19 @@ void Matrix::solve_host(void* b, int nrhs, bool permute, int part) {
   } else {
     throw std::runtime_error("solve: complex types are not supported by this engine");
   }
+}
+
+void Matrix::release_temporaries() {
+  if (!fstate_) return;
+  HMX_CHECK(hipStreamSynchronize(stream));
+  FactorState& fs = *fstate_;
+  fs.drop_graphs();
+  fs.solve_plans.clear();
+  fs.solve_rhs.clear();
+  fs.chunks.clear();
+  fs.chunk = fs.chunk_used = fs.temp_total = fs.temp_used = 0;
+  fs.temp_budget = 0;
+  fs.ws.release();
 }
 
 void Matrix::truncate_all() {

@@ -132,6 +132,8 @@ _EXPORTS = {
     "hmx_gemv_device": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_double, _vp, C.c_int]),
     "hmx_gemv_device_noalloc": (C.c_int, [_vp, C.c_int, C.c_double, _vp, C.c_double, _vp,
                                           C.c_int]),
+    "hmx_run_batch": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double, _vp, _vp,
+                                C.c_int, _vp, _vp]),
     "hmx_solve_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int]),
     "hmx_factorize_mixed": (C.c_int, [_vp, C.c_int]),
     "hmx_solve_refined_device": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int,
@@ -227,8 +229,10 @@ class HMatrix:
     def n(self):
         return self.tree.n
 
-    def assemble(self, kind=KERNEL_EXPONENTIAL, length=0.5, nugget=1e-2, eps=1e-4,
-                 method="aca_plus", factorization=FACT_NONE, shift=0.0):
+    def assemble_context(self, kind=KERNEL_EXPONENTIAL, length=0.5, nugget=1e-2, eps=1e-4,
+                         method="aca_plus", factorization=FACT_NONE, shift=0.0):
+        """An hmat_assemble_context_t for this matrix (the kernel as data, see
+        hmat_mi355x_kernel_t), plus the compression object to delete after use."""
         lib = load()
         k = hmat_mi355x_kernel_t()
         k.kind, k.dimension, k.n = kind, self.tree.dim, self.tree.n
@@ -245,8 +249,13 @@ class HMatrix:
         ctx.user_context = C.cast(C.pointer(k), _vp).value
         ctx.lower_symmetric = 1 if self.lower_symmetric else 0
         ctx.factorization = factorization
+        return ctx, comp
+
+    def assemble(self, kind=KERNEL_EXPONENTIAL, length=0.5, nugget=1e-2, eps=1e-4,
+                 method="aca_plus", factorization=FACT_NONE, shift=0.0):
+        ctx, comp = self.assemble_context(kind, length, nugget, eps, method, factorization, shift)
         rc = self.iface.assemble_generic(self.handle, C.byref(ctx))
-        lib.hmat_delete_compression(comp)
+        load().hmat_delete_compression(comp)
         _check(rc, "assemble_generic")
 
     def factorize(self, algo=FACT_LLT, mixed=False):
@@ -435,3 +444,40 @@ class HMatrix:
             self.close()
         except Exception:
             pass
+
+
+def run_batch(problems, algo=FACT_LLT, mixed=False, tol=1e-10, threads=8):
+    """Assemble, factorise and solve independent systems through hmx_run_batch.
+
+    problems: list of dicts with 'tree' (ClusterTree), 'b' (N x nrhs, user
+    numbering) and the assemble() keywords ('kind', 'length', 'nugget', 'eps',
+    optionally 'lowrank_eps').  Returns (solutions, seconds per system,
+    status per system); the H-matrices are destroyed afterwards."""
+    lib = load()
+    count = len(problems)
+    mats, ctxs, comps, bs = [], [], [], []
+    try:
+        for p in problems:
+            h = HMatrix(p["tree"], lowrank_eps=p.get("lowrank_eps", p.get("eps", 1e-4)))
+            ctx, comp = h.assemble_context(kind=p.get("kind", KERNEL_EXPONENTIAL),
+                                           length=p["length"], nugget=p["nugget"],
+                                           eps=p.get("eps", 1e-4))
+            b = np.array(p["b"], dtype=np.float64, order="F", copy=True)
+            mats.append(h)
+            ctxs.append(ctx)
+            comps.append(comp)
+            bs.append(b)
+        handles = (_vp * count)(*[m.handle for m in mats])
+        cptrs = (_vp * count)(*[C.cast(C.pointer(c), _vp).value for c in ctxs])
+        bptrs = (_vp * count)(*[b.ctypes.data for b in bs])
+        nrhs = (C.c_int * count)(*[1 if b.ndim == 1 else b.shape[1] for b in bs])
+        secs = (C.c_double * count)()
+        status = (C.c_int * count)()
+        _check(lib.hmx_run_batch(handles, cptrs, count, algo, 1 if mixed else 0, float(tol), bptrs,
+                                 nrhs, int(threads), secs, status), "hmx_run_batch")
+        return bs, list(secs), list(status)
+    finally:
+        for c in comps:
+            lib.hmat_delete_compression(c)
+        for m in mats:
+            m.close()

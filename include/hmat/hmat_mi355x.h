@@ -90,6 +90,19 @@ HMAT_API int hmx_factorize_mixed(hmat_matrix_t *m, int algo);
 HMAT_API int hmx_solve_refined_device(hmat_matrix_t *m, double *b, int nrhs, double tol,
                                       int max_iter, double *resid);
 
+/* No counterpart in hmat-oss, which handles one system per call
+ * (assemble_generic c_wrapping.hpp:57, factorize_generic :139, solve_systems
+ * :503).  Runs `count` independent systems: for each, assemble_generic(ms[i],
+ * ctx[i]), factorise with `algo` (an FP32 factor refined against the FP64
+ * matrix to `tol` when `mixed`, else a plain factor in the matrix's
+ * precision) and solve b[i] (N_i x nrhs[i], user numbering, in place).
+ * `threads` host threads take systems in turn, each system on its own
+ * stream, so the device runs several systems' launches side by side.
+ * seconds[i] receives the wall time of system i, status[i] 0 or 1. */
+HMAT_API int hmx_run_batch(hmat_matrix_t **ms, hmat_assemble_context_t **ctx, int count,
+                           int algo, int mixed, double tol, double **b, const int *nrhs,
+                           int threads, double *seconds, int *status);
+
 /* Partition in preorder: 5 ints per block node (row_offset, row_size,
  * col_offset, col_size, kind 0 full / 1 rk / 2 inner).  Returns the node
  * count; writes only when out != NULL. */
