@@ -90,11 +90,27 @@ int assemble_generic(hmat_matrix_t* h, hmat_assemble_context_t* ctx) {
     if (!ctx->compression) throw std::runtime_error("No compression algorithm defined in hmat_assemble_context_t");
     if (ctx->lower_symmetric && !m->lower_sym)
       throw std::runtime_error("lower_symmetric assembly needs a matrix created lower-symmetric");
-    if (ctx->simple_compute != hmat_mi355x_kernel_compute || ctx->assembly || ctx->block_compute ||
-        ctx->advanced_compute)
-      throw std::runtime_error(
-          "the MI355X engine assembles on the device: set simple_compute = "
-          "hmat_mi355x_kernel_compute with an hmat_mi355x_kernel_t user_context");
+    if (ctx->assembly)
+      throw std::runtime_error("assembly: C++ assembly objects are not supported by this engine");
+    const auto* c = reinterpret_cast<const Compression*>(ctx->compression);
+    hmx::CompressionSpec cs{c->method, c->eps};
+    if (ctx->simple_compute != hmat_mi355x_kernel_compute) {
+      // the caller's host functions: blocks evaluated on the host, compressed
+      // on the device (only one of advanced / block / simple is used, advanced
+      // first, as assembly.cpp and c_wrapping.hpp:66-95 do)
+      hmx::HostFunctions f;
+      f.user_context = ctx->user_context;
+      f.prepare = ctx->prepare;
+      if (ctx->advanced_compute)
+        f.advanced = ctx->advanced_compute;
+      else if (ctx->block_compute)
+        f.block = ctx->block_compute;
+      else
+        f.simple = ctx->simple_compute;
+      m->assemble_host(f, cs);
+      if (ctx->factorization != hmat_factorization_none) m->factorize(ctx->factorization, false);
+      return;
+    }
     const auto* k = static_cast<const hmat_mi355x_kernel_t*>(ctx->user_context);
     hmx::KernelSpec spec;
     spec.kind = k->kind;
@@ -106,8 +122,6 @@ int assemble_generic(hmat_matrix_t* h, hmat_assemble_context_t* ctx) {
     spec.dim = k->dimension;
     spec.n = k->n;
     spec.points.assign(k->points, k->points + (size_t)k->n * k->dimension);
-    const auto* c = reinterpret_cast<const Compression*>(ctx->compression);
-    hmx::CompressionSpec cs{c->method, c->eps};
     m->assemble(spec, cs);
     if (ctx->factorization != hmat_factorization_none) m->factorize(ctx->factorization, false);
   });

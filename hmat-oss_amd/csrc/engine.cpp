@@ -195,6 +195,10 @@ void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
                        s64.ranks.as<int>(), overflow_.as<int>(), ws.as<double>(), rp, cp, kp,
                        comp.eps, lowrank_eps, partial, stream);
   }
+  finish_assembly(t0);
+}
+
+void Matrix::finish_assembly(double t0) {
   HMX_CHECK(hipStreamSynchronize(stream));
   int ov = 0;
   HMX_CHECK(hipMemcpy(&ov, overflow_.p, sizeof(int), hipMemcpyDeviceToHost));
@@ -215,6 +219,138 @@ void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
   plan_.nrhs = 0;
   build_gemv_plan();
   stats.t_assemble = now() - t0;
+}
+
+namespace {
+// One leaf block (rows R, cols C) from the caller's host functions, column-major
+// with ld m (SimpleFunction::assemble / BlockFunction::assemble, assembly.cpp:75-185).
+void evaluate_block(const HostFunctions& f, const ClusterTree& rt, const ClusterTree& ct,
+                    const Cluster* R, const Cluster* C, double* out) {
+  const int m = R->size, n = C->size;
+  if (f.simple) {
+    const int* ri = rt.indices.data() + R->offset;
+    const int* ci = ct.indices.data() + C->offset;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) f.simple(f.user_context, ri[i], ci[j], out + i + (size_t)j * m);
+    return;
+  }
+  hmat_block_info_t info;
+  std::memset(&info, 0, sizeof(info));
+  info.block_type = hmat_block_full;
+  info.number_of_strata = 1;
+  auto prepare = [&]() {
+    f.prepare(R->offset, m, C->offset, n, const_cast<int*>(rt.indices.data()),
+              const_cast<int*>(rt.indices_rev.data()), const_cast<int*>(ct.indices.data()),
+              const_cast<int*>(ct.indices_rev.data()), f.user_context, &info);
+  };
+  prepare();
+  if (info.needed_memory != 0) prepare();  // BlockFunction::prepareBlock calls it again
+  if (info.block_type == hmat_block_null) {
+    std::memset(out, 0, sizeof(double) * (size_t)m * n);
+  } else if (f.advanced) {
+    hmat_block_compute_context_t ac;
+    ac.block = out;
+    ac.row_start = 0;
+    ac.row_count = m;
+    ac.col_start = 0;
+    ac.col_count = n;
+    ac.stratum = -1;
+    ac.user_data = info.user_data;
+    f.advanced(&ac);
+  } else {
+    f.block(info.user_data, 0, m, 0, n, out);
+  }
+  if (info.release_user_data) info.release_user_data(info.user_data);
+}
+}  // namespace
+
+void Matrix::assemble_host(const HostFunctions& f, const CompressionSpec& comp) {
+  const double t0 = now();
+  if (!f.simple && !((f.block || f.advanced) && f.prepare))
+    throw std::runtime_error("assembly: set simple_compute, or block_compute / advanced_compute with prepare");
+  device_init();
+  init_store(s64);
+  HMX_CHECK(hipMemsetAsync(overflow_.p, 0, sizeof(int), stream));
+  PointSet none{nullptr, nullptr, 0};
+  KernelParams kp{kExponential, 1.0, 0.0, 0.0};
+  // leaves in staging batches of at most kStage doubles (a leaf larger than
+  // that is a batch of its own)
+  const int64_t kStageElems = (int64_t)1 << 26;  // 512 MB of doubles
+  const int64_t aca_budget = (int64_t)4 << 27;   // doubles of ACA workspace per launch
+  const int nleaf = (int)bt->leaves.size();
+  std::vector<double> host;
+  DevBuf table, ws, dl, doff, dt;
+  int l0 = 0;
+  while (l0 < nleaf) {
+    int64_t used = 0;
+    int l1 = l0;
+    while (l1 < nleaf) {
+      const int64_t e = (int64_t)s64.hdesc[l1].m * s64.hdesc[l1].n;
+      if (l1 > l0 && used + e > kStageElems) break;
+      used += e;
+      ++l1;
+    }
+    host.resize((size_t)std::max<int64_t>(used, 1));
+    std::vector<int> full, svd;
+    std::vector<int64_t> full_off, svd_off;
+    std::vector<AcaTask> aca;
+    int64_t off = 0;
+    for (int l = l0; l < l1; ++l) {
+      const Block* b = bt->leaves[l];
+      const LeafDesc& L = s64.hdesc[l];
+      evaluate_block(f, *rows, *cols, b->rows, b->cols, host.data() + off);
+      if (L.kind == 0) {
+        full.push_back(l);
+        full_off.push_back(off);
+      } else if (std::max(L.m, L.n) < 100 || comp.method == 0 /*svd*/) {
+        if (std::max(L.m, L.n) >= 100)
+          throw std::runtime_error("SVD compression is implemented for blocks below 100 rows/cols");
+        svd.push_back(l);
+        svd_off.push_back(off);
+      } else {
+        AcaTask t{l, std::min(std::min(L.m, L.n), 1024), 0};
+        t.tab = off;
+        aca.push_back(t);
+      }
+      off += (int64_t)L.m * L.n;
+    }
+    HMX_CHECK(hipStreamSynchronize(stream));
+    upload(table, host.data(), (size_t)used * sizeof(double), stream);
+    upload(dl, full.data(), full.size() * sizeof(int), stream);
+    upload(doff, full_off.data(), full_off.size() * sizeof(int64_t), stream);
+    launch_table_full<double>(s64.desc.as<LeafDesc>(), dl.as<int>(), doff.as<int64_t>(),
+                              (int)full.size(), table.as<double>(), s64.data(), stream);
+    HMX_CHECK(hipStreamSynchronize(stream));
+    upload(dl, svd.data(), svd.size() * sizeof(int), stream);
+    upload(doff, svd_off.data(), svd_off.size() * sizeof(int64_t), stream);
+    launch_svd_compress<double>(s64.desc.as<LeafDesc>(), dl.as<int>(), (int)svd.size(), s64.data(),
+                                s64.ranks.as<int>(), overflow_.as<int>(), none, none, kp,
+                                std::max(comp.eps, lowrank_eps), stream, doff.as<int64_t>(),
+                                table.as<double>());
+    size_t i = 0;
+    while (i < aca.size()) {
+      std::vector<AcaTask> batch;
+      int64_t wsu = 0;
+      while (i < aca.size()) {
+        const LeafDesc& L = s64.hdesc[aca[i].leaf];
+        const int64_t need = aca_workspace(L.m, L.n, aca[i].kmax);
+        if (!batch.empty() && wsu + need > aca_budget) break;
+        AcaTask t = aca[i];
+        t.ws = wsu;
+        batch.push_back(t);
+        wsu += need;
+        ++i;
+      }
+      HMX_CHECK(hipStreamSynchronize(stream));
+      ws.ensure((size_t)wsu * sizeof(double));
+      upload(dt, batch.data(), batch.size() * sizeof(AcaTask), stream);
+      launch_aca<double>(s64.desc.as<LeafDesc>(), dt.as<AcaTask>(), (int)batch.size(), s64.data(),
+                         s64.ranks.as<int>(), overflow_.as<int>(), ws.as<double>(), none, none,
+                         kp, comp.eps, lowrank_eps, comp.method == 2, stream, table.as<double>());
+    }
+    l0 = l1;
+  }
+  finish_assembly(t0);
 }
 
 // ---------------------------------------------------------------------------

@@ -51,6 +51,16 @@ struct CompressionSpec {
   double eps = 1e-4;
 };
 
+// Entries from the caller's host functions (hmat_assemble_context_t,
+// assembly.cpp:75-292): one of simple, block or advanced is set.
+struct HostFunctions {
+  hmat_interaction_func_t simple = nullptr;
+  hmat_compute_func_t block = nullptr;
+  void (*advanced)(struct hmat_block_compute_context_t*) = nullptr;
+  hmat_prepare_func_t prepare = nullptr;
+  void* user_context = nullptr;
+};
+
 struct KernelSpec {
   int kind = kExponential;
   double length = 1.0;
@@ -123,6 +133,10 @@ class Matrix {
 
   void device_init();   // stream + device scratch, created on first GPU use
   void assemble(const KernelSpec& k, const CompressionSpec& c);
+  // Assembly from host functions: every leaf block is evaluated on the host
+  // (staged in batches), then compressed on the device exactly as the
+  // built-in kernels' blocks are.
+  void assemble_host(const HostFunctions& f, const CompressionSpec& c);
   // y = alpha * A * x + beta * y, device pointers in hmat numbering, ld = N.
   template <typename T>
   void gemv_device(double alpha, const T* x, double beta, T* y, int nrhs, int ldx, int ldy,
@@ -166,6 +180,7 @@ class Matrix {
 
  private:
   void build_storage();
+  void finish_assembly(double t0);
   template <typename T>
   void init_store(Store<T>& st);
   void build_gemv_plan();

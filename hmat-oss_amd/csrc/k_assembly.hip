@@ -29,13 +29,28 @@ __global__ void __launch_bounds__(kBlock) k_assemble_full(const LeafDesc* desc, 
   }
 }
 
+// Full leaves whose entries were evaluated on the host (user callbacks): the
+// staged block (column-major, ld m, doubles) converted into the leaf storage.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_table_full(const LeafDesc* desc, const int* list,
+                                                       const int64_t* tab_off, const double* table,
+                                                       T* arena) {
+  const LeafDesc d = desc[list[blockIdx.x]];
+  const double* src = table + tab_off[blockIdx.x];
+  T* out = arena + d.a;
+  const int total = d.m * d.n;
+  for (int e = threadIdx.x; e < total; e += kBlock) out[e] = (T)src[e];
+}
+
 constexpr int kSvdMax = 99;  // compressionMinLeafSize - 1
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_svd_compress(const LeafDesc* desc, const int* list,
                                                          T* arena, int* ranks, int* overflow,
                                                          PointSet rp, PointSet cp,
-                                                         KernelParams kp, double eps) {
+                                                         KernelParams kp, double eps,
+                                                         const int64_t* tab_off,
+                                                         const double* table) {
   __shared__ double G[kSvdMax * kSvdMax];
   __shared__ double V[kSvdMax * kSvdMax];
   __shared__ double sig[kSvdMax];
@@ -46,9 +61,11 @@ __global__ void __launch_bounds__(kBlock) k_svd_compress(const LeafDesc* desc, c
   // G = A (m >= n) or A^T (m < n), so that G has min(m, n) columns.
   const bool tr = d.m < d.n;
   const int gr = tr ? d.n : d.m, gc = tr ? d.m : d.n;
+  // entries from the kernel, or from a block staged by the host (tab_off)
+  const double* tab = tab_off ? table + tab_off[blockIdx.x] : nullptr;
   for (int e = threadIdx.x; e < d.m * d.n; e += kBlock) {
     const int i = e % d.m, j = e / d.m;
-    const double v = entry(rp, cp, kp, d.r0 + i, d.c0 + j);
+    const double v = tab ? tab[e] : entry(rp, cp, kp, d.r0 + i, d.c0 + j);
     if (tr)
       G[j + i * gr] = v;
     else
@@ -124,12 +141,21 @@ struct AcaCtx {
   const PointSet* cp;
   const KernelParams* kp;
   int r0, c0, m, n;
+  const double* tab;  // the block staged by the host (ld m), or null: evaluate the kernel
 };
 
 __device__ inline void get_row(const AcaCtx& c, int i, double* out) {
+  if (c.tab) {
+    for (int j = threadIdx.x; j < c.n; j += kBlock) out[j] = c.tab[i + (size_t)j * c.m];
+    return;
+  }
   for (int j = threadIdx.x; j < c.n; j += kBlock) out[j] = entry(*c.rp, *c.cp, *c.kp, c.r0 + i, c.c0 + j);
 }
 __device__ inline void get_col(const AcaCtx& c, int j, double* out) {
+  if (c.tab) {
+    for (int i = threadIdx.x; i < c.m; i += kBlock) out[i] = c.tab[i + (size_t)j * c.m];
+    return;
+  }
   for (int i = threadIdx.x; i < c.m; i += kBlock) out[i] = entry(*c.rp, *c.cp, *c.kp, c.r0 + i, c.c0 + j);
 }
 // vec[t] -= sum_{l<k} other_l[idx] * same_l[t]   (updateRow compression.cpp:82-89)
@@ -429,7 +455,8 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_aca(const LeafDesc* desc, const AcaTask* tasks,
                                                 T* arena, int* ranks, int* overflow, double* ws,
                                                 PointSet rp, PointSet cp, KernelParams kp,
-                                                double aca_eps, double trunc_eps, int partial) {
+                                                double aca_eps, double trunc_eps, int partial,
+                                                const double* table) {
   __shared__ double red[kWaves * kQrChunk];
   __shared__ double rv[kWaves];
   __shared__ int ri[kWaves];
@@ -438,7 +465,7 @@ __global__ void __launch_bounds__(kBlock) k_aca(const LeafDesc* desc, const AcaT
   const AcaTask t = tasks[blockIdx.x];
   const LeafDesc d = desc[t.leaf];
   AcaWs w = carve(ws + t.ws, d.m, d.n, t.kmax);
-  AcaCtx c{&rp, &cp, &kp, d.r0, d.c0, d.m, d.n};
+  AcaCtx c{&rp, &cp, &kp, d.r0, d.c0, d.m, d.n, t.tab >= 0 ? table + t.tab : nullptr};
   int k;
   if (partial || (long)d.n * 100 < (long)d.m)
     k = aca_partial(c, w, t.kmax, aca_eps, red, rv, ri, tmp);
@@ -475,20 +502,30 @@ void launch_assemble_full(const LeafDesc* desc, const int* list, int count, T* a
 template <typename T>
 void launch_svd_compress(const LeafDesc* desc, const int* list, int count, T* arena, int* ranks,
                          int* overflow, PointSet rp, PointSet cp, KernelParams kp, double eps,
-                         hipStream_t s) {
+                         hipStream_t s, const int64_t* tab_off, const double* table) {
   if (count == 0) return;
   hipLaunchKernelGGL(k_svd_compress<T>, dim3(count), dim3(kBlock), 0, s, desc, list, arena, ranks,
-                     overflow, rp, cp, kp, eps);
+                     overflow, rp, cp, kp, eps, tab_off, table);
   HMX_CHECK(hipGetLastError());
 }
 
 template <typename T>
 void launch_aca(const LeafDesc* desc, const AcaTask* tasks, int count, T* arena, int* ranks,
                 int* overflow, double* ws, PointSet rp, PointSet cp, KernelParams kp,
-                double aca_eps, double trunc_eps, int partial, hipStream_t s) {
+                double aca_eps, double trunc_eps, int partial, hipStream_t s,
+                const double* table) {
   if (count == 0) return;
   hipLaunchKernelGGL(k_aca<T>, dim3(count), dim3(kBlock), 0, s, desc, tasks, arena, ranks,
-                     overflow, ws, rp, cp, kp, aca_eps, trunc_eps, partial);
+                     overflow, ws, rp, cp, kp, aca_eps, trunc_eps, partial, table);
+  HMX_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_table_full(const LeafDesc* desc, const int* list, const int64_t* tab_off, int count,
+                       const double* table, T* arena, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_table_full<T>, dim3(count), dim3(kBlock), 0, s, desc, list, tab_off, table,
+                     arena);
   HMX_CHECK(hipGetLastError());
 }
 
@@ -496,10 +533,13 @@ void launch_aca(const LeafDesc* desc, const AcaTask* tasks, int count, T* arena,
   template void launch_assemble_full<T>(const LeafDesc*, const int*, int, T*, PointSet,       \
                                         PointSet, KernelParams, hipStream_t);                 \
   template void launch_svd_compress<T>(const LeafDesc*, const int*, int, T*, int*, int*,      \
-                                       PointSet, PointSet, KernelParams, double, hipStream_t); \
+                                       PointSet, PointSet, KernelParams, double, hipStream_t,  \
+                                       const int64_t*, const double*);                         \
   template void launch_aca<T>(const LeafDesc*, const AcaTask*, int, T*, int*, int*, double*,  \
                               PointSet, PointSet, KernelParams, double, double, int,          \
-                              hipStream_t);
+                              hipStream_t, const double*);                                    \
+  template void launch_table_full<T>(const LeafDesc*, const int*, const int64_t*, int,        \
+                                     const double*, T*, hipStream_t);
 HMX_INST(double)
 HMX_INST(float)
 

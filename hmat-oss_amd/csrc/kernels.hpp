@@ -35,15 +35,24 @@ void launch_assemble_full(const LeafDesc* desc, const int* list, int count, T* a
 // Admissible leaves with max(m, n) < 100: dense evaluation + Jacobi SVD +
 // truncation (compression.cpp:224-251).  ranks[leaf] set; overflow[0] |= 1 if
 // a rank exceeds the leaf capacity.
+// tab_off (one offset per listed leaf, device) and table: take the entries
+// from blocks staged by the host (column-major, ld m) instead of the kernel.
 template <typename T>
 void launch_svd_compress(const LeafDesc* desc, const int* list, int count, T* arena, int* ranks,
                          int* overflow, PointSet rp, PointSet cp, KernelParams kp, double eps,
-                         hipStream_t s);
+                         hipStream_t s, const int64_t* tab_off = nullptr,
+                         const double* table = nullptr);
+
+// Full leaves from blocks staged by the host (one offset per listed leaf).
+template <typename T>
+void launch_table_full(const LeafDesc* desc, const int* list, const int64_t* tab_off, int count,
+                       const double* table, T* arena, hipStream_t s);
 
 struct AcaTask {
   int leaf;
   int kmax;       // capacity of the ACA panels
   int64_t ws;     // offset (doubles) of this task's workspace
+  int64_t tab = -1;  // offset of the block staged by the host in the table, or -1
 };
 
 // Workspace (doubles) needed by one ACA task followed by its truncation.
@@ -55,7 +64,8 @@ int64_t aca_workspace(int m, int n, int kmax);
 template <typename T>
 void launch_aca(const LeafDesc* desc, const AcaTask* tasks, int count, T* arena, int* ranks,
                 int* overflow, double* ws, PointSet rp, PointSet cp, KernelParams kp,
-                double aca_eps, double trunc_eps, int partial, hipStream_t s);
+                double aca_eps, double trunc_eps, int partial, hipStream_t s,
+                const double* table = nullptr);
 
 // ---- H-gemv -----------------------------------------------------------------
 // Work items of the z pass: z_part = P^T x over a row range of a panel of at

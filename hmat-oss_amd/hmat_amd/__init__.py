@@ -44,6 +44,21 @@ class hmat_assemble_context_t(C.Structure):
                 ("progress", _vp)]
 
 
+class hmat_block_info_t(C.Structure):
+    _fields_ = [("block_type", C.c_int), ("user_data", _vp), ("release_user_data", _vp),
+                ("is_guaranteed_null_row", _vp), ("is_guaranteed_null_col", _vp),
+                ("needed_memory", C.c_size_t), ("number_of_strata", C.c_int)]
+
+
+# host-function types of hmat_assemble_context_t (hmat.h: hmat_interaction_func_t,
+# hmat_prepare_func_t, hmat_compute_func_t)
+INTERACTION_FUNC = C.CFUNCTYPE(None, _vp, C.c_int, C.c_int, _vp)
+PREPARE_FUNC = C.CFUNCTYPE(None, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                           C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _vp,
+                           C.POINTER(hmat_block_info_t))
+COMPUTE_FUNC = C.CFUNCTYPE(None, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp)
+
+
 class hmat_factorization_context_t(C.Structure):
     _fields_ = [("factorization", C.c_int), ("progress", _vp)]
 
@@ -256,6 +271,29 @@ class HMatrix:
         ctx, comp = self.assemble_context(kind, length, nugget, eps, method, factorization, shift)
         rc = self.iface.assemble_generic(self.handle, C.byref(ctx))
         load().hmat_delete_compression(comp)
+        _check(rc, "assemble_generic")
+
+    def assemble_functions(self, simple=None, block=None, prepare=None, eps=1e-4,
+                           method="aca_plus"):
+        """assemble_generic with the caller's host functions (ctypes callbacks
+        of INTERACTION_FUNC, or COMPUTE_FUNC + PREPARE_FUNC): every block is
+        evaluated on the host, then compressed on the device."""
+        lib = load()
+        ctx = hmat_assemble_context_t()
+        lib.hmat_assemble_context_init(C.byref(ctx))
+        comp = {"aca_plus": lib.hmat_create_compression_aca_plus,
+                "aca_partial": lib.hmat_create_compression_aca_partial,
+                "svd": lib.hmat_create_compression_svd}[method](float(eps))
+        ctx.compression = comp
+        if simple is not None:
+            ctx.simple_compute = C.cast(simple, _vp).value
+        if block is not None:
+            ctx.block_compute = C.cast(block, _vp).value
+            ctx.prepare = C.cast(prepare, _vp).value
+        ctx.lower_symmetric = 1 if self.lower_symmetric else 0
+        self._callbacks = (simple, block, prepare)
+        rc = self.iface.assemble_generic(self.handle, C.byref(ctx))
+        lib.hmat_delete_compression(comp)
         _check(rc, "assemble_generic")
 
     def factorize(self, algo=FACT_LLT, mixed=False):
