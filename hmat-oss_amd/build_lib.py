@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HOST_ONLY = ["tree.cpp"]
-HIP_SRC = ["engine.cpp", "factor.cpp", "sweep.cpp", "ops.cpp", "capi.cpp", "k_assembly.hip",
+HIP_SRC = ["engine.cpp", "factor.cpp", "sweep.cpp", "ops.cpp", "serialize.cpp", "capi.cpp", "k_assembly.hip",
            "k_gemv.hip", "k_factor.hip", "k_sweep.hip", "k_ops.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]

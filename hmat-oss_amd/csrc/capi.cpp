@@ -325,8 +325,22 @@ int u_set_ct(hmat_matrix_t*, const hmat_cluster_tree_t*, const hmat_cluster_tree
 int u_values(struct hmat_get_values_context_t*) { return unsupported("get_block/get_values"); }
 int u_walk(hmat_matrix_t*, hmat_procedure_t*) { return unsupported("walk"); }
 int u_leaf(hmat_matrix_t*, const hmat_leaf_procedure_t*) { return unsupported("apply_on_leaf"); }
-hmat_matrix_t* u_read_struct(hmat_iostream, void*) { unsupported("read_struct"); return nullptr; }
-void u_write(hmat_matrix_t*, hmat_iostream, void*) { unsupported("write_struct/read_data/write_data"); }
+// read_struct / write_struct / read_data / write_data c_wrapping.hpp:921-946
+template <int VT>
+hmat_matrix_t* read_struct(hmat_iostream f, void* ud) {
+  hmat_matrix_t* r = nullptr;
+  guard([&] { r = reinterpret_cast<hmat_matrix_t*>(hmx::serialize_read_struct(f, ud, VT, g_rank_cap)); });
+  return r;
+}
+void write_struct(hmat_matrix_t* h, hmat_iostream f, void* ud) {
+  guard([&] { hmx::serialize_write_struct(*M(h), f, ud); });
+}
+void read_data(hmat_matrix_t* h, hmat_iostream f, void* ud) {
+  guard([&] { hmx::serialize_read_data(*M(h), f, ud); });
+}
+void write_data(hmat_matrix_t* h, hmat_iostream f, void* ud) {
+  guard([&] { hmx::serialize_write_data(*M(h), f, ud); });
+}
 void u_progress(hmat_matrix_t*, hmat_progress_t*) {}
 int u_diag_block(hmat_matrix_t*, int, void*) { return unsupported("extract_diagonal_block"); }
 
@@ -385,10 +399,10 @@ void fill(hmat_interface_t* i) {
   i->apply_on_leaf = u_leaf;
   i->value_type = (hmat_value_t)VT;
   i->internal = nullptr;
-  i->read_struct = u_read_struct;
-  i->write_struct = u_write;
-  i->read_data = u_write;
-  i->write_data = u_write;
+  i->read_struct = read_struct<VT>;
+  i->write_struct = write_struct;
+  i->read_data = read_data;
+  i->write_data = write_data;
   i->set_progressbar = u_progress;
   i->extract_diagonal_block = u_diag_block;
 }

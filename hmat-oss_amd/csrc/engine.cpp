@@ -52,6 +52,14 @@ Matrix::Matrix(const ClusterTree* r, const ClusterTree* c, bool sym,
   build_storage();
 }
 
+Matrix::Matrix(const ClusterTree* r, const ClusterTree* c, bool sym, std::unique_ptr<BlockTree> tree,
+               int vt, int rank_cap, const std::vector<int>& cap_hint)
+    : value_type(vt), rows(r), cols(c), lower_sym(sym), rk_cap_max(rank_cap) {
+  if (sym && r != c) throw std::runtime_error("lower-symmetric storage needs rows tree == cols tree");
+  bt = std::move(tree);
+  build_storage(&cap_hint);
+}
+
 void Matrix::device_init() {
   if (own_stream) return;
   HMX_CHECK(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
@@ -69,7 +77,7 @@ Matrix::~Matrix() {
 
 // One arena layout shared by both precisions: full leaves m*n, Rk leaves
 // (m + n) * cap with cap = min(m, n, rk_cap_max); every leaf 64-element aligned.
-void Matrix::build_storage() {
+void Matrix::build_storage(const std::vector<int>* cap_hint) {
   std::vector<LeafDesc> d(bt->leaves.size());
   int64_t off = 0;
   auto align = [](int64_t x) { return (x + 63) / 64 * 64; };
@@ -88,7 +96,8 @@ void Matrix::build_storage() {
       off = align(off + (int64_t)L.m * L.n);
     } else {
       L.kind = 1;
-      L.cap = std::min(std::min(L.m, L.n), rk_cap_max);
+      L.cap = std::min(std::min(L.m, L.n),
+                       std::max(rk_cap_max, cap_hint && i < cap_hint->size() ? (*cap_hint)[i] : 0));
       L.a = off;
       off = align(off + (int64_t)L.m * L.cap);
       L.b = off;

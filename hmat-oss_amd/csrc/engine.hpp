@@ -92,6 +92,11 @@ class Matrix {
  public:
   Matrix(const ClusterTree* rows, const ClusterTree* cols, bool lower_sym,
          const AdmissibilityParams& adm, int value_type, int rank_cap);
+  // A matrix over a given partition (read from a serialized structure);
+  // cap_hint[l] raises the Rk capacity of leaf l (the rank it will hold).
+  Matrix(const ClusterTree* rows, const ClusterTree* cols, bool lower_sym,
+         std::unique_ptr<BlockTree> tree, int value_type, int rank_cap,
+         const std::vector<int>& cap_hint);
   AdmissibilityParams adm;
   ~Matrix();
 
@@ -179,7 +184,7 @@ class Matrix {
   Store<T>& store();
 
  private:
-  void build_storage();
+  void build_storage(const std::vector<int>* cap_hint = nullptr);
   void finish_assembly(double t0);
   template <typename T>
   void init_store(Store<T>& st);
@@ -191,6 +196,13 @@ class Matrix {
   DevBuf overflow_;
   std::shared_ptr<struct FactorState> fstate_;
   friend struct FactorState;
+  friend struct Serializer;
 };
+
+// Serialization in the hmat-oss stream format (serialize.cpp).
+void serialize_write_struct(Matrix& m, hmat_iostream f, void* user_data);
+void serialize_write_data(Matrix& m, hmat_iostream f, void* user_data);
+Matrix* serialize_read_struct(hmat_iostream f, void* user_data, int value_type, int rank_cap);
+void serialize_read_data(Matrix& m, hmat_iostream f, void* user_data);
 
 }  // namespace hmx

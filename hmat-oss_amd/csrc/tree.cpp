@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <stdexcept>
 #include <utility>
 
@@ -125,6 +126,40 @@ std::unique_ptr<ClusterTree> build_cluster_tree(const double* coords, int dim, i
   return t;
 }
 
+std::unique_ptr<ClusterTree> cluster_tree_from(const double* coords, int dim, int n,
+                                               const std::vector<int>& indices,
+                                               const std::vector<ClusterNodeDesc>& nodes) {
+  if (dim < 1 || dim > 3) throw std::runtime_error("cluster tree: dimension must be 1..3");
+  if (n < 1 || (int)indices.size() != n || nodes.empty())
+    throw std::runtime_error("cluster tree: inconsistent serialized tree");
+  auto t = std::make_unique<ClusterTree>();
+  t->n = n;
+  t->dim = dim;
+  t->coords.assign(coords, coords + (size_t)n * dim);
+  t->indices = indices;
+  size_t at = 0;
+  std::function<Cluster*(Cluster*, int)> rec = [&](Cluster* father, int depth) -> Cluster* {
+    if (at >= nodes.size()) throw std::runtime_error("cluster tree: truncated serialized tree");
+    const ClusterNodeDesc d = nodes[at++];
+    if (d.offset < 0 || d.size < 1 || d.offset + d.size > n || d.nchild < 0)
+      throw std::runtime_error("cluster tree: bad node in serialized tree");
+    Cluster* c = new_cluster(*t, d.offset, d.size, depth, father);
+    for (int k = 0; k < d.nchild; ++k) c->children.push_back(rec(c, depth + 1));
+    t->max_leaf = std::max(t->max_leaf, c->is_leaf() ? c->size : 0);
+    return c;
+  };
+  t->root = rec(nullptr, 0);
+  int id = 0;
+  finish(*t, t->root, id);
+  t->indices_rev.assign(n, -1);
+  for (int i = 0; i < n; ++i) {
+    if (indices[i] < 0 || indices[i] >= n || t->indices_rev[indices[i]] >= 0)
+      throw std::runtime_error("cluster tree: the serialized indices are not a permutation");
+    t->indices_rev[indices[i]] = i;
+  }
+  return t;
+}
+
 namespace {
 
 struct Builder {
@@ -192,6 +227,53 @@ struct Builder {
 };
 
 }  // namespace
+
+std::unique_ptr<BlockTree> block_tree_from(const ClusterTree& rows, const ClusterTree& cols,
+                                           const std::vector<BlockNodeDesc>& nodes) {
+  auto bt = std::make_unique<BlockTree>();
+  bt->dim = rows.dim;
+  size_t at = 0;
+  std::function<Block*(const Cluster*, const Cluster*, int)> rec =
+      [&](const Cluster* r, const Cluster* c, int depth) -> Block* {
+    if (at >= nodes.size()) throw std::runtime_error("block tree: truncated serialized tree");
+    const BlockNodeDesc d = nodes[at++];
+    if (d.null) return nullptr;
+    bt->pool.emplace_back();
+    Block* b = &bt->pool.back();
+    b->rows = r;
+    b->cols = c;
+    b->depth = depth;
+    if (d.nchild == 0) {
+      if (d.rank == -1)
+        b->kind = Kind::Full;
+      else if (d.rank >= 0)
+        b->kind = Kind::Rk;
+      else
+        throw std::runtime_error("block tree: uninitialized leaf in serialized tree");
+      return b;
+    }
+    const bool same_r = d.bits & (1 << 4), same_c = d.bits & (1 << 5);
+    b->kind = Kind::Node;
+    b->nrow = same_r ? 1 : (int)r->children.size();
+    b->ncol = same_c ? 1 : (int)c->children.size();
+    if (b->nrow * b->ncol > 4 || b->nrow < 1 || b->ncol < 1 || d.nchild > b->nrow * b->ncol)
+      throw std::runtime_error("block tree: node shape does not match the cluster trees");
+    for (int k = 0; k < d.nchild; ++k) {
+      const int i = k % b->nrow, j = k / b->nrow;
+      b->child[k] = rec(same_r ? r : r->children[i], same_c ? c : c->children[j], depth + 1);
+    }
+    // lower-symmetric storage: a square diagonal node without its upper child
+    b->sym = r == c && b->nrow == b->ncol && b->nrow > 1 && !b->get(0, 1);
+    return b;
+  };
+  bt->root = rec(rows.root, cols.root, 0);
+  if (!bt->root) throw std::runtime_error("block tree: null root in serialized tree");
+  if (at != nodes.size()) throw std::runtime_error("block tree: trailing nodes in serialized tree");
+  Builder b{bt.get(), AdmissibilityParams{}, rows.dim};
+  int id = 0;
+  b.number(bt->root, id);
+  return bt;
+}
 
 std::unique_ptr<BlockTree> build_block_tree(const ClusterTree& rows, const ClusterTree& cols,
                                             const AdmissibilityParams& p, bool lower_sym) {

@@ -47,6 +47,16 @@ struct ClusterTree {
 std::unique_ptr<ClusterTree> build_cluster_tree(const double* coords, int dim, int n,
                                                 int max_leaf_size);
 
+// A cluster tree from its parts (as serialized, MatrixStructUnmarshaller::
+// readClusterTree serialization.cpp:134-160): coordinates in user numbering,
+// the hmat -> user permutation, and the nodes in preorder.
+struct ClusterNodeDesc {
+  int offset, size, nchild;
+};
+std::unique_ptr<ClusterTree> cluster_tree_from(const double* coords, int dim, int n,
+                                               const std::vector<int>& indices,
+                                               const std::vector<ClusterNodeDesc>& nodes);
+
 enum class Kind : int8_t { Full = 0, Rk = 1, Node = 2 };
 
 struct Block {
@@ -81,5 +91,18 @@ struct BlockTree {
 
 std::unique_ptr<BlockTree> build_block_tree(const ClusterTree& rows, const ClusterTree& cols,
                                             const AdmissibilityParams& p, bool lower_sym);
+
+// A block tree from a serialized H-matrix tree (MatrixStructUnmarshaller::
+// readTree serialization.cpp:206-232): nodes in preorder, null children
+// included.  rank >= 0: Rk leaf; -1: full leaf; -2: inner node.
+struct BlockNodeDesc {
+  bool null;
+  unsigned char bits;  // isUpper | isLower << 1 | isTriUpper << 2 | isTriLower << 3 |
+                       // keepSameRows << 4 | keepSameCols << 5
+  int rank;
+  int nchild;
+};
+std::unique_ptr<BlockTree> block_tree_from(const ClusterTree& rows, const ClusterTree& cols,
+                                           const std::vector<BlockNodeDesc>& nodes);
 
 }  // namespace hmx

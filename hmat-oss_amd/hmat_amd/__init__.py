@@ -28,6 +28,11 @@ KERNEL_EXPONENTIAL_DLENGTH, KERNEL_GAUSSIAN_DLENGTH = 3, 4
 _vp = C.c_void_p
 
 
+class hmat_cluster_info_t(C.Structure):
+    _fields_ = [("spatial_dimension", C.c_int), ("dimension", C.c_int),
+                ("nr_tree_nodes", C.c_size_t)]
+
+
 class hmat_info_t(C.Structure):
     _fields_ = [("full_zeros", C.c_int), ("full_size", C.c_size_t), ("rk_size", C.c_size_t),
                 ("full_count", C.c_size_t), ("rk_count", C.c_size_t),
@@ -94,6 +99,10 @@ _SCALAR_H = C.CFUNCTYPE(C.c_int, _vp, _vp)
 _AXPY = C.CFUNCTYPE(C.c_int, _vp, _vp, _vp)
 _DIAG = C.CFUNCTYPE(C.c_int, _vp, _vp, C.c_int)
 _SOLVE_TRI = C.CFUNCTYPE(C.c_int, _vp, C.c_int, _vp, C.c_int)
+IOSTREAM = C.CFUNCTYPE(None, _vp, C.c_size_t, _vp)   # hmat_iostream
+_READ_STRUCT = C.CFUNCTYPE(_vp, IOSTREAM, _vp)
+_WRITE_IO = C.CFUNCTYPE(None, _vp, IOSTREAM, _vp)
+_GET_TREES = C.CFUNCTYPE(C.c_int, _vp, C.POINTER(_vp), C.POINTER(_vp))
 
 # slot order of hmat_interface_t (hmat.h:717-1103); unused slots are plain pointers
 _SLOTS = [
@@ -109,11 +118,11 @@ _SLOTS = [
     ("finalize", _INIT), ("get_info", _INFO), ("get_profile", _vp), ("get_ratio", _vp),
     ("FPcompress", _vp), ("FPdecompress", _vp), ("GetFPCompressionSettings", _vp),
     ("SetFPCompressionSettings", _vp), ("SetFPCompressionSettingsParams", _vp),
-    ("dump_info", _vp), ("get_cluster_trees", _vp), ("set_cluster_trees", _vp),
+    ("dump_info", _vp), ("get_cluster_trees", _GET_TREES), ("set_cluster_trees", _vp),
     ("extract_diagonal", _DIAG), ("solve_lower_triangular", _SOLVE_TRI),
     ("solve_lower_triangular_dense", _SOLVE_TRI), ("get_block", _vp), ("get_values", _vp),
     ("walk", _vp), ("apply_on_leaf", _vp), ("value_type", C.c_int), ("internal", _vp),
-    ("read_struct", _vp), ("write_struct", _vp), ("read_data", _vp), ("write_data", _vp),
+    ("read_struct", _READ_STRUCT), ("write_struct", _WRITE_IO), ("read_data", _WRITE_IO), ("write_data", _WRITE_IO),
     ("set_progressbar", _vp), ("extract_diagonal_block", _vp),
 ]
 
@@ -132,6 +141,7 @@ _EXPORTS = {
     "hmat_delete_cluster_tree": (None, [_vp]),
     "hmat_cluster_get_indices": (C.POINTER(C.c_int), [_vp]),
     "hmat_tree_nodes_count": (C.c_int, [_vp]),
+    "hmat_cluster_get_info": (C.c_int, [_vp, _vp]),
     "hmat_create_admissibility_standard": (_vp, [C.c_double]),
     "hmat_delete_admissibility": (None, [_vp]),
     "hmat_create_compression_aca_plus": (_vp, [C.c_double]),
@@ -219,6 +229,36 @@ class ClusterTree:
         if getattr(self, "handle", None) and _lib is not None:
             _lib.hmat_delete_cluster_tree(self.handle)
             self.handle = None
+
+
+def _source(data, pos):
+    """An hmat_iostream reading `data` from pos[0]; past the end it yields zeros."""
+    data = bytes(data)
+
+    @IOSTREAM
+    def source(buf, n, ud):
+        if n:
+            chunk = data[pos[0]:pos[0] + n]
+            C.memmove(buf, chunk, len(chunk))
+            if len(chunk) < n:
+                C.memset(buf + len(chunk), 0, n - len(chunk))
+            pos[0] += n
+    return source
+
+
+class _BorrowedTree:
+    """A cluster tree owned by a matrix (read from a stream): size and permutation."""
+
+    def __init__(self, handle):
+        self.handle = handle
+        info = hmat_cluster_info_t()
+        load().hmat_cluster_get_info(handle, C.byref(info))
+        self.n, self.dim = int(info.dimension), int(info.spatial_dimension)
+
+    @property
+    def indices(self):
+        p = load().hmat_cluster_get_indices(self.handle)
+        return np.ctypeslib.as_array(p, shape=(self.n,)).copy()
 
 
 class HMatrix:
@@ -433,6 +473,58 @@ class HMatrix:
         fn = self.iface.solve_lower_triangular if permute else self.iface.solve_lower_triangular_dense
         _check(fn(self.handle, 1 if transpose else 0, x.ctypes.data, nrhs), "solve_lower_triangular")
         return x
+
+    # ---- serialization (hmat-oss stream format) ---------------------------------
+    def _write(self, slot):
+        chunks = []
+
+        @IOSTREAM
+        def sink(buf, n, ud):
+            if n:
+                chunks.append(C.string_at(buf, n))
+        getattr(self.iface, slot)(self.handle, sink, None)
+        err = load().hmx_last_error().decode()
+        if err:
+            raise EngineError(f"{slot} failed: {err}")
+        return b"".join(chunks)
+
+    def write_struct(self):
+        """The matrix structure (type, factorisation, cluster trees, block tree)."""
+        return self._write("write_struct")
+
+    def write_data(self):
+        """The leaves, in preorder."""
+        return self._write("write_data")
+
+    def read_data(self, data):
+        pos = [0]
+
+        source = _source(data, pos)
+        self.iface.read_data(self.handle, source, None)
+        err = load().hmx_last_error().decode()
+        if err:
+            raise EngineError(f"read_data failed: {err}")
+        return pos[0]
+
+    @staticmethod
+    def read_struct(data, value_type=HMAT_DOUBLE_PRECISION):
+        """A matrix (unassembled) from a serialized structure; its cluster trees
+        are those of the stream, owned by the matrix."""
+        iface = interface(value_type)
+        pos = [0]
+
+        source = _source(data, pos)
+        h = iface.read_struct(source, None)
+        if not h:
+            raise EngineError(f"read_struct failed: {load().hmx_last_error().decode()}")
+        r, c = _vp(), _vp()
+        iface.get_cluster_trees(h, C.byref(r), C.byref(c))
+        m = object.__new__(HMatrix)
+        m.tree = _BorrowedTree(r.value)
+        m.iface, m.value_type, m.handle, m._kernel = iface, value_type, h, None
+        m.lower_symmetric = bool(r.value == c.value)
+        m._consumed = pos[0]
+        return m
 
     def logdet(self):
         r = C.c_double()

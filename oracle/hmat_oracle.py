@@ -22,6 +22,7 @@ below (paths are relative to the hmat-oss source tree):
                               scalar_array.cpp:694-705, 852-874
 * logdet                     h_matrix.cpp:2925-2936
 * info counters              h_matrix.cpp:464-502
+* structure serialization    serialization.cpp:34-130 (MatrixStructMarshaller)
 
 Every function here is pinned by the property tests under ``tests/`` (dense
 reconstruction bounds, exact-rank recovery, LAPACK identities, partition
@@ -1492,3 +1493,71 @@ def cylinder_points(n, radius, height, seed):
     p[:, 1] = radius * np.sin(th)
     p[:, 2] = height * u[:, 1]
     return p
+
+
+# ---------------------------------------------------------------------------
+# structure serialization (MatrixStructMarshaller, serialization.cpp:34-130)
+# ---------------------------------------------------------------------------
+def write_struct(bt, row_root, col_root, idx_rows, idx_cols, points, eps, value_type=1,
+                 factorization=-1, lower_symmetric=True, ranks=None):
+    """Bytes of MatrixStructMarshaller::write for an assembled (or, with
+    factorization=2, LLt-factorised) block tree: scalar type, factorisation,
+    both cluster trees (serialization.cpp:34-53: size, dimension, coordinates in
+    user numbering, no group index, the permutation, then the nodes as
+    offset, size, number of children) and the block tree (writeTreeNode
+    serialization.cpp:74-103: flag byte isUpper | isLower << 1 | isTriUpper << 2
+    | isTriLower << 3 | keepSameRows << 4 | keepSameCols << 5, or 1 << 7 for a
+    null child; approximate rank 25 at leaves, -3 elsewhere (h_matrix.cpp:150,
+    165); rank, -1 full, -2 inner; low-rank epsilon; number of children).
+    `ranks`, if given, replaces the Rk leaves' ranks in preorder (a factor
+    computed elsewhere on the same partition)."""
+    import struct as st
+    out = bytearray()
+    rank_iter = iter(ranks) if ranks is not None else None
+    out += st.pack("<ii", value_type, factorization)
+
+    def tree(root, idx):
+        n, dim = points.shape
+        out.extend(st.pack("<ii", n, dim))
+        out.extend(np.ascontiguousarray(points, dtype="<f8").tobytes())
+        out.extend(st.pack("<i", 0))
+        out.extend(np.ascontiguousarray(idx, dtype="<i4").tobytes())
+
+        def rec(c):
+            out.extend(st.pack("<iii", c.offset, c.size, len(c.children)))
+            for k in c.children:
+                rec(k)
+        rec(root)
+    tree(row_root, idx_rows)
+    tree(col_root, idx_cols)
+    tri = factorization == 2
+
+    def node(b):
+        if b is None:
+            out.extend(st.pack("<B", 1 << 7))
+            return
+        leaf = b.is_leaf()
+        bits = 0
+        diag = lower_symmetric and b.rows is b.cols
+        if not leaf and b.sym and not tri:
+            bits |= 1 << 1
+        if diag and tri:
+            bits |= 1 << 3
+        if leaf or b.nrow == 1:
+            bits |= 1 << 4
+        if leaf or b.ncol == 1:
+            bits |= 1 << 5
+        if leaf and b.kind == RK:
+            rank = next(rank_iter) if rank_iter is not None else b.rk.rank
+        else:
+            rank = -1 if leaf else -2
+        kids = []
+        if not leaf:
+            last = max((k for k, ch in enumerate(b.children) if ch is not None), default=-1)
+            kids = b.children[:last + 1]
+        out.extend(st.pack("<Biidi", bits, 25 if leaf else -3, rank, eps, len(kids)))
+        for ch in kids:
+            node(ch)
+    node(bt)
+    return bytes(out)
+
