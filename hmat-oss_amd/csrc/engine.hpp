@@ -129,6 +129,7 @@ class Matrix {
   int factorization = -1;    // hmat_factorization_t
   bool factor32 = false;      // factor stored in s32 (mixed precision)
   DevBuf pivots;              // LU: getrf interchanges of each diagonal leaf, at its rows
+  DevBuf ldlt_diag;           // LDLt: D at the matrix's rows, in the factor's precision
   int overflow = 0;
   Stats stats;
 
@@ -158,7 +159,7 @@ class Matrix {
   // factorisation / solve (factor.cpp)
   void factorize(int algo, bool fp32);
   // part 0: the whole solve; 1: the lower-triangular sweep only (L x = b);
-  // 2: the upper one only (L^T x = b) -- LLt factors
+  // 2: the upper one only (L^T x = b) -- LLt and LDLt factors
   template <typename T>
   void solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part = 0);
   void solve_host(void* b, int nrhs, bool permute, int part = 0);

@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <stdexcept>
+#include <deque>
 #include <unordered_map>
 
 #include "engine.hpp"
@@ -39,6 +40,10 @@ double now_s() {
 }
 
 constexpr int64_t kTempBase = int64_t(1) << 40;
+// the LDLt diagonal (one vector at the matrix's rows), tracked per leaf cluster
+constexpr int64_t kDiagBuf = kTempBase + (int64_t(1) << 31);
+// leaf numbers from here on name scaled copies held by the executor (LDLt)
+constexpr int kShadowLeaf = 1 << 28;
 
 // Dense view; n may be dynamic (rank slot dn).
 struct DV {
@@ -273,6 +278,7 @@ class Exec {
 
   // ---- operands from the tree ---------------------------------------------
   DV full_of(const Block* b) const {
+    if (b->leaf >= kShadowLeaf) return shadow_.at(b->leaf - kShadowLeaf).d;
     const LeafDesc& L = st_.hdesc[b->leaf];
     DV d;
     d.p = st_.data() + L.a;
@@ -283,6 +289,7 @@ class Exec {
     return d;
   }
   RV rk_of(const Block* b) const {
+    if (b->leaf >= kShadowLeaf) return shadow_.at(b->leaf - kShadowLeaf).r;
     const LeafDesc& L = st_.hdesc[b->leaf];
     RV r;
     r.a = st_.data() + L.a;
@@ -1496,6 +1503,160 @@ class Exec {
     }
   }
 
+  // ---- H-LDLt (HMatrix::ldltDecomposition h_matrix.cpp:2747-2769 +
+  // recursiveLdltDecomposition recursion.cpp:37-83) ----------------------------------
+  // D lives in one vector at the matrix's rows (the reference keeps it per
+  // diagonal leaf, full_matrix.cpp:205-216); the leaves hold the unit lower L.
+  T* diag_ptr() const {
+    if (!M_.ldlt_diag.p) throw std::runtime_error("LDLt diagonal not allocated");
+    return M_.ldlt_diag.template as<T>();
+  }
+  DV diag_of(int row0, int len) const {
+    DV d;
+    d.p = diag_ptr() + row0;
+    d.ld = std::max(len, 1);
+    d.m = len;
+    d.n = 1;
+    d.buf = kDiagBuf;
+    d.row0 = row0;
+    return d;
+  }
+
+  void ldlt_leaf(const Block* b) {
+    DV a = full_of(b);
+    DV dg = diag_of(b->rows->offset, b->m());
+    FTask t{};
+    t.type = T_LDLT;
+    t.m = dim(a.m);
+    t.A = a.p;
+    t.lda = a.ld;
+    t.B = dg.p;
+    rd(a);
+    wr(a);
+    wr(dg);
+    emit(t);
+  }
+
+  // C = A diag(d) (cols) or diag(d) A, or with the inverse; d starts at the
+  // matrix row / column d0.  C may be A.
+  void dscale(const DV& C, const DV& A, bool cols, bool inverse, int d0, Dim m, Dim n) {
+    DV dg = diag_of(d0, cols ? n.s : m.s);
+    FTask t{};
+    t.type = T_DSCALE;
+    t.mode = (cols ? kDscaleCols : 0) | (inverse ? kDscaleInv : 0);
+    t.m = m;
+    t.n = n;
+    t.A = A.p;
+    t.lda = A.ld;
+    t.B = dg.p;
+    t.C = C.p;
+    t.ldc = C.ld;
+    rd(A);
+    rd(dg);
+    if (C.p == A.p) rd(C);
+    wr(C);
+    emit(t);
+  }
+
+  // HMatrix::multiplyWithDiag(D, Side::RIGHT, inverse) h_matrix.cpp:1953-1998:
+  // every leaf times the diagonal at its columns (an Rk leaf: the rows of B)
+  void scale_by_diag(const Block* b, bool inverse) {
+    if (!b) return;
+    if (!b->is_leaf()) {
+      for (Block* c : b->child)
+        if (c) scale_by_diag(c, inverse);
+      return;
+    }
+    if (b->kind == Kind::Full) {
+      DV f = full_of(b);
+      dscale(f, f, true, inverse, b->cols->offset, dim(f.m), dim(f.n));
+    } else {
+      RV r = rk_of(b);
+      DV B = dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0);
+      dscale(B, B, false, inverse, b->cols->offset, dim(r.n), dim(r.cap, r.rank));
+    }
+  }
+
+  // A copy of b times the diagonal at its columns, the operand M D of
+  // mdntProduct / mdmtProduct (h_matrix.cpp:2585-2690: copy, multiplyWithDiag,
+  // product): full leaves into temporaries, Rk leaves share A and the rank slot
+  // and get a scaled copy of B.  Readable through full_of / rk_of until
+  // release_twins().
+  const Block* scaled_twin(const Block* b) {
+    twins_.push_back(*b);
+    Block* t = &twins_.back();
+    if (!b->is_leaf()) {
+      for (int q = 0; q < 4; ++q)
+        if (b->child[q]) t->child[q] = const_cast<Block*>(scaled_twin(b->child[q]));
+      return t;
+    }
+    Shadow sh;
+    if (b->kind == Kind::Full) {
+      DV f = full_of(b);
+      sh.d = temp_dense(f.m, f.n, b->rows->offset);
+      dscale(sh.d, f, true, false, b->cols->offset, dim(f.m), dim(f.n));
+    } else {
+      RV r = rk_of(b);
+      sh.r = r;
+      sh.r.b = talloc((size_t)r.n * r.cap);
+      sh.r.lb = r.n;
+      sh.r.bufb = kTempBase + fs_.next_temp++;
+      ++live_;
+      DV src = dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0);
+      DV dst = dv_panel(sh.r.b, sh.r.lb, r.n, r.cap, sh.r.bufb, r.col0);
+      dscale(dst, src, false, false, b->cols->offset, dim(r.n), dim(r.cap, r.rank));
+    }
+    t->leaf = kShadowLeaf + (int)shadow_.size();
+    shadow_.push_back(sh);
+    ++twin_live_;
+    return t;
+  }
+  void release_twins() {
+    live_ -= twin_live_;
+    twin_live_ = 0;
+    shadow_.clear();
+    twins_.clear();
+  }
+
+  void ldlt(const Block* h) {
+    if (h->is_leaf()) {
+      if (h->kind != Kind::Full) throw std::runtime_error("ldlt: Rk diagonal block");
+      ldlt_leaf(h);
+      return;
+    }
+    if (h->nrow != h->ncol) throw std::runtime_error("ldlt: non-square block grid");
+    const int n = h->nrow;
+    for (int k = 0; k < n; ++k) {
+      ldlt(h->get(k, k));
+      // L_ik = H_ik L_kk^{-T} D_k^{-1}
+      for (int i = k + 1; i < n; ++i)
+        if (h->get(i, k)) {
+          solve_upper_right_h(h->get(k, k), h->get(i, k));
+          scale_by_diag(h->get(i, k), true);
+        }
+      maybe_flush();
+      // H_ij -= (L_ik D_k) L_jk^T for k < j <= i
+      for (int i = k + 1; i < n; ++i) {
+        if (!h->get(i, k)) continue;
+        const Block* w = scaled_twin(h->get(i, k));
+        for (int j = k + 1; j <= i; ++j)
+          if (h->get(i, j) && h->get(j, k)) hgemm(h->get(i, j), 'N', 'T', -1.0, w, h->get(j, k));
+        release_twins();
+        maybe_flush();
+      }
+    }
+  }
+
+  // HMatrix::solveDiagonal h_matrix.cpp:2969-2983 (LDLt): X <- D^{-1} X, one
+  // task per diagonal leaf
+  void solve_diag(const DV& X, Dim ncol) {
+    for (const Block* b : M_.bt->leaves) {
+      if (b->rows != b->cols) continue;
+      DV xs = rows_of(X, b->rows->offset, b->m());
+      dscale(xs, xs, false, true, b->rows->offset, dim(b->m()), ncol);
+    }
+  }
+
   // ---- whole-matrix updates -----------------------------------------------------------
   // RkMatrix::truncate (rk_matrix.cpp:490-585) of every Rk leaf at eps: a
   // formatted-add task whose target is recompressed alone (mode 1)
@@ -1657,6 +1818,8 @@ class Exec {
     int f[2] = {0, 0};
     HMX_CHECK(hipMemcpy(f, fs_.flags.p, sizeof(f), hipMemcpyDeviceToHost));
     if (f[1] && lu_mode_) throw std::runtime_error("luDecomposition: getrf found a zero pivot (singular matrix)");
+    if (f[1] && ldlt_mode_)
+      throw std::runtime_error("ldltDecomposition: zero pivot (no LDLt factorisation without pivoting)");
     if (f[1]) throw std::runtime_error("lltDecomposition: matrix is not positive definite");
     if (f[0]) throw std::runtime_error("an Rk rank exceeded the panel capacity during the factorisation");
   }
@@ -1664,6 +1827,7 @@ class Exec {
   int live() const { return live_; }
   void record_into(PlanList* p) { record_ = p; }
   void set_lu(bool v) { lu_mode_ = v; }
+  void set_ldlt(bool v) { ldlt_mode_ = v; }
   void report(double total) const {
 #if HMX_DIAGNOSTICS
     if (getenv("HMX_TRACE_LAUNCHES")) {
@@ -1693,8 +1857,15 @@ class Exec {
   std::vector<uint64_t> r_, w_;
   int max_wave_ = 0;
   PlanList* record_ = nullptr;
-  bool lu_mode_ = false;
+  bool lu_mode_ = false, ldlt_mode_ = false;
   double t_alloc_ = 0.0, t_seg_ = 0.0;
+  struct Shadow {
+    DV d;
+    RV r;
+  };
+  std::deque<Block> twins_;
+  std::vector<Shadow> shadow_;
+  int twin_live_ = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -1703,10 +1874,11 @@ class Exec {
 void Matrix::factorize(int algo, bool fp32) {
   if (!assembled) throw std::runtime_error("factorize: matrix not assembled");
   if (factorization != -1) throw std::runtime_error("factorize: matrix already factorised");
-  if (algo != hmat_factorization_llt && algo != hmat_factorization_lu)
-    throw std::runtime_error("factorize: only hmat_factorization_llt and hmat_factorization_lu are provided");
-  if (algo == hmat_factorization_llt && !lower_sym)
-    throw std::runtime_error("factorize: LLt needs lower-symmetric storage");
+  if (algo != hmat_factorization_llt && algo != hmat_factorization_lu && algo != hmat_factorization_ldlt)
+    throw std::runtime_error("factorize: only hmat_factorization_llt, hmat_factorization_ldlt and "
+                             "hmat_factorization_lu are provided");
+  if ((algo == hmat_factorization_llt || algo == hmat_factorization_ldlt) && !lower_sym)
+    throw std::runtime_error("factorize: LLt and LDLt need lower-symmetric storage");
   if (algo == hmat_factorization_lu && lower_sym)
     throw std::runtime_error("factorize: LU needs non-symmetric storage");
   if (algo == hmat_factorization_lu && rows->n != cols->n)
@@ -1727,9 +1899,13 @@ void Matrix::factorize(int algo, bool fp32) {
     }
     Exec<float> ex(*this, s32, *fstate_);
     ex.set_lu(algo == hmat_factorization_lu);
+    ex.set_ldlt(algo == hmat_factorization_ldlt);
     if (algo == hmat_factorization_lu) {
       pivots.alloc(sizeof(int) * (size_t)std::max(rows->n, 1));
       ex.lu(bt->root);
+    } else if (algo == hmat_factorization_ldlt) {
+      ldlt_diag.alloc(sizeof(float) * (size_t)std::max(rows->n, 1));
+      ex.ldlt(bt->root);
     } else {
       ex.llt(bt->root);
     }
@@ -1740,9 +1916,13 @@ void Matrix::factorize(int algo, bool fp32) {
   } else {
     Exec<double> ex(*this, s64, *fstate_);
     ex.set_lu(algo == hmat_factorization_lu);
+    ex.set_ldlt(algo == hmat_factorization_ldlt);
     if (algo == hmat_factorization_lu) {
       pivots.alloc(sizeof(int) * (size_t)std::max(rows->n, 1));
       ex.lu(bt->root);
+    } else if (algo == hmat_factorization_ldlt) {
+      ldlt_diag.alloc(sizeof(double) * (size_t)std::max(rows->n, 1));
+      ex.ldlt(bt->root);
     } else {
       ex.llt(bt->root);
     }
@@ -1772,8 +1952,8 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part)
   if (sweep_env && atoi(sweep_env))
     throw std::runtime_error("solve: the persistent-sweep solve is disabled (it did not complete "
                              "on the device; cause not found)");
-  if (part != 0 && factorization != hmat_factorization_llt)
-    throw std::runtime_error("solve_lower_triangular: provided for LLt factors only");
+  if (part != 0 && factorization == hmat_factorization_lu)
+    throw std::runtime_error("solve_lower_triangular: provided for LLt and LDLt factors only");
   const auto key = std::make_pair((int)sizeof(T) + 16 * part, nrhs);
   const size_t bytes = (size_t)n * nrhs * sizeof(T);
   auto it = fs.solve_plans.find(key);
@@ -1801,7 +1981,10 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part)
     } else {
       // HMatrix::solveLlt h_matrix.cpp:3010-3018 (part 1: L x = b only, part 2:
       // L^T x = b only, as solveLower(b, transpose) of the solve_lower_triangular slot)
+      // (LDLt, HMatrix::solveLdlt h_matrix.cpp:2990-3002: the unit L sweeps
+      // with X <- D^{-1} X between them)
       if (part != 2) ex.solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
+      if (part == 0 && factorization == hmat_factorization_ldlt) ex.solve_diag(X, Exec<T>::dim(nrhs));
       if (part != 1) ex.solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
     }
     ex.flush(false);
@@ -2031,8 +2214,9 @@ int Matrix::solve_refined(double* b, int nrhs, int ldb, double tol, int max_iter
     ++it;
   };
   try {
-    if (factorization == hmat_factorization_lu) {
-      // non-symmetric A: plain refinement sweeps x += M r, r = b - A x
+    if (factorization != hmat_factorization_llt) {
+      // LU (non-symmetric A) and LDLt (A possibly indefinite): plain
+      // refinement sweeps x += M r, r = b - A x
       while (it < max_iter) {
         precondition();
         launch_vec_add(x, z, cnt, stream);
@@ -2044,7 +2228,7 @@ int Matrix::solve_refined(double* b, int nrhs, int ldb, double tol, int max_iter
       }
     }
     bool fresh = true;  // p must restart from z
-    while (factorization != hmat_factorization_lu && it < max_iter) {
+    while (factorization == hmat_factorization_llt && it < max_iter) {
       precondition();
       const int nxt = cur ^ 1;
       launch_col_dot(r, z, n, nrhs, rz[nxt], stream);

@@ -24,6 +24,8 @@ enum TaskType : int {
   T_AXPY,         // C += alpha op(A) (mode 1: C = alpha op(A))
   T_GETRF,        // A <- P L U in place (L unit lower), row interchanges into B (int*)
   T_SUMK,         // C (+)= sum over p.s slices of A (ld lda, slice stride ldb): split-K GEMM
+  T_LDLT,         // A <- L D L^T in place (L unit lower), D into B (T*)
+  T_DSCALE,       // C = A scaled by the diagonal B (T*): mode kDscale* flags
   T_NTYPES
 };
 
@@ -36,6 +38,11 @@ constexpr int kTrsmBase = 3;
 constexpr int kTrsmTransA = 4;
 constexpr int kTrsmUnit = 8;
 constexpr int kTrsmPiv = 16;
+
+// T_DSCALE mode flags: scale columns (C_ij = A_ij d_j; rows otherwise,
+// C_ij = A_ij d_i), by the inverse of the diagonal.
+constexpr int kDscaleCols = 1;
+constexpr int kDscaleInv = 2;
 
 struct Dim {
   int s;             // static size / capacity

@@ -215,6 +215,86 @@ __global__ void __launch_bounds__(kBlock) k_potrf(const FTask* tasks, int* info)
 }
 
 // ---------------------------------------------------------------------------
+// LDLt of a diagonal leaf (ScalarArray::ldltDecomposition scalar_array.cpp:
+// 737-779): A = L D L^T in place, L unit lower with its diagonal set to 1 and
+// the upper triangle zeroed, D into d.  Left-looking, one column per step:
+// d_j = a_jj - sum_l l_jl^2 d_l, l_ij = (a_ij - sum_l l_il l_jl d_l) / d_j.
+// A zero or non-finite pivot sets info (the reference throws).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ void ldlt_body(T* L, int ld, int n, T* d, int* info) {
+  for (int j = 0; j < n; ++j) {
+    for (int i = j + threadIdx.x; i < n; i += kBlock) {
+      T s = L[i + (size_t)j * ld];
+      for (int l = 0; l < j; ++l) s -= L[i + (size_t)l * ld] * (L[j + (size_t)l * ld] * d[l]);
+      L[i + (size_t)j * ld] = s;
+    }
+    __syncthreads();
+    const T dj = L[j + (size_t)j * ld];
+    const T inv = (T)1 / dj;
+    for (int i = j + 1 + threadIdx.x; i < n; i += kBlock) L[i + (size_t)j * ld] *= inv;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (!(dj != (T)0) || !isfinite((double)dj)) atomicOr(info, 1);
+      d[j] = dj;
+      L[j + (size_t)j * ld] = (T)1;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_ldlt(const FTask* tasks, int* info) {
+  __shared__ T lds[kLdsDense * kLdsDense];
+  __shared__ T dl[kLdsDense];
+  const FTask& t = tasks[blockIdx.x];
+  const int n = t.m.s;
+  T* A = (T*)t.A;
+  T* D = (T*)t.B;
+  const int lda = t.lda;
+  if (n <= kLdsDense) {
+    stage_in(lds, n, A, 1L, (long)lda, n, n);
+    __syncthreads();
+    ldlt_body(lds, n, n, dl, info);
+    for (int e = threadIdx.x; e < n * n; e += kBlock) {
+      const int i = e % n, j = e / n;
+      A[i + (size_t)j * lda] = (i >= j) ? lds[i + j * n] : (T)0;
+    }
+    for (int i = threadIdx.x; i < n; i += kBlock) D[i] = dl[i];
+  } else {
+    ldlt_body(A, lda, n, D, info);
+    for (int e = threadIdx.x; e < n * n; e += kBlock) {
+      const int i = e % n, j = e / n;
+      if (i < j) A[i + (size_t)j * lda] = (T)0;
+    }
+  }
+}
+
+// C = A times a diagonal (ScalarArray::multiplyWithDiagOrDiagInv
+// scalar_array.cpp:1434-1470): the columns (kDscaleCols) or the rows of A
+// scaled by d or, with kDscaleInv, by 1 / d.  A may be C.  kDscaleSplit
+// workgroups share a task.
+constexpr int kDscaleSplit = 8;
+template <typename T>
+__global__ void k_dscale(const FTask* tasks, const int* fail) {
+  if (*fail) return;
+  const FTask& t = tasks[blockIdx.x / kDscaleSplit];
+  const int part = blockIdx.x % kDscaleSplit;
+  const T* A = (const T*)t.A;
+  const T* d = (const T*)t.B;
+  T* C = (T*)t.C;
+  const int m = eff(t.m), n = eff(t.n);
+  const bool cols = t.mode & kDscaleCols, inv = t.mode & kDscaleInv;
+  const int64_t total = (int64_t)m * n;
+  for (int64_t e = part * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)kDscaleSplit * blockDim.x) {
+    const int i = (int)(e % m), j = (int)(e / m);
+    const T f = d[cols ? j : i];
+    C[i + (size_t)j * t.ldc] = A[i + (size_t)j * t.lda] * (inv ? (T)1 / f : f);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // GETRF of a diagonal leaf (ScalarArray::luDecomposition, scalar_array.cpp:
 // 694-705): A = P L U in place, L unit lower, U upper, partial pivoting as
 // LAPACK getrf (pivot = first row of largest |a_ij| in the column), 0-based
@@ -1385,6 +1465,12 @@ void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile*
       break;
     case T_SUMK:
       hipLaunchKernelGGL(k_sumk<T>, dim3(count), dim3(256), 0, s, tasks, flags + 1);
+      break;
+    case T_LDLT:
+      hipLaunchKernelGGL(k_ldlt<T>, dim3(count), dim3(kBlock), 0, s, tasks, flags + 1);
+      break;
+    case T_DSCALE:
+      hipLaunchKernelGGL(k_dscale<T>, dim3(count * kDscaleSplit), dim3(256), 0, s, tasks, flags + 1);
       break;
     default:
       throw std::runtime_error("unknown task type");

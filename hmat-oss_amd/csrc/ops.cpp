@@ -65,6 +65,11 @@ Matrix* Matrix::clone(bool structure_only) {
         HMX_CHECK(hipMemcpyAsync(c->pivots.p, pivots.p, pivots.bytes, hipMemcpyDeviceToDevice,
                                  c->stream));
       }
+      if (ldlt_diag.p) {
+        c->ldlt_diag.alloc(ldlt_diag.bytes);
+        HMX_CHECK(hipMemcpyAsync(c->ldlt_diag.p, ldlt_diag.p, ldlt_diag.bytes,
+                                 hipMemcpyDeviceToDevice, c->stream));
+      }
     }
     HMX_CHECK(hipStreamSynchronize(c->stream));
     c->assembled = true;
@@ -135,6 +140,18 @@ void Matrix::add_identity(double alpha) {
 void Matrix::extract_diagonal(double* diag) {
   if (!assembled) throw std::runtime_error("extract_diagonal: matrix not assembled");
   if (rows != cols) throw std::runtime_error("extract_diagonal: the matrix is not square");
+  if (factorization == hmat_factorization_ldlt) {
+    // an LDLt factor gives D (HMatrix::extractDiagonal h_matrix.cpp:2904-2906)
+    const int n = n_rows();
+    if (factor32) {
+      std::vector<float> d(n);
+      HMX_CHECK(hipMemcpy(d.data(), ldlt_diag.p, n * sizeof(float), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) diag[i] = d[i];
+    } else {
+      HMX_CHECK(hipMemcpy(diag, ldlt_diag.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return;
+  }
   std::vector<int> list = leaves_where(*bt, true);
   DevBuf dl, dout;
   upload(dl, list.data(), list.size() * sizeof(int), stream);

@@ -135,7 +135,8 @@ struct Serializer {
     write_cluster_tree(o, *M.rows);
     write_cluster_tree(o, *M.cols);
     const std::vector<int> rk = ranks(M);
-    const bool tri = fact == hmat_factorization_llt;  // lltDecomposition flags, h_matrix.cpp:2562
+    // lltDecomposition / ldltDecomposition flags, h_matrix.cpp:2562, 2766
+    const bool tri = fact == hmat_factorization_llt || fact == hmat_factorization_ldlt;
     std::function<void(const Block*)> rec = [&](const Block* b) {
       if (!b) {
         o.put<unsigned char>(1 << 7);
@@ -171,6 +172,11 @@ struct Serializer {
       piv.resize(M.n_rows());
       HMX_CHECK(hipMemcpy(piv.data(), M.pivots.p, piv.size() * sizeof(int), hipMemcpyDeviceToHost));
     }
+    std::vector<T> dg;
+    if (written_factorization(M) == hmat_factorization_ldlt) {
+      dg.resize(M.n_rows());
+      HMX_CHECK(hipMemcpy(dg.data(), M.ldlt_diag.p, dg.size() * sizeof(T), hipMemcpyDeviceToHost));
+    }
     std::vector<T> buf;
     for (const Block* b : M.bt->leaves) {
       const LeafDesc& L = st.hdesc[b->leaf];
@@ -190,7 +196,8 @@ struct Serializer {
         continue;
       }
       const bool has_piv = !piv.empty() && b->rows == b->cols;
-      o.put<int>(has_piv ? 2 : 0);
+      const bool has_diag = !dg.empty() && b->rows == b->cols;
+      o.put<int>((has_piv ? 2 : 0) | (has_diag ? 4 : 0));
       buf.resize((size_t)L.m * L.n);
       HMX_CHECK(hipMemcpy(buf.data(), st.data() + L.a, buf.size() * sizeof(T), hipMemcpyDeviceToHost));
       o.bytes(buf.data(), buf.size() * sizeof(T));
@@ -199,6 +206,7 @@ struct Serializer {
         for (int i = 0; i < L.m; ++i) p[i] = piv[L.r0 + i] + 1;  // LAPACK getrf numbering
         o.bytes(p.data(), p.size() * sizeof(int));
       }
+      if (has_diag) o.bytes(dg.data() + L.r0, sizeof(T) * (size_t)L.m);
     }
   }
 
@@ -220,8 +228,9 @@ struct Serializer {
       throw std::runtime_error("read_struct: type mismatch (stream " + std::to_string(type) +
                                ", interface " + std::to_string(type_code(value_type)) + ")");
     const int fact = in.get<int>();
-    if (fact != -1 && fact != hmat_factorization_lu && fact != hmat_factorization_llt)
-      throw std::runtime_error("read_struct: only unfactorised, LU and LLt matrices are supported");
+    if (fact != -1 && fact != hmat_factorization_lu && fact != hmat_factorization_llt &&
+        fact != hmat_factorization_ldlt)
+      throw std::runtime_error("read_struct: only unfactorised, LU, LDLt and LLt matrices are supported");
     std::unique_ptr<ClusterTree> rt = read_cluster_tree(in);
     in.end_marker();
     std::unique_ptr<ClusterTree> ct = read_cluster_tree(in);
@@ -284,6 +293,8 @@ struct Serializer {
     std::vector<int> rk(st.hdesc.size());
     std::vector<int> piv;
     if (M.factorization == hmat_factorization_lu) piv.assign(M.n_rows(), 0);
+    std::vector<T> dg;
+    if (M.factorization == hmat_factorization_ldlt) dg.assign(M.n_rows(), T(0));
     for (const Block* b : M.bt->leaves) {
       const LeafDesc& L = st.hdesc[b->leaf];
       const int header = in.get<int>();
@@ -302,7 +313,8 @@ struct Serializer {
       }
       rk[b->leaf] = -1;
       if (header & 1) continue;  // null full block
-      if (header & 4) throw std::runtime_error("read_data: LDLt diagonals are not supported");
+      if ((header & 4) && (dg.empty() || b->rows != b->cols))
+        throw std::runtime_error("read_data: an LDLt diagonal outside an LDLt diagonal leaf");
       in.bytes(image.data() + L.a, sizeof(T) * (size_t)L.m * L.n);
       if (header & 2) {
         std::vector<int> p(L.m);
@@ -312,12 +324,17 @@ struct Serializer {
         if (!piv.empty())
           for (int i = 0; i < L.m; ++i) piv[L.r0 + i] = p[i] - 1;
       }
+      if (header & 4) in.bytes(dg.data() + L.r0, sizeof(T) * (size_t)L.m);
     }
     in.end_marker();
     M.init_store(st);
     HMX_CHECK(hipMemcpy(st.arena.p, image.data(), image.size() * sizeof(T), hipMemcpyHostToDevice));
     HMX_CHECK(hipMemcpy(st.ranks.p, rk.data(), rk.size() * sizeof(int), hipMemcpyHostToDevice));
     st.valid = true;
+    if (!dg.empty()) {
+      M.ldlt_diag.alloc(sizeof(T) * dg.size());
+      HMX_CHECK(hipMemcpy(M.ldlt_diag.p, dg.data(), dg.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
     if (!piv.empty()) {
       M.pivots.alloc(sizeof(int) * piv.size());
       HMX_CHECK(hipMemcpy(M.pivots.p, piv.data(), piv.size() * sizeof(int), hipMemcpyHostToDevice));

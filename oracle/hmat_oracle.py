@@ -18,6 +18,9 @@ below (paths are relative to the hmat-oss source tree):
 * H-gemm / axpy              h_matrix.cpp:870-1073, 1202-1620, 1874-1950
 * H-LLt and triangular solves recursion.cpp:86-134, 191-233, 349-444,
                               h_matrix.cpp:2297-2564, 3010-3018
+* H-LDLt                     recursion.cpp:37-83, h_matrix.cpp:1953-1998,
+                              2585-2690, 2747-2769, 2969-3002,
+                              scalar_array.cpp:737-779
 * H-LU                       recursion.cpp:238-287, h_matrix.cpp:2567, 2772-2779,
                               scalar_array.cpp:694-705, 852-874
 * logdet                     h_matrix.cpp:2925-2936
@@ -1241,6 +1244,92 @@ def llt(h, eps):
                         hgemm(h.get(i, j), "N", "T", -1.0, h.get(i, k), h.get(j, k), eps)
     h.tri_lower = True
     h.sym = False
+
+
+def ldlt_leaf(a):
+    """ScalarArray::ldltDecomposition scalar_array.cpp:737-779: A = L D L^T
+    without pivoting; returns the unit lower L (diagonal 1, upper triangle 0)
+    and D.  Raises on a zero pivot above the last row, as the reference does."""
+    n = a.shape[0]
+    L = np.array(a, dtype=np.float64, copy=True)
+    d = np.empty(n)
+    for j in range(n):
+        v = L[j, :j] * d[:j]
+        dj = L[j, j] - L[j, :j] @ v
+        d[j] = dj
+        if j + 1 < n:
+            if dj == 0.0:
+                raise ZeroDivisionError("ldltDecomposition: zero pivot")
+            L[j + 1:, j] = (L[j + 1:, j] - L[j + 1:, :j] @ v) / dj
+    return np.tril(L, -1) + np.eye(n), d
+
+
+def _copy_block(b):
+    """HMatrix::copy of a block subtree (data copied)."""
+    if b is None:
+        return None
+    c = Block(b.rows, b.cols, b.depth)
+    c.kind, c.nrow, c.ncol, c.sym = b.kind, b.nrow, b.ncol, b.sym
+    if b.is_leaf():
+        c.full = None if b.full is None else b.full.copy()
+        if b.rk is not None:
+            c.rk = Rk(None if b.rk.a is None else b.rk.a.copy(),
+                      None if b.rk.b is None else b.rk.b.copy())
+    else:
+        c.children = [_copy_block(ch) for ch in b.children]
+    return c
+
+
+def multiply_with_diag(b, d, inverse):
+    """HMatrix::multiplyWithDiag(D, Side::RIGHT, inverse) h_matrix.cpp:1953-1998:
+    each leaf's columns times d (the diagonal in hmat numbering) or 1 / d; an
+    Rk leaf scales the rows of B."""
+    for lf in leaves(b):
+        s = d[lf.cols.offset:lf.cols.offset + lf.n]
+        if inverse:
+            s = 1.0 / s
+        if lf.kind == FULL:
+            if lf.full is not None:
+                lf.full = lf.full * s[None, :]
+        elif lf.rk is not None and lf.rk.rank > 0:
+            lf.rk = Rk(lf.rk.a, lf.rk.b * s[:, None])
+
+
+def ldlt(h, eps, d):
+    """HMatrix::ldltDecomposition h_matrix.cpp:2747-2769 +
+    recursiveLdltDecomposition recursion.cpp:37-83.  The leaves get the unit
+    lower L; D goes into d (hmat numbering).  L_ik = H_ik L_kk^-T D_k^-1, then
+    H_ij -= (L_ik D_k) L_jk^T for k < j <= i (mdntProduct / mdmtProduct
+    h_matrix.cpp:2585-2690: a scaled copy of L_ik times L_jk^T)."""
+    if h.is_leaf():
+        h.full, dd = ldlt_leaf(h.full)
+        d[h.rows.offset:h.rows.offset + h.m] = dd
+    else:
+        n = h.nrow
+        for k in range(n):
+            ldlt(h.get(k, k), eps, d)
+            for i in range(k + 1, n):
+                if h.get(i, k) is not None:
+                    solve_upper_right_h(h.get(k, k), h.get(i, k), eps)
+                    multiply_with_diag(h.get(i, k), d, True)
+            for i in range(k + 1, n):
+                if h.get(i, k) is None:
+                    continue
+                w = _copy_block(h.get(i, k))
+                multiply_with_diag(w, d, False)
+                for j in range(k + 1, i + 1):
+                    if h.get(i, j) is not None and h.get(j, k) is not None:
+                        hgemm(h.get(i, j), "N", "T", -1.0, w, h.get(j, k), eps)
+    h.tri_lower = True
+    h.sym = False
+
+
+def solve_ldlt(L, d, b):
+    """HMatrix::solveLdlt h_matrix.cpp:2990-3002 on a (N, nrhs) array in hmat
+    order: L y = b, y <- D^-1 y (solveDiagonal :2969-2983), L^T x = y."""
+    solve_lower_left_dense(L, b)
+    b *= (1.0 / d)[:, None]
+    solve_upper_left_dense(L, b)
 
 
 def solve_llt(L, b):

@@ -226,3 +226,77 @@ def test_hlu_upper_solves_against_dense():
     w = z.copy()
     ho.lu_solve_upper_right_dense(bt, w)
     assert np.allclose(w @ U, z, rtol=0, atol=1e-9 * np.abs(z).max())
+
+
+def test_ldlt_leaf_identities():
+    rng = np.random.default_rng(17)
+    n = 40
+    # SPD: L D L^T = A, L unit lower, and the Cholesky identities
+    # D = diag(C)^2, L = C diag(C)^-1 for A = C C^T
+    g = rng.standard_normal((n, n))
+    a = g @ g.T + n * np.eye(n)
+    L, d = ho.ldlt_leaf(a)
+    assert np.allclose(np.diag(L), 1.0) and np.allclose(np.triu(L, 1), 0.0)
+    assert np.linalg.norm(L @ np.diag(d) @ L.T - a) <= 1e-13 * np.linalg.norm(a)
+    c = np.linalg.cholesky(a)
+    assert np.allclose(d, np.diag(c) ** 2, rtol=1e-12)
+    assert np.allclose(L, c / np.diag(c)[None, :], rtol=1e-11, atol=1e-13)
+    # symmetric indefinite (leading minors nonzero): reconstruction, and the
+    # signs of D follow Sylvester's law of inertia
+    s = a - 1.3 * n * np.eye(n)
+    L, d = ho.ldlt_leaf(s)
+    assert np.linalg.norm(L @ np.diag(d) @ L.T - s) <= 1e-11 * np.linalg.norm(s)
+    ev = np.linalg.eigvalsh(s)
+    assert (d < 0).sum() == (ev < 0).sum() and 0 < (ev < 0).sum() < n
+    with pytest.raises(ZeroDivisionError):
+        ho.ldlt_leaf(np.array([[0.0, 1.0], [1.0, 2.0]]))
+
+
+def _hmatrix(n, leaf, nugget, seed=5, height=1.5):
+    pts = ho.cylinder_points(n, 1.0, height, seed)
+    root, idx = ho.build_cluster_tree(pts, leaf)
+    bt = ho.build_block_tree(root, root, 2.0, True)
+    ho.assemble(bt, idx, ho.ExpKernel(pts, 0.5, nugget))
+    return bt, ho.to_dense(bt, n, n)
+
+
+def test_hldlt_against_llt_and_dense():
+    n, leaf, eps = 1500, 64, 1e-4
+    bt, H = _hmatrix(n, leaf, 1e-2)
+    d = np.zeros(n)
+    ho.ldlt(bt, eps, d)
+    L = ho.factor_dense(bt, n)
+    assert np.allclose(np.diag(L), 1.0)
+    assert np.linalg.norm(L @ np.diag(d) @ L.T - H) <= 5e-5 * np.linalg.norm(H)
+    b = np.random.default_rng(1).standard_normal((n, 2))
+    x = b.copy()
+    ho.solve_ldlt(bt, d, x)
+    # the H-LLt factor of the same matrix: D = diag(C)^2 and the same solution
+    # (measured 2e-14: the recompressions see the same blocks up to a diagonal
+    # scaling, so they keep the same ranks)
+    bc, _ = _hmatrix(n, leaf, 1e-2)
+    ho.llt(bc, eps)
+    C = ho.factor_dense(bc, n)
+    xc = b.copy()
+    ho.solve_llt(bc, xc)
+    assert np.abs(d / np.diag(C) ** 2 - 1).max() <= 1e-12
+    assert np.linalg.norm(x - xc) <= 1e-12 * np.linalg.norm(xc)
+    assert np.linalg.norm(H @ x - b) <= 2e-3 * np.linalg.norm(b)
+
+
+def test_hldlt_indefinite_inertia_and_residual():
+    # exp kernel minus 87.44 I: three positive eigenvalues (the shift sits in
+    # the gap between the third and fourth largest), condition number ~4
+    n, leaf, eps = 1500, 64, 1e-4
+    bt, H = _hmatrix(n, leaf, -87.44226910401406)
+    ev = np.linalg.eigvalsh(H)
+    assert (ev > 0).sum() == 3
+    d = np.zeros(n)
+    ho.ldlt(bt, eps, d)
+    L = ho.factor_dense(bt, n)
+    assert (d > 0).sum() == 3          # Sylvester's law of inertia
+    assert np.linalg.norm(L @ np.diag(d) @ L.T - H) <= 2e-4 * np.linalg.norm(H)
+    b = np.random.default_rng(2).standard_normal((n, 3))
+    x = b.copy()
+    ho.solve_ldlt(bt, d, x)
+    assert np.linalg.norm(H @ x - b) <= 2e-4 * np.linalg.norm(b)
