@@ -154,12 +154,17 @@ int solve_dense(hmat_matrix_t* h, void* b, int nrhs) {
 }
 
 // solve_generic c_wrapping.hpp:500-540: lower / upper select one triangular
-// sweep of an LLt factor (solveLower(b, transpose))
+// sweep of an LLt factor (solveLower(b, transpose)); nr_rhs == 0 means that
+// values is an H-matrix
 int solve_generic(hmat_matrix_t* h, const struct hmat_solve_context_t* ctx) {
   return guard([&] {
-    if (ctx->nr_rhs == 0) throw std::runtime_error("solve_generic: H-matrix right-hand sides are not supported");
     if (ctx->lower && ctx->upper)
       throw std::runtime_error("lower and upper cannot both be true in hmat_solve_context_t");
+    if (ctx->nr_rhs == 0) {
+      M(h)->solve_mat(*M(reinterpret_cast<hmat_matrix_t*>(ctx->values)),
+                      ctx->lower ? 1 : (ctx->upper ? 2 : 0));
+      return;
+    }
     M(h)->solve_host(ctx->values, ctx->nr_rhs, ctx->no_permutation == 0,
                      ctx->lower ? 1 : (ctx->upper ? 2 : 0));
   });
@@ -207,6 +212,16 @@ int truncate_matrix(hmat_matrix_t* h) {
 template <typename T>
 int axpy_matrix(void* a, hmat_matrix_t* x, hmat_matrix_t* y) {
   return guard([&] { M(y)->axpy_from((double)*(const T*)a, *M(x)); });
+}
+
+// gemm c_wrapping.hpp:202-215 (HMatrix::gemm h_matrix.cpp:1570-1620):
+// C <- beta C + alpha op(A) op(B) on the device
+template <typename T>
+int gemm_h(char ta, char tb, const void* alpha, hmat_matrix_t* a, hmat_matrix_t* b,
+           const void* beta, hmat_matrix_t* c) {
+  return guard([&] {
+    M(c)->gemm_from(ta, tb, (double)*(const T*)alpha, *M(a), *M(b), (double)*(const T*)beta);
+  });
 }
 
 // extract_diagonal c_wrapping.hpp:760-773: the diagonal, in user numbering
@@ -311,8 +326,10 @@ int unsupported(const char* what) {
 int u_inverse(hmat_matrix_t*) { return unsupported("inverse"); }
 hmat_matrix_t* u_get_child(hmat_matrix_t*, int, int) { unsupported("get_child"); return nullptr; }
 int u_h(hmat_matrix_t*) { return unsupported("operation"); }
-int u_solve_mat(hmat_matrix_t*, hmat_matrix_t*) { return unsupported("solve_mat"); }
-int u_gemm(char, char, const void*, hmat_matrix_t*, hmat_matrix_t*, const void*, hmat_matrix_t*) { return unsupported("gemm"); }
+// solve_mat c_wrapping.hpp:547-557 (HMatrix::solve(HMatrix*) h_matrix.cpp:2937-2966)
+int solve_mat_h(hmat_matrix_t* h, hmat_matrix_t* b) {
+  return guard([&] { M(h)->solve_mat(*M(b)); });
+}
 int u_trsm(char, char, char, char, int, int, void*, hmat_matrix_t*, int, void*) { return unsupported("trsm"); }
 int u_full_gemm(char, char, int, int, void*, void*, void*, hmat_matrix_t*, void*) { return unsupported("full_gemm"); }
 int u_profile(hmat_matrix_t*, hmat_profile_t*) { return unsupported("get_profile"); }
@@ -361,7 +378,7 @@ void fill(hmat_interface_t* i) {
   i->get_child = u_get_child;
   i->destroy_child = u_h;
   i->solve_generic = solve_generic;
-  i->solve_mat = u_solve_mat;
+  i->solve_mat = solve_mat_h;
   i->solve_systems = solve_systems;
   i->solve_dense = solve_dense;
   i->transpose = u_h;
@@ -369,7 +386,7 @@ void fill(hmat_interface_t* i) {
   i->truncate = truncate_matrix;
   i->vector_reorder = vector_order<T, false>;
   i->vector_restore = vector_order<T, true>;
-  i->gemm = u_gemm;
+  i->gemm = gemm_h<T>;
   i->axpy = axpy_matrix<T>;
   i->trsm = u_trsm;
   i->gemv = gemv;

@@ -177,6 +177,11 @@ class Matrix {
   void extract_diagonal(double* diag);      // diagonal in hmat numbering (n_rows doubles)
   void truncate_all();                      // recompress every Rk leaf at lowrank_eps
   void axpy_from(double alpha, Matrix& x);  // A <- A + alpha X, X of the same partition
+  // this <- beta this + alpha op(a) op(b) (H-gemm), a and b over the same trees
+  void gemm_from(char ta, char tb, double alpha, Matrix& a, Matrix& b, double beta);
+  // b <- this^{-1} b for a factorised matrix and an H-matrix b (solve_mat);
+  // part 1 / 2: L^{-1} b / L^{-T} b only (LLt)
+  void solve_mat(Matrix& b, int part = 0);
 
   int n_rows() const { return rows->n; }
   int n_cols() const { return cols->n; }

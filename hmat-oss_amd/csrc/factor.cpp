@@ -1558,18 +1558,23 @@ class Exec {
     emit(t);
   }
 
-  // HMatrix::multiplyWithDiag(D, Side::RIGHT, inverse) h_matrix.cpp:1953-1998:
-  // every leaf times the diagonal at its columns (an Rk leaf: the rows of B)
-  void scale_by_diag(const Block* b, bool inverse) {
+  // HMatrix::multiplyWithDiag(D, side, inverse) h_matrix.cpp:1953-1998: every
+  // leaf times the diagonal at its columns (Side::RIGHT; an Rk leaf: the rows
+  // of B) or at its rows (left; an Rk leaf: the rows of A)
+  void scale_by_diag(const Block* b, bool inverse, bool left = false) {
     if (!b) return;
     if (!b->is_leaf()) {
       for (Block* c : b->child)
-        if (c) scale_by_diag(c, inverse);
+        if (c) scale_by_diag(c, inverse, left);
       return;
     }
     if (b->kind == Kind::Full) {
       DV f = full_of(b);
-      dscale(f, f, true, inverse, b->cols->offset, dim(f.m), dim(f.n));
+      dscale(f, f, !left, inverse, left ? b->rows->offset : b->cols->offset, dim(f.m), dim(f.n));
+    } else if (left) {
+      RV r = rk_of(b);
+      DV A = dv_panel(r.a, r.la, r.m, r.cap, r.bufa, r.row0);
+      dscale(A, A, false, inverse, b->rows->offset, dim(r.m), dim(r.cap, r.rank));
     } else {
       RV r = rk_of(b);
       DV B = dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0);
@@ -1644,6 +1649,138 @@ class Exec {
         release_twins();
         maybe_flush();
       }
+    }
+  }
+
+  // A view of another matrix's leaves (the same cluster trees): a twin of its
+  // block tree whose leaves alias its store under buffer ids base + leaf.
+  const Block* foreign_view(const Block* b, Store<T>& xs, int64_t base) {
+    twins_.push_back(*b);
+    Block* t = &twins_.back();
+    if (!b->is_leaf()) {
+      for (int q = 0; q < 4; ++q)
+        if (b->child[q]) t->child[q] = const_cast<Block*>(foreign_view(b->child[q], xs, base));
+      return t;
+    }
+    const LeafDesc& L = xs.hdesc[b->leaf];
+    Shadow sh;
+    sh.d.p = xs.data() + L.a;
+    sh.d.ld = L.m;
+    sh.d.m = L.m;
+    sh.d.n = L.n;
+    sh.d.buf = base + b->leaf;
+    sh.r.a = xs.data() + L.a;
+    sh.r.b = xs.data() + L.b;
+    sh.r.la = L.m;
+    sh.r.lb = L.n;
+    sh.r.m = L.m;
+    sh.r.n = L.n;
+    sh.r.cap = L.cap;
+    sh.r.rank = xs.ranks.template as<int>() + b->leaf;
+    sh.r.bufa = sh.r.bufb = base + b->leaf;
+    sh.r.row0 = b->rows->offset;
+    sh.r.col0 = b->cols->offset;
+    t->leaf = kShadowLeaf + (int)shadow_.size();
+    shadow_.push_back(sh);
+    return t;
+  }
+
+  // HMatrix::gemm h_matrix.cpp:1570-1620 (after the scaling by beta): this
+  // matrix += alpha op(A) op(B), A and B other matrices over the same trees
+  void gemm_matrices(char ta, char tb, double alpha, Matrix& A, Store<T>& as, Matrix& B,
+                     Store<T>& bs) {
+    constexpr int64_t kABase = int64_t(1) << 36, kBBase = kABase + (int64_t(1) << 30);
+    const Block* va = foreign_view(A.bt->root, as, kABase);
+    const Block* vb = foreign_view(B.bt->root, bs, kBBase);
+    hgemm(M_.bt->root, ta, tb, alpha, va, vb);
+    maybe_flush();
+  }
+
+  // Left triangular solves with an H-matrix right-hand side b (in place):
+  // HMatrix::solveLowerTriangularLeft(HMatrix) h_matrix.cpp:2297-2335 +
+  // recursion.cpp:191-233 and solveUpperTriangularLeft(HMatrix) :2463-2500 +
+  // recursion.cpp:393-444.  kind 0: L X = B (LLt / LDLt factor), 1: L^T X = B,
+  // 2: L X = B (LU, unit L with the leaves' interchanges), 3: U X = B (LU).
+  void dense_left_solve(int kind, const Block* F, const DV& X, Dim ncol) {
+    switch (kind) {
+      case 0: solve_lower_left(F, X, ncol); break;
+      case 1: solve_upper_left(F, X, ncol); break;
+      case 2: lu_solve_lower_left(F, X, ncol); break;
+      default: lu_solve_upper_left(F, X, ncol); break;
+    }
+  }
+  void solve_left_h(int kind, const Block* F, const Block* b) {
+    if (!b) return;
+    const bool lower = kind == 0 || kind == 2;
+    if (!F->is_leaf() && !b->is_leaf()) {
+      if (F->ncol == b->nrow) {
+        for (int k = 0; k < b->ncol; ++k) {
+          if (lower) {
+            for (int i = 0; i < F->nrow; ++i) {
+              const Block* bik = b->get(i, k);
+              if (!bik) continue;
+              for (int j = 0; j < i; ++j)
+                if (F->get(i, j) && b->get(j, k)) hgemm(bik, 'N', 'N', -1.0, F->get(i, j), b->get(j, k));
+              solve_left_h(kind, F->get(i, i), bik);
+            }
+          } else {
+            for (int i = F->nrow - 1; i >= 0; --i) {
+              const Block* bik = b->get(i, k);
+              if (!bik) continue;
+              solve_left_h(kind, F->get(i, i), bik);
+              for (int j = 0; j < i; ++j) {
+                const Block* bjk = b->get(j, k);
+                if (!bjk) continue;
+                if (kind == 1 && F->get(i, j)) hgemm(bjk, 'T', 'N', -1.0, F->get(i, j), bik);
+                if (kind == 3 && F->get(j, i)) hgemm(bjk, 'N', 'N', -1.0, F->get(j, i), bik);
+              }
+            }
+          }
+          maybe_flush();
+        }
+      } else if (F->ncol > 1 && b->nrow == 1 && b->ncol > 1) {
+        for (int k = 0; k < b->ncol; ++k) solve_left_h(kind, F, b->get(0, k));
+      } else {
+        throw std::runtime_error("solve_mat: triangular solve case not handled");
+      }
+      return;
+    }
+    if (!b->is_leaf()) {
+      if (b->nrow == 1) {
+        for (int k = 0; k < b->ncol; ++k) solve_left_h(kind, F, b->get(0, k));
+        return;
+      }
+      DV D = eval_block(b);
+      dense_left_solve(kind, F, D, dim(D.n));
+      clear_and_set(b, D);
+      release();
+      return;
+    }
+    if (b->kind == Kind::Full) {
+      dense_left_solve(kind, F, full_of(b), dim(b->n()));
+    } else {
+      RV r = rk_of(b);
+      dense_left_solve(kind, F, dv_panel(r.a, r.la, r.m, r.cap, r.bufa, r.row0), dim(r.cap, r.rank));
+    }
+  }
+
+  // HMatrix::solve(HMatrix* b, algo) h_matrix.cpp:2937-2966: B <- A^{-1} B for
+  // another matrix B whose rows use this matrix's column tree
+  // (part 1: L X = B only, part 2: L^T X = B only, as DefaultEngine::solveLower
+  // default_engine.cpp:243-253 for LLt factors)
+  void solve_matrix(Matrix& B, Store<T>& bs, int part) {
+    constexpr int64_t kBBase = (int64_t(1) << 36) + (int64_t(1) << 30);
+    const Block* vb = foreign_view(B.bt->root, bs, kBBase);
+    const Block* F = M_.bt->root;
+    if (part != 0) {
+      solve_left_h(part == 1 ? 0 : 1, F, vb);
+    } else if (M_.factorization == hmat_factorization_lu) {
+      solve_left_h(2, F, vb);
+      solve_left_h(3, F, vb);
+    } else {
+      solve_left_h(0, F, vb);
+      if (M_.factorization == hmat_factorization_ldlt) scale_by_diag(vb, true, true);
+      solve_left_h(1, F, vb);
     }
   }
 
@@ -2131,6 +2268,80 @@ void Matrix::axpy_from(double alpha, Matrix& x) {
     ex.check_flags();
   } else {
     throw std::runtime_error("axpy: no data in a common precision");
+  }
+}
+
+void Matrix::gemm_from(char ta, char tb, double alpha, Matrix& a, Matrix& b, double beta) {
+  auto norm_t = [](char t) {
+    if (t == 'N' || t == 'n') return 'N';
+    if (t == 'T' || t == 't' || t == 'C' || t == 'c') return 'T';
+    throw std::runtime_error(std::string("gemm: bad transpose flag ") + t);
+  };
+  ta = norm_t(ta);
+  tb = norm_t(tb);
+  if (&a == this || &b == this) throw std::runtime_error("gemm: C must differ from A and B");
+  if (!assembled || !a.assembled || !b.assembled) throw std::runtime_error("gemm: matrix not assembled");
+  if (factorization != -1 || a.factorization != -1 || b.factorization != -1)
+    throw std::runtime_error("gemm: factorised matrices are not supported");
+  if (a.value_type != value_type || b.value_type != value_type)
+    throw std::runtime_error("gemm: the matrices must have the same scalar type");
+  const ClusterTree* ar = ta == 'N' ? a.rows : a.cols;
+  const ClusterTree* ac = ta == 'N' ? a.cols : a.rows;
+  const ClusterTree* br = tb == 'N' ? b.rows : b.cols;
+  const ClusterTree* bc = tb == 'N' ? b.cols : b.rows;
+  if (ar != rows || bc != cols || ac != br)
+    throw std::runtime_error("gemm: the matrices must share their cluster trees (op(A) rows = C rows, "
+                             "op(B) cols = C cols, op(A) cols = op(B) rows)");
+  if (beta != 1.0) scale(beta);
+  device_init();
+  a.device_init();
+  b.device_init();
+  HMX_CHECK(hipStreamSynchronize(a.stream));
+  HMX_CHECK(hipStreamSynchronize(b.stream));
+  if (!fstate_) fstate_ = std::make_shared<FactorState>();
+  if (s64.valid && a.s64.valid && b.s64.valid) {
+    Exec<double> ex(*this, s64, *fstate_);
+    ex.gemm_matrices(ta, tb, alpha, a, a.s64, b, b.s64);
+    ex.flush(false);
+    ex.check_flags();
+  } else if (s32.valid && a.s32.valid && b.s32.valid) {
+    Exec<float> ex(*this, s32, *fstate_);
+    ex.gemm_matrices(ta, tb, alpha, a, a.s32, b, b.s32);
+    ex.flush(false);
+    ex.check_flags();
+  } else {
+    throw std::runtime_error("gemm: no data in a common precision");
+  }
+}
+
+void Matrix::solve_mat(Matrix& b, int part) {
+  if (factorization == -1) throw std::runtime_error("solve_mat: matrix not factorised");
+  if (part != 0 && factorization != hmat_factorization_llt)
+    throw std::runtime_error("solveLower only supported by LLt factorization.");
+  if (&b == this) throw std::runtime_error("solve_mat: B must differ from the factorised matrix");
+  if (!b.assembled) throw std::runtime_error("solve_mat: B not assembled");
+  if (b.factorization != -1) throw std::runtime_error("solve_mat: B has been factorised");
+  if (b.lower_sym) throw std::runtime_error("solve_mat: B must be stored non-symmetric");
+  if (b.rows != cols) throw std::runtime_error("solve_mat: B's rows must use the matrix's column tree");
+  if (b.value_type != value_type) throw std::runtime_error("solve_mat: B has another scalar type");
+  if (factor32 && value_type == HMAT_DOUBLE_PRECISION)
+    throw std::runtime_error("solve_mat: mixed-precision factors are not supported");
+  device_init();
+  b.device_init();
+  HMX_CHECK(hipStreamSynchronize(b.stream));
+  if (!fstate_) fstate_ = std::make_shared<FactorState>();
+  if (factor32) {
+    if (!b.s32.valid) throw std::runtime_error("solve_mat: B has no single-precision data");
+    Exec<float> ex(*this, s32, *fstate_);
+    ex.solve_matrix(b, b.s32, part);
+    ex.flush(false);
+    ex.check_flags();
+  } else {
+    if (!b.s64.valid) throw std::runtime_error("solve_mat: B has no double-precision data");
+    Exec<double> ex(*this, s64, *fstate_);
+    ex.solve_matrix(b, b.s64, part);
+    ex.flush(false);
+    ex.check_flags();
   }
 }
 

@@ -97,6 +97,8 @@ _COPY = C.CFUNCTYPE(_vp, _vp)
 _NORM = C.CFUNCTYPE(C.c_double, _vp)
 _SCALAR_H = C.CFUNCTYPE(C.c_int, _vp, _vp)
 _AXPY = C.CFUNCTYPE(C.c_int, _vp, _vp, _vp)
+_GEMM_H = C.CFUNCTYPE(C.c_int, C.c_char, C.c_char, _vp, _vp, _vp, _vp, _vp)
+_SOLVE_MAT = C.CFUNCTYPE(C.c_int, _vp, _vp)
 _DIAG = C.CFUNCTYPE(C.c_int, _vp, _vp, C.c_int)
 _SOLVE_TRI = C.CFUNCTYPE(C.c_int, _vp, C.c_int, _vp, C.c_int)
 IOSTREAM = C.CFUNCTYPE(None, _vp, C.c_size_t, _vp)   # hmat_iostream
@@ -110,10 +112,10 @@ _SLOTS = [
     ("set_low_rank_epsilon", _VOID_EPS), ("assemble_generic", _ASSEMBLE), ("copy", _COPY),
     ("copy_struct", _COPY), ("norm", _NORM), ("logdet", _LOGDET), ("inverse", _vp),
     ("factorize_generic", _FACTOR), ("destroy", _P_INT), ("get_child", _vp),
-    ("destroy_child", _vp), ("solve_generic", _SOLVE_GEN), ("solve_mat", _vp),
+    ("destroy_child", _vp), ("solve_generic", _SOLVE_GEN), ("solve_mat", _SOLVE_MAT),
     ("solve_systems", _SOLVE), ("solve_dense", _SOLVE), ("transpose", _vp), ("scale", _SCALAR_H),
     ("truncate", _P_INT), ("vector_reorder", _REORDER), ("vector_restore", _REORDER),
-    ("gemm", _vp), ("axpy", _AXPY), ("trsm", _vp), ("gemv", _GEMV), ("gemm_scalar", _GEMV),
+    ("gemm", _GEMM_H), ("axpy", _AXPY), ("trsm", _vp), ("gemv", _GEMV), ("gemm_scalar", _GEMV),
     ("full_gemm", _vp), ("gemm_dense", _GEMM_DENSE), ("add_identity", _SCALAR_H), ("init", _INIT),
     ("finalize", _INIT), ("get_info", _INFO), ("get_profile", _vp), ("get_ratio", _vp),
     ("FPcompress", _vp), ("FPdecompress", _vp), ("GetFPCompressionSettings", _vp),
@@ -460,6 +462,25 @@ class HMatrix:
         """self <- self + alpha x (x of the same partition)."""
         a = self._scalar(alpha)
         _check(self.iface.axpy(C.cast(C.byref(a), _vp), x.handle, self.handle), "axpy")
+
+    def gemm(self, ta, tb, alpha, a, b, beta):
+        """self <- beta self + alpha op(a) op(b), H-matrices over the same trees."""
+        al, be = self._scalar(alpha), self._scalar(beta)
+        _check(self.iface.gemm(ta.encode() if isinstance(ta, str) else ta,
+                               tb.encode() if isinstance(tb, str) else tb,
+                               C.cast(C.byref(al), _vp), a.handle, b.handle,
+                               C.cast(C.byref(be), _vp), self.handle), "gemm")
+
+    def solve_mat(self, b, lower=False, upper=False):
+        """b <- self^-1 b for a factorised self and an H-matrix b (in place);
+        lower / upper: L^-1 b / L^-T b only (solve_generic with nr_rhs = 0)."""
+        if not lower and not upper:
+            _check(self.iface.solve_mat(self.handle, b.handle), "solve_mat")
+            return
+        ctx = hmat_solve_context_t()
+        load().hmat_solve_context_init(C.byref(ctx))
+        ctx.values, ctx.nr_rhs, ctx.lower, ctx.upper = b.handle, 0, int(lower), int(upper)
+        _check(self.iface.solve_generic(self.handle, C.byref(ctx)), "solve_generic")
 
     def extract_diagonal(self):
         d = np.zeros(self.n, dtype=np.float64 if self.value_type else np.float32)
