@@ -544,6 +544,12 @@ hmat_admissibility_t* hmat_create_admissibility_standard(double eta) {
   a->p.eta = eta;
   return reinterpret_cast<hmat_admissibility_t*>(a);
 }
+// hmat_create_admissibility_hodlr c_default_interface.cpp:220-222
+hmat_admissibility_t* hmat_create_admissibility_hodlr(void) {
+  auto* a = new Admissibility();
+  a->p.hodlr = true;
+  return reinterpret_cast<hmat_admissibility_t*>(a);
+}
 void hmat_delete_admissibility(hmat_admissibility_t* c) { delete reinterpret_cast<Admissibility*>(c); }
 
 hmat_compression_algorithm_t* hmat_create_compression_svd(double eps) {

@@ -130,6 +130,17 @@ class Matrix {
   bool factor32 = false;      // factor stored in s32 (mixed precision)
   DevBuf pivots;              // LU: getrf interchanges of each diagonal leaf, at its rows
   DevBuf ldlt_diag;           // LDLt: D at the matrix's rows, in the factor's precision
+  // HODLR (symmetric) factor: per 2 x 2 node, the r x r blocks X11 and KK of
+  // the Woodbury form (hodlr.cpp:163-285) in the factor's precision, and the
+  // log of det(M11) (FP64)
+  struct HodlrNode {
+    const Block* b;   // the 2 x 2 node; b->get(1, 0) is its Rk block
+    int cap;          // capacity of that block's panels
+    int64_t x11, kk;  // offsets (elements) into hodlr_buf
+  };
+  std::vector<HodlrNode> hodlr_nodes;
+  std::map<const Block*, int> hodlr_index;
+  DevBuf hodlr_buf, hodlr_logdet;
   int overflow = 0;
   Stats stats;
 
@@ -179,6 +190,8 @@ class Matrix {
   void axpy_from(double alpha, Matrix& x);  // A <- A + alpha X, X of the same partition
   // this <- beta this + alpha op(a) op(b) (H-gemm), a and b over the same trees
   void gemm_from(char ta, char tb, double alpha, Matrix& a, Matrix& b, double beta);
+  void hodlr_layout();            // node list of a HODLR factor; checks the structure
+  void hodlr_alloc(size_t elem);  // its r x r blocks and log-determinants
   // b <- this^{-1} b for a factorised matrix and an H-matrix b (solve_mat);
   // part 1 / 2: L^{-1} b / L^{-T} b only (LLt)
   void solve_mat(Matrix& b, int part = 0);

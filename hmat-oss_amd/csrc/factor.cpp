@@ -20,6 +20,7 @@
 #include <memory>
 #include <stdexcept>
 #include <deque>
+#include <functional>
 #include <unordered_map>
 
 #include "engine.hpp"
@@ -44,6 +45,8 @@ constexpr int64_t kTempBase = int64_t(1) << 40;
 constexpr int64_t kDiagBuf = kTempBase + (int64_t(1) << 31);
 // leaf numbers from here on name scaled copies held by the executor (LDLt)
 constexpr int kShadowLeaf = 1 << 28;
+// the r x r blocks of the HODLR nodes (two per node)
+constexpr int64_t kHodlrBase = int64_t(1) << 38;
 
 // Dense view; n may be dynamic (rank slot dn).
 struct DV {
@@ -1784,6 +1787,116 @@ class Exec {
     }
   }
 
+  // ---- symmetric HODLR (hodlr.cpp) -------------------------------------------------
+  // A 2 x 2 node m = [m00, .; a b^T, m11] is factored as
+  // P W W^T P^T with P = diag(L00, L11) (the children's factors) and
+  // W = I + U X U^T, U = diag(b~, a~) the Rk panels after the children's
+  // lower solves; a node keeps X11 and KK = (I + X11 a~^T a~)^{-1} X11.
+  DV hodlr_block(int q, int which) const {
+    const Matrix::HodlrNode& nd = M_.hodlr_nodes[q];
+    DV d;
+    d.p = M_.hodlr_buf.template as<T>() + (which == 0 ? nd.x11 : nd.kk);
+    d.ld = std::max(nd.cap, 1);
+    d.m = nd.cap;
+    d.n = nd.cap;
+    d.buf = kHodlrBase + 2 * (int64_t)q + which;
+    return d;
+  }
+
+  // HODLR<T>::factorizeSym / factorizeSym hodlr.cpp:287-312
+  void hodlr_factor(const Block* a) {
+    const Block *a00 = a->get(0, 0), *a11 = a->get(1, 1), *a10 = a->get(1, 0);
+    if (a00->is_leaf()) potrf(a00); else hodlr_factor(a00);
+    if (a11->is_leaf()) potrf(a11); else hodlr_factor(a11);
+    RV r = rk_of(a10);
+    const Dim rk = dim(r.cap, r.rank);
+    DV A = dv_panel(r.a, r.la, r.m, r.cap, r.bufa, r.row0);
+    DV B = dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0);
+    hodlr_lower(a00, B, rk);  // b <- L00^-1 b
+    hodlr_lower(a11, A, rk);  // a <- L11^-1 a
+    const int q = M_.hodlr_index.at(a);
+    DV G = temp_dense(r.cap, r.cap, -1), P = temp_dense(r.cap, r.cap, -1);
+    DV At = A, Bt = B;
+    At.trans = Bt.trans = true;
+    gemm(G, false, 1.0, At, A, rk, rk, dim(r.m));
+    gemm(P, false, 1.0, Bt, B, rk, rk, dim(r.n));
+    DV X = hodlr_block(q, 0), K = hodlr_block(q, 1);
+    FTask t{};
+    t.type = T_HODLR;
+    t.m = rk;
+    t.A = G.p;
+    t.B = P.p;
+    t.lda = r.cap;
+    t.C = X.p;
+    t.tgt.a = K.p;
+    t.tgt.b = M_.hodlr_logdet.template as<double>() + q;
+    rd(G);
+    rd(P);
+    wr(X);
+    wr(K);
+    emit(t, hodlr_workspace(r.cap));
+    release();
+    release();
+    maybe_flush();
+  }
+
+  // X <- L^{-1} X for the HODLR factor of m (solveLowerTriangularLeft
+  // hodlr.cpp:138-156): the children, then x1 -= a (b^T x0), then the
+  // Woodbury correction x1 -= a KK a^T x1.
+  void hodlr_lower(const Block* m, const DV& X, Dim ncol) {
+    if (m->is_leaf()) {
+      trsm(1, m, X, ncol);
+      return;
+    }
+    const int n0 = m->get(0, 0)->m(), n1 = m->get(1, 1)->m();
+    DV X0 = rows_of(X, 0, n0), X1 = rows_of(X, n0, n1);
+    hodlr_lower(m->get(0, 0), X0, ncol);
+    hodlr_lower(m->get(1, 1), X1, ncol);
+    RV r = rk_of(m->get(1, 0));
+    const Dim rk = dim(r.cap, r.rank);
+    DV A = dv_panel(r.a, r.la, r.m, r.cap, r.bufa, r.row0);
+    DV B = dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0);
+    DV At = A, Bt = B;
+    At.trans = Bt.trans = true;
+    DV z = temp_dense(r.cap, std::max(ncol.s, 1), -1), u = temp_dense(r.cap, std::max(ncol.s, 1), -1);
+    gemm(z, false, 1.0, Bt, X0, rk, ncol, dim(n0));
+    gemm(X1, true, -1.0, A, z, dim(n1), ncol, rk);
+    gemm(z, false, 1.0, At, X1, rk, ncol, dim(n1));
+    gemm(u, false, 1.0, hodlr_block(M_.hodlr_index.at(m), 1), z, rk, ncol, rk);
+    gemm(X1, true, -1.0, A, u, dim(n1), ncol, rk);
+    release();
+    release();
+  }
+
+  // X <- L^{-T} X (solveUpperTriangularLeft hodlr.cpp:113-136): the Woodbury
+  // correction x1 -= a KK^T a^T x1, then x0 -= b a^T x1, then the children.
+  void hodlr_upper(const Block* m, const DV& X, Dim ncol) {
+    if (m->is_leaf()) {
+      trsm(2, m, X, ncol);
+      return;
+    }
+    const int n0 = m->get(0, 0)->m(), n1 = m->get(1, 1)->m();
+    DV X0 = rows_of(X, 0, n0), X1 = rows_of(X, n0, n1);
+    RV r = rk_of(m->get(1, 0));
+    const Dim rk = dim(r.cap, r.rank);
+    DV A = dv_panel(r.a, r.la, r.m, r.cap, r.bufa, r.row0);
+    DV B = dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0);
+    DV At = A;
+    At.trans = true;
+    DV Kt = hodlr_block(M_.hodlr_index.at(m), 1);
+    Kt.trans = true;
+    DV z = temp_dense(r.cap, std::max(ncol.s, 1), -1), u = temp_dense(r.cap, std::max(ncol.s, 1), -1);
+    gemm(z, false, 1.0, At, X1, rk, ncol, dim(n1));
+    gemm(u, false, 1.0, Kt, z, rk, ncol, rk);
+    gemm(X1, true, -1.0, A, u, dim(n1), ncol, rk);
+    gemm(z, false, 1.0, At, X1, rk, ncol, dim(n1));
+    gemm(X0, true, -1.0, B, z, dim(n0), ncol, rk);
+    release();
+    release();
+    hodlr_upper(m->get(0, 0), X0, ncol);
+    hodlr_upper(m->get(1, 1), X1, ncol);
+  }
+
   // HMatrix::solveDiagonal h_matrix.cpp:2969-2983 (LDLt): X <- D^{-1} X, one
   // task per diagonal leaf
   void solve_diag(const DV& X, Dim ncol) {
@@ -2011,11 +2124,17 @@ class Exec {
 void Matrix::factorize(int algo, bool fp32) {
   if (!assembled) throw std::runtime_error("factorize: matrix not assembled");
   if (factorization != -1) throw std::runtime_error("factorize: matrix already factorised");
-  if (algo != hmat_factorization_llt && algo != hmat_factorization_lu && algo != hmat_factorization_ldlt)
-    throw std::runtime_error("factorize: only hmat_factorization_llt, hmat_factorization_ldlt and "
-                             "hmat_factorization_lu are provided");
-  if ((algo == hmat_factorization_llt || algo == hmat_factorization_ldlt) && !lower_sym)
-    throw std::runtime_error("factorize: LLt and LDLt need lower-symmetric storage");
+  if (algo == hmat_factorization_hodlr)
+    throw std::runtime_error("factorize: the non-symmetric HODLR factorisation is not provided "
+                             "(hmat_factorization_hodlrsym is)");
+  if (algo != hmat_factorization_llt && algo != hmat_factorization_lu && algo != hmat_factorization_ldlt &&
+      algo != hmat_factorization_hodlrsym)
+    throw std::runtime_error("factorize: only hmat_factorization_llt, hmat_factorization_ldlt, "
+                             "hmat_factorization_hodlrsym and hmat_factorization_lu are provided");
+  if ((algo == hmat_factorization_llt || algo == hmat_factorization_ldlt ||
+       algo == hmat_factorization_hodlrsym) && !lower_sym)
+    throw std::runtime_error("factorize: LLt, LDLt and HODLR need lower-symmetric storage");
+  if (algo == hmat_factorization_hodlrsym) hodlr_layout();
   if (algo == hmat_factorization_lu && lower_sym)
     throw std::runtime_error("factorize: LU needs non-symmetric storage");
   if (algo == hmat_factorization_lu && rows->n != cols->n)
@@ -2043,6 +2162,9 @@ void Matrix::factorize(int algo, bool fp32) {
     } else if (algo == hmat_factorization_ldlt) {
       ldlt_diag.alloc(sizeof(float) * (size_t)std::max(rows->n, 1));
       ex.ldlt(bt->root);
+    } else if (algo == hmat_factorization_hodlrsym) {
+      hodlr_alloc(sizeof(float));
+      ex.hodlr_factor(bt->root);
     } else {
       ex.llt(bt->root);
     }
@@ -2060,6 +2182,9 @@ void Matrix::factorize(int algo, bool fp32) {
     } else if (algo == hmat_factorization_ldlt) {
       ldlt_diag.alloc(sizeof(double) * (size_t)std::max(rows->n, 1));
       ex.ldlt(bt->root);
+    } else if (algo == hmat_factorization_hodlrsym) {
+      hodlr_alloc(sizeof(double));
+      ex.hodlr_factor(bt->root);
     } else {
       ex.llt(bt->root);
     }
@@ -2090,7 +2215,7 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part)
     throw std::runtime_error("solve: the persistent-sweep solve is disabled (it did not complete "
                              "on the device; cause not found)");
   if (part != 0 && factorization == hmat_factorization_lu)
-    throw std::runtime_error("solve_lower_triangular: provided for LLt and LDLt factors only");
+    throw std::runtime_error("solve_lower_triangular: provided for LLt, LDLt and HODLR factors only");
   const auto key = std::make_pair((int)sizeof(T) + 16 * part, nrhs);
   const size_t bytes = (size_t)n * nrhs * sizeof(T);
   auto it = fs.solve_plans.find(key);
@@ -2115,6 +2240,11 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part)
       // HMatrix::solve (LU) h_matrix.cpp:2772-2779
       ex.lu_solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
       ex.lu_solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
+    } else if (factorization == hmat_factorization_hodlrsym) {
+      // HODLR<T>::solveSymLower then solveSymUpper (DefaultEngine::solve
+      // default_engine.cpp:205-208, hodlr.cpp:455-470)
+      if (part != 2) ex.hodlr_lower(bt->root, X, Exec<T>::dim(nrhs));
+      if (part != 1) ex.hodlr_upper(bt->root, X, Exec<T>::dim(nrhs));
     } else {
       // HMatrix::solveLlt h_matrix.cpp:3010-3018 (part 1: L x = b only, part 2:
       // L^T x = b only, as solveLower(b, transpose) of the solve_lower_triangular slot)
@@ -2271,6 +2401,41 @@ void Matrix::axpy_from(double alpha, Matrix& x) {
   }
 }
 
+// The node list of a symmetric HODLR factor (HODLRNode::create hodlr.cpp:
+// 394-404); the matrix must have the HODLR structure: every inner block a
+// lower-symmetric 2 x 2 node whose (1, 0) child is an Rk leaf, every leaf
+// below the diagonal Rk, the diagonal leaves full.
+void Matrix::hodlr_layout() {
+  hodlr_nodes.clear();
+  hodlr_index.clear();
+  int64_t off = 0;
+  std::function<void(const Block*)> walk = [&](const Block* b) {
+    if (b->is_leaf()) {
+      if (b->kind != Kind::Full) throw std::runtime_error("Not HODLR matrix");
+      return;
+    }
+    const Block* b10 = b->nrow == 2 && b->ncol == 2 ? b->get(1, 0) : nullptr;
+    if (!b->sym || !b10 || b->get(0, 1) || !b10->is_leaf() || b10->kind != Kind::Rk)
+      throw std::runtime_error("Not HODLR matrix");
+    const int cap = s64.hdesc.at(b10->leaf).cap;
+    hodlr_index[b] = (int)hodlr_nodes.size();
+    hodlr_nodes.push_back(HodlrNode{b, cap, off, off + (int64_t)cap * cap});
+    off += 2 * (int64_t)cap * cap;
+    walk(b->get(0, 0));
+    walk(b->get(1, 1));
+  };
+  if (bt->root->is_leaf()) throw std::runtime_error("Not HODLR matrix");
+  walk(bt->root);
+}
+
+void Matrix::hodlr_alloc(size_t elem) {
+  int64_t elems = 1;
+  for (const HodlrNode& nd : hodlr_nodes) elems = std::max(elems, nd.kk + (int64_t)nd.cap * nd.cap);
+  hodlr_buf.alloc(elem * (size_t)elems);
+  hodlr_logdet.alloc(sizeof(double) * std::max<size_t>(hodlr_nodes.size(), 1));
+  HMX_CHECK(hipMemsetAsync(hodlr_logdet.p, 0, hodlr_logdet.bytes, stream));
+}
+
 void Matrix::gemm_from(char ta, char tb, double alpha, Matrix& a, Matrix& b, double beta) {
   auto norm_t = [](char t) {
     if (t == 'N' || t == 'n') return 'N';
@@ -2316,6 +2481,8 @@ void Matrix::gemm_from(char ta, char tb, double alpha, Matrix& a, Matrix& b, dou
 
 void Matrix::solve_mat(Matrix& b, int part) {
   if (factorization == -1) throw std::runtime_error("solve_mat: matrix not factorised");
+  if (factorization == hmat_factorization_hodlrsym)
+    throw std::runtime_error("solve_mat: H-matrix right-hand sides are not supported with HODLR factors");
   if (part != 0 && factorization != hmat_factorization_llt)
     throw std::runtime_error("solveLower only supported by LLt factorization.");
   if (&b == this) throw std::runtime_error("solve_mat: B must differ from the factorised matrix");
@@ -2347,9 +2514,15 @@ void Matrix::solve_mat(Matrix& b, int part) {
 
 double Matrix::logdet() {
   // HMatrix::logdet h_matrix.cpp:2925-2936: sum of log of the diagonal of L
-  if (factorization != hmat_factorization_llt)
-    throw std::runtime_error("logdet is only supported for LLt factorized matrices.");
+  if (factorization != hmat_factorization_llt && factorization != hmat_factorization_hodlrsym)
+    throw std::runtime_error("logdet is only supported for LLt or HODLR factorized matrices.");
   double r = 0.0;
+  if (factorization == hmat_factorization_hodlrsym) {
+    // logdet hodlr.cpp:158-166: the leaves' log det L plus log det M11 per node
+    std::vector<double> ld(hodlr_nodes.size());
+    HMX_CHECK(hipMemcpy(ld.data(), hodlr_logdet.p, ld.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (double v : ld) r += v;
+  }
   for (const Block* b : bt->leaves) {
     if (b->kind != Kind::Full || b->rows != b->cols) continue;
     const LeafDesc& L = s64.hdesc[b->leaf];

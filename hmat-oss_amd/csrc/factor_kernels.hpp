@@ -15,6 +15,7 @@ namespace hmx {
 int64_t fadd_workspace(int m, int n, int kmax_concat);
 int64_t fadd_dense_workspace(int m, int n);
 int64_t rkrk_workspace(int k1, int k2);
+int64_t hodlr_workspace(int cap);
 
 // p[0..n) = v on stream s (a kernel, so it may be captured in a graph).
 void launch_fill_int(int* p, size_t n, int v, hipStream_t s);

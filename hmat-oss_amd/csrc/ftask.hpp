@@ -26,6 +26,8 @@ enum TaskType : int {
   T_SUMK,         // C (+)= sum over p.s slices of A (ld lda, slice stride ldb): split-K GEMM
   T_LDLT,         // A <- L D L^T in place (L unit lower), D into B (T*)
   T_DSCALE,       // C = A scaled by the diagonal B (T*): mode kDscale* flags
+  T_HODLR,        // r x r blocks of a symmetric HODLR node from the Grams A = a^T a,
+                  // B = b^T b: X11 into C, KK into tgt.a, log det(M11) into tgt.b (double*)
   T_NTYPES
 };
 

@@ -375,6 +375,130 @@ __global__ void __launch_bounds__(kBlock) k_getrf(const FTask* tasks, int* info)
 }
 
 // ---------------------------------------------------------------------------
+// The r x r blocks of a symmetric HODLR node (computeSymX11 + computeSymKK,
+// hodlr.cpp:163-219, the Gram form the reference selects with
+// HMAT_HODLR_NOQR; its default QR form is the same in exact arithmetic).
+// With G = a^T a and P = b^T b of the node's transformed Rk block a b^T:
+//   La La^T = G,  M11 M11^T = I - La^T P La,  log det M11 into tgt.b,
+//   X11 = La^-T (M11 - I) La^-1 into C,  KK = (I + X11 G)^-1 X11 into tgt.a.
+// FP64 in the task's workspace (5 r^2 + r values), one workgroup per node.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_hodlr(const FTask* tasks, double* ws_base, int* info) {
+  const FTask& t = tasks[blockIdx.x];
+  const int n = eff(t.m), ld = t.lda;
+  const T* G = (const T*)t.A;
+  const T* P = (const T*)t.B;
+  T* X11 = (T*)t.C;
+  T* KK = (T*)t.tgt.a;
+  double* logdet = (double*)t.tgt.b;
+  if (n == 0) {
+    if (threadIdx.x == 0) *logdet = 0.0;
+    return;
+  }
+  const size_t nn = (size_t)n * n;
+  double* la = ws_base + t.ws;
+  double* x = la + nn;
+  double* tmp = x + nn;
+  double* g = tmp + nn;
+  double* y = g + nn;
+  int* piv = (int*)(y + nn);
+  for (size_t e = threadIdx.x; e < nn; e += kBlock) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    g[e] = (double)G[i + (size_t)j * ld];
+    la[e] = g[e];
+    tmp[e] = (double)P[i + (size_t)j * ld];
+  }
+  __syncthreads();
+  potrf_body<double>(la, n, n, info);
+  __syncthreads();
+  for (size_t e = threadIdx.x; e < nn; e += kBlock)
+    if ((int)(e % n) < (int)(e / n)) la[e] = 0.0;
+  __syncthreads();
+  // x = -P La (La lower)
+  for (size_t e = threadIdx.x; e < nn; e += kBlock) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    double s = 0.0;
+    for (int k = j; k < n; ++k) s += tmp[i + (size_t)k * n] * la[k + (size_t)j * n];
+    x[e] = -s;
+  }
+  __syncthreads();
+  // tmp = I + La^T x
+  for (size_t e = threadIdx.x; e < nn; e += kBlock) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    double s = 0.0;
+    for (int k = i; k < n; ++k) s += la[k + (size_t)i * n] * x[k + (size_t)j * n];
+    tmp[e] = s + (i == j ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  potrf_body<double>(tmp, n, n, info);  // M11 in the lower triangle
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += log(tmp[i + (size_t)i * n]);
+    *logdet = s;
+  }
+  // x = M11 - I (lower)
+  for (size_t e = threadIdx.x; e < nn; e += kBlock) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    x[e] = i >= j ? tmp[e] - (i == j ? 1.0 : 0.0) : 0.0;
+  }
+  __syncthreads();
+  // x <- x La^-1 (each row on its own)
+  for (int i = threadIdx.x; i < n; i += kBlock)
+    for (int j = n - 1; j >= 0; --j) {
+      double s = x[i + (size_t)j * n];
+      for (int k = j + 1; k < n; ++k) s -= x[i + (size_t)k * n] * la[k + (size_t)j * n];
+      x[i + (size_t)j * n] = s / la[j + (size_t)j * n];
+    }
+  __syncthreads();
+  // x <- La^-T x (each column on its own)
+  for (int j = threadIdx.x; j < n; j += kBlock)
+    for (int i = n - 1; i >= 0; --i) {
+      double s = x[i + (size_t)j * n];
+      for (int k = i + 1; k < n; ++k) s -= la[k + (size_t)i * n] * x[k + (size_t)j * n];
+      x[i + (size_t)j * n] = s / la[i + (size_t)i * n];
+    }
+  __syncthreads();
+  // X11 out; tmp = I + X11 G; y = X11
+  for (size_t e = threadIdx.x; e < nn; e += kBlock) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    X11[i + (size_t)j * ld] = (T)x[e];
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += x[i + (size_t)k * n] * g[k + (size_t)j * n];
+    tmp[e] = s + (i == j ? 1.0 : 0.0);
+    y[e] = x[e];
+  }
+  __syncthreads();
+  getrf_body<double>(tmp, n, n, piv, info);
+  __syncthreads();
+  // KK = tmp^-1 X11: interchanges, unit lower, upper
+  for (int j = 0; j < n; ++j) {
+    const int p = piv[j];
+    if (p != j && p >= 0 && p < n)
+      for (int c = threadIdx.x; c < n; c += kBlock) {
+        const double a = y[j + (size_t)c * n];
+        y[j + (size_t)c * n] = y[p + (size_t)c * n];
+        y[p + (size_t)c * n] = a;
+      }
+    __syncthreads();
+  }
+  for (int c = threadIdx.x; c < n; c += kBlock) {
+    for (int i = 0; i < n; ++i) {
+      double s = y[i + (size_t)c * n];
+      for (int k = 0; k < i; ++k) s -= tmp[i + (size_t)k * n] * y[k + (size_t)c * n];
+      y[i + (size_t)c * n] = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double s = y[i + (size_t)c * n];
+      for (int k = i + 1; k < n; ++k) s -= tmp[i + (size_t)k * n] * y[k + (size_t)c * n];
+      y[i + (size_t)c * n] = s / tmp[i + (size_t)i * n];
+    }
+    for (int i = 0; i < n; ++i) KK[i + (size_t)c * ld] = (T)y[i + (size_t)c * n];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // TRSM with a lower-triangular leaf L (n x n), L staged in LDS:
 //   mode 0: X (m x n) <- X L^{-T}     (solveUpperTriangularRight, LLt)
 //   mode 1: X (n x k) <- L^{-1} X     (solveLowerTriangularLeft)
@@ -1408,6 +1532,7 @@ int64_t fadd_dense_workspace(int m, int n) {
   const int mn = m < n ? m : n;
   return ((int64_t)m * n + (int64_t)mn * mn + 2 * (int64_t)mn + 127) / 64 * 64;
 }
+int64_t hodlr_workspace(int cap) { return ((int64_t)5 * cap * cap + cap + 127) / 64 * 64; }
 int64_t rkrk_workspace(int k1, int k2) {
   const int64_t a = k1 > k2 ? k1 : k2, b = k1 < k2 ? k1 : k2;
   // G, V, sig, order (+ pad), then the two output factors (a + b) x b
@@ -1471,6 +1596,9 @@ void launch_task_kernel(int type, const FTask* tasks, int count, const GemmTile*
       break;
     case T_DSCALE:
       hipLaunchKernelGGL(k_dscale<T>, dim3(count * kDscaleSplit), dim3(256), 0, s, tasks, flags + 1);
+      break;
+    case T_HODLR:
+      hipLaunchKernelGGL(k_hodlr<T>, dim3(count), dim3(kBlock), 0, s, tasks, ws, flags + 1);
       break;
     default:
       throw std::runtime_error("unknown task type");

@@ -112,6 +112,10 @@ struct Serializer {
     return M.value_type == HMAT_SIMPLE_PRECISION;
   }
   static int written_factorization(const Matrix& M) {
+    // the HODLR factor's node blocks have no place in the stream (the
+    // reference keeps them outside the matrix and cannot read them back)
+    if (M.factorization == hmat_factorization_hodlrsym || M.factorization == hmat_factorization_hodlr)
+      throw std::runtime_error("serialization of HODLR factors is not supported");
     // an FP32 factor of an FP64 matrix (mixed precision) is not a factor of the
     // stored matrix's type: the FP64 matrix is written unfactorised
     if (M.factorization != -1 && M.factor32 && M.value_type == HMAT_DOUBLE_PRECISION) return -1;

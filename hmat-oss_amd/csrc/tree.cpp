@@ -179,11 +179,14 @@ struct Builder {
     b->rows = r;
     b->cols = c;
     b->depth = depth;
-    const bool low = is_low_rank(r, c);
-    const bool stop = r->size < 2 || c->size < 2;
+    const bool same = r->offset == c->offset && r->size == c->size;
+    const bool low = p.hodlr ? !same : is_low_rank(r, c);
+    // the standard condition stops and keeps full below 2 rows or columns; the
+    // HODLR condition keeps the base class's defaults (never)
+    const bool stop = !p.hodlr && (r->size < 2 || c->size < 2);
     const bool force_rec = (size_t)r->size > p.max_width || (size_t)c->size > p.max_width;
     if ((r->is_leaf() && c->is_leaf()) || stop || (low && !force_rec)) {
-      const bool force_full = r->size < 2 || c->size < 2;
+      const bool force_full = !p.hodlr && (r->size < 2 || c->size < 2);
       b->kind = (low && !force_full) ? Kind::Rk : Kind::Full;
       return b;
     }

@@ -80,6 +80,9 @@ struct AdmissibilityParams {
   double eta = 2.0;
   double ratio = 0.0;
   size_t max_width = (size_t)-1;
+  // HODLRAdmissibilityCondition (admissibility.cpp:231-241): every block
+  // between two different clusters is low rank, whatever its size
+  bool hodlr = false;
 };
 
 struct BlockTree {

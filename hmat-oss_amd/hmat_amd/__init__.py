@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libhmat_mi355x.so")
 
 HMAT_SIMPLE_PRECISION, HMAT_DOUBLE_PRECISION = 0, 1
-FACT_NONE, FACT_LU, FACT_LDLT, FACT_LLT = -1, 0, 1, 2
+FACT_NONE, FACT_LU, FACT_LDLT, FACT_LLT, FACT_HODLR, FACT_HODLRSYM = -1, 0, 1, 2, 3, 4
 KERNEL_EXPONENTIAL, KERNEL_GAUSSIAN, KERNEL_SHIFTED_EXPONENTIAL = 0, 1, 2
 # derivatives with respect to the correlation length (for the GP likelihood gradient)
 KERNEL_EXPONENTIAL_DLENGTH, KERNEL_GAUSSIAN_DLENGTH = 3, 4
@@ -145,6 +145,7 @@ _EXPORTS = {
     "hmat_tree_nodes_count": (C.c_int, [_vp]),
     "hmat_cluster_get_info": (C.c_int, [_vp, _vp]),
     "hmat_create_admissibility_standard": (_vp, [C.c_double]),
+    "hmat_create_admissibility_hodlr": (_vp, []),
     "hmat_delete_admissibility": (None, [_vp]),
     "hmat_create_compression_aca_plus": (_vp, [C.c_double]),
     "hmat_create_compression_aca_partial": (_vp, [C.c_double]),
@@ -267,12 +268,13 @@ class HMatrix:
     """An H-matrix created and driven through hmat_interface_t."""
 
     def __init__(self, tree: ClusterTree, eta=2.0, lower_symmetric=True,
-                 value_type=HMAT_DOUBLE_PRECISION, lowrank_eps=1e-4):
+                 value_type=HMAT_DOUBLE_PRECISION, lowrank_eps=1e-4, hodlr=False):
         lib = load()
         self.tree = tree
         self.iface = interface(value_type)
         self.value_type = value_type
-        adm = lib.hmat_create_admissibility_standard(float(eta))
+        adm = (lib.hmat_create_admissibility_hodlr() if hodlr
+               else lib.hmat_create_admissibility_standard(float(eta)))
         self.handle = self.iface.create_empty_hmatrix_admissibility(
             tree.handle, tree.handle, 1 if lower_symmetric else 0, adm)
         lib.hmat_delete_admissibility(adm)

@@ -143,6 +143,9 @@ typedef struct hmat_admissibility_condition hmat_admissibility_t;
 HMAT_API hmat_admissibility_t *hmat_create_admissibility(hmat_admissibility_param_t *);
 HMAT_API void hmat_update_admissibility(hmat_admissibility_t *, hmat_admissibility_param_t *);
 HMAT_API hmat_admissibility_t *hmat_create_admissibility_standard(double eta);
+/* Admissibility of a HODLR matrix: every block between two different
+ * clusters is low rank (hmat.h:358-362 of hmat-oss). */
+HMAT_API hmat_admissibility_t *hmat_create_admissibility_hodlr(void);
 HMAT_API void hmat_delete_admissibility(hmat_admissibility_t *cond);
 
 typedef struct hmat_compression_algorithm_struct hmat_compression_algorithm_t;

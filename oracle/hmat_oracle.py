@@ -18,6 +18,8 @@ below (paths are relative to the hmat-oss source tree):
 * H-gemm / axpy              h_matrix.cpp:870-1073, 1202-1620, 1874-1950
 * H-LLt and triangular solves recursion.cpp:86-134, 191-233, 349-444,
                               h_matrix.cpp:2297-2564, 3010-3018
+* HODLR (symmetric)          hodlr.cpp:113-166, 221-312, 365-404,
+                              admissibility.cpp:231-241
 * H-LDLt                     recursion.cpp:37-83, h_matrix.cpp:1953-1998,
                               2585-2690, 2747-2769, 2969-3002,
                               scalar_array.cpp:737-779
@@ -217,16 +219,22 @@ def _is_low_rank(r, c, eta):
     return min_d > 0.0 and min_d <= eta * r.distance_to(c)
 
 
-def build_block_tree(row_root, col_root, eta=2.0, lower_symmetric=True):
+def build_block_tree(row_root, col_root, eta=2.0, lower_symmetric=True, hodlr=False):
     """HMatrix constructor + split (h_matrix.cpp:144-221) with the standard
-    admissibility (ratio 0, max_width unbounded)."""
+    admissibility (ratio 0, max_width unbounded), or with hodlr=True the HODLR
+    one (HODLRAdmissibilityCondition admissibility.cpp:231-241: a block between
+    different clusters is low rank; no size limits)."""
 
     def make(r, c, sym, depth):
         b = Block(r, c, depth)
-        low = _is_low_rank(r, c, eta)
-        stop = r.size < 2 or c.size < 2
+        if hodlr:
+            low = not (r.offset == c.offset and r.size == c.size)
+            stop = False
+        else:
+            low = _is_low_rank(r, c, eta)
+            stop = r.size < 2 or c.size < 2
         if (r.is_leaf() and c.is_leaf()) or stop or low:
-            force_full = r.size < 2 or c.size < 2
+            force_full = not hodlr and (r.size < 2 or c.size < 2)
             b.kind = RK if (low and not force_full) else FULL
             return b
         split_r, split_c = (not r.is_leaf()), (not c.is_leaf())
@@ -1330,6 +1338,101 @@ def solve_ldlt(L, d, b):
     solve_lower_left_dense(L, b)
     b *= (1.0 / d)[:, None]
     solve_upper_left_dense(L, b)
+
+
+class HodlrNode:
+    """HODLRNode hodlr.cpp:365-404: the r x r blocks X11 and KK of a symmetric
+    node and det(M11)."""
+    __slots__ = ("x11", "kk", "detm11", "child0", "child1")
+
+    def __init__(self):
+        self.x11 = self.kk = None
+        self.detm11 = 1.0
+        self.child0 = self.child1 = None
+
+
+def _tri_solve(L, x, lower, trans=False):
+    import scipy.linalg as sl
+    return sl.solve_triangular(L, x, lower=lower, trans=1 if trans else 0)
+
+
+def hodlr_factor_sym(a):
+    """HODLR<T>::factorizeSym hodlr.cpp:287-312 on a HODLR block tree (every
+    inner block a lower-symmetric 2 x 2 node with an Rk (1, 0) child); the
+    node blocks in the reference's default QR form (computeQRSymX11 /
+    computeQRSymKK hodlr.cpp:221-285).  The leaves get their Cholesky factors,
+    the Rk blocks the panels a <- L11^-1 a, b <- L00^-1 b.  Returns the node
+    tree."""
+    if a.is_leaf():
+        raise ValueError("Not HODLR matrix")
+    a00, a11, a10 = a.get(0, 0), a.get(1, 1), a.get(1, 0)
+    if a10 is None or a10.kind != RK or a.get(0, 1) is not None:
+        raise ValueError("Not HODLR matrix")
+    node = HodlrNode()
+    for k, ch in ((0, a00), (1, a11)):
+        if ch.is_leaf():
+            ch.full = np.linalg.cholesky(ch.full)
+        elif k == 0:
+            node.child0 = hodlr_factor_sym(ch)
+        else:
+            node.child1 = hodlr_factor_sym(ch)
+    r = a10.rk.rank
+    if r == 0:
+        node.x11 = node.kk = np.zeros((0, 0))
+        return node
+    b = a10.rk.b.copy()
+    hodlr_lower(a00, b, node.child0)
+    ua = a10.rk.a.copy()
+    hodlr_lower(a11, ua, node.child1)
+    a10.rk = Rk(ua, b)
+    ra = np.linalg.qr(ua, mode="r")
+    x = ra @ (-(b.T @ b)) @ ra.T + np.eye(r)
+    m11 = np.linalg.cholesky(x)
+    node.detm11 = float(np.prod(np.diag(m11)))
+    x = _tri_solve(ra, m11 - np.eye(r), lower=False)          # Ra^-1 (M11 - I)
+    tmp = x @ ra + np.eye(r)                                  # Ra^-1 (M11 - I) Ra + I
+    node.x11 = _tri_solve(ra, x.T, lower=False).T             # Ra^-1 (M11 - I) Ra^-T
+    node.kk = np.linalg.solve(tmp, node.x11)
+    return node
+
+
+def hodlr_lower(m, x, node):
+    """x <- L^-1 x (solveLowerTriangularLeft hodlr.cpp:138-156)."""
+    if m.is_leaf():
+        x[:] = _tri_solve(m.full, x, lower=True)
+        return
+    n0 = m.get(0, 0).m
+    x0, x1 = x[:n0], x[n0:]
+    hodlr_lower(m.get(0, 0), x0, node.child0)
+    hodlr_lower(m.get(1, 1), x1, node.child1)
+    rk = m.get(1, 0).rk
+    if rk.rank == 0:
+        return
+    x1 -= rk.a @ (rk.b.T @ x0)
+    x1 -= rk.a @ (node.kk @ (rk.a.T @ x1))
+
+
+def hodlr_upper(m, x, node):
+    """x <- L^-T x (solveUpperTriangularLeft hodlr.cpp:113-136)."""
+    if m.is_leaf():
+        x[:] = _tri_solve(m.full, x, lower=True, trans=True)
+        return
+    n0 = m.get(0, 0).m
+    x0, x1 = x[:n0], x[n0:]
+    rk = m.get(1, 0).rk
+    if rk.rank > 0:
+        x1 -= rk.a @ (node.kk.T @ (rk.a.T @ x1))
+        x0 -= rk.b @ (rk.a.T @ x1)
+    hodlr_upper(m.get(0, 0), x0, node.child0)
+    hodlr_upper(m.get(1, 1), x1, node.child1)
+
+
+def hodlr_logdet(m, node):
+    """logdet hodlr.cpp:158-166 (log det L = 1/2 log det A)."""
+    if m.is_leaf():
+        return float(np.sum(np.log(np.diag(m.full))))
+    return (hodlr_logdet(m.get(0, 0), node.child0) + hodlr_logdet(m.get(1, 1), node.child1)
+            + math.log(node.detm11))
 
 
 def solve_llt(L, b):
