@@ -168,8 +168,8 @@ __device__ inline bool wg_xty_any(const T* X, int ldx, int kx, const T* Y, int l
 // out (rows x nk, ld ldo) = M (rows x k, ld ldm) C (k x nk, ld ldc; LDS or
 // global): each thread owns rows and streams a row of M once per eight
 // output columns.
-template <typename T>
-__device__ void rows_times_small(const T* M, int ldm, int rows, int k, const double* C, int ldc, int nk, T* out,
+template <typename T, typename TO = T>
+__device__ void rows_times_small(const T* M, int ldm, int rows, int k, const double* C, int ldc, int nk, TO* out,
                                  int ldo) {
   constexpr int kC = 8;
   for (int c0 = 0; c0 < nk; c0 += kC) {
@@ -198,7 +198,7 @@ __device__ void rows_times_small(const T* M, int ldm, int rows, int k, const dou
       }
 #pragma unroll
       for (int b = 0; b < kC; ++b)
-        if (c0 + b < nk) out[i + (size_t)(c0 + b) * ldo] = (T)acc[b];
+        if (c0 + b < nk) out[i + (size_t)(c0 + b) * ldo] = (TO)acc[b];
     }
   }
   __syncthreads();

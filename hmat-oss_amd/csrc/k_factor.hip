@@ -1126,6 +1126,189 @@ __device__ void qr_finish(const double* Va, int lda, int m, const double* Vb, in
 }
 
 // ---------------------------------------------------------------------------
+// Gram recompression of single-precision panels.  The formatted sum is
+// [A] [B]^T with A (m x K), B (n x K) the concatenated panels (FP64 copies).
+// Instead of Householder QR of each panel (one column step at a time), the
+// Grams A^T A and B^T B are formed on the FP64 matrix cores in one pass over
+// the panels and factored by pivoted Cholesky, A P = Q [R11 R12] with Q =
+// A P1 R11^-1, stopping where the remaining diagonal falls below 1e-13 of the
+// largest (columns below 3e-7 of the largest norm, far under the FP32 data's
+// own rounding relative to eps).  The core R~a R~b^T (R with its columns back
+// in panel order) gets the same Jacobi SVD and eps cut as the QR form of
+// RkMatrix::truncate (rk_matrix.cpp:490-585), and the new panels are
+// A' = A P1 R11^-1 U sqrt(S), B' = B P1 R11^-1 V sqrt(S).  Q is orthonormal to
+// about 1e-16 / 1e-13 = 1e-3, so the truncated product stays within eps.
+// ---------------------------------------------------------------------------
+constexpr int kGramMaxK = 32;
+constexpr double kGramTol = 1e-13;
+
+// Pivoted Cholesky of G (K x K, ld K, symmetric, K <= 64) by one wavefront:
+// returns the rank r; rows 0..r-1 of the upper triangle hold R in the pivoted
+// column order, piv[0..K) the order.
+__device__ inline int wave_pchol(double* G, int K, int* piv) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if (lane < K) piv[lane] = lane;
+  const double dmax = wave_argmax(ArgMax{lane < K ? G[lane + (size_t)lane * K] : -1.0, lane < K ? lane : -1}).v;
+  const double tol = kGramTol * dmax;
+  __builtin_amdgcn_wave_barrier();
+  int r = 0;
+  for (int j = 0; j < K; ++j) {
+    const bool in = lane >= j && lane < K;
+    const ArgMax a = wave_argmax(ArgMax{in ? G[lane + (size_t)lane * K] : -1.0, in ? lane : -1});
+    if (!(a.v > tol)) break;  // also on NaN
+    const int p = a.i;
+    if (p != j) {
+      if (lane < K) {
+        const double x = G[j + (size_t)lane * K];
+        G[j + (size_t)lane * K] = G[p + (size_t)lane * K];
+        G[p + (size_t)lane * K] = x;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (lane < K) {
+        const double x = G[lane + (size_t)j * K];
+        G[lane + (size_t)j * K] = G[lane + (size_t)p * K];
+        G[lane + (size_t)p * K] = x;
+      }
+      if (lane == 0) {
+        const int q = piv[j];
+        piv[j] = piv[p];
+        piv[p] = q;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    const double d = sqrt(G[j + (size_t)j * K]);
+    __builtin_amdgcn_wave_barrier();
+    if (lane > j && lane < K) G[j + (size_t)lane * K] /= d;
+    if (lane == j) G[j + (size_t)j * K] = d;
+    __builtin_amdgcn_wave_barrier();
+    const int tr = K - j - 1;
+    for (int e = lane; e < tr * tr; e += kWave) {
+      const int i = j + 1 + e % tr, l = j + 1 + e / tr;
+      G[i + (size_t)l * K] -= G[j + (size_t)i * K] * G[j + (size_t)l * K];
+    }
+    __builtin_amdgcn_wave_barrier();
+    r = j + 1;
+  }
+  return r;
+}
+
+// LDS doubles of gram_recompress for K columns
+__device__ inline int gram_lds(int K) { return 6 * K * K + 4 * K + xty_part(K, K); }
+
+template <typename T>
+__device__ void gram_recompress(const FTask& t, int kt, int K, double* gws, double* sh, double eps,
+                                int* overflow, int* flag) {
+  __shared__ int ranks[2];
+  const RkRef tg = t.tgt;
+  const int m = tg.m, n = tg.n;
+  double* Va = gws;
+  double* Vb = gws + (size_t)m * K;
+  concat_side<T>(Va, m, m, false, t, kt);
+  concat_side<T>(Vb, n, n, true, t, kt);
+  __syncthreads();
+  const int K2 = K * K;
+  double* Ga = sh;
+  double* Gb = Ga + K2;
+  double* Cm = Gb + K2;
+  double* Cin = Cm + K2;
+  double* Ma = Cin + K2;
+  double* Mb = Ma + K2;
+  double* sig = Mb + K2;
+  int* order = (int*)(sig + K);
+  int* pa = order + K;
+  int* pb = pa + K;
+  double* part = sig + 4 * K;
+  wg_xty_any<double>(Va, m, K, Va, m, K, m, Ga, K, false, part);
+  wg_xty_any<double>(Vb, n, K, Vb, n, K, n, Gb, K, false, part);
+  const int wid = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  if (wid == 0) {
+    const int r = wave_pchol(Ga, K, pa);
+    if (lane == 0) ranks[0] = r;
+  } else if (wid == 1) {
+    const int r = wave_pchol(Gb, K, pb);
+    if (lane == 0) ranks[1] = r;
+  }
+  __syncthreads();
+  const int ra = ranks[0], rb = ranks[1];
+  if (ra == 0 || rb == 0) {
+    if (threadIdx.x == 0) *tg.rank = 0;
+    return;
+  }
+  // R~a (ra x K) and R~b (rb x K), ld K, columns in panel order
+  for (int e = threadIdx.x; e < K2; e += kBlock) Ma[e] = Mb[e] = 0.0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < ra * K; e += kBlock) {
+    const int x = e % ra, i = e / ra;
+    if (i >= x) Ma[x + (size_t)pa[i] * K] = Ga[x + (size_t)i * K];
+  }
+  for (int e = threadIdx.x; e < rb * K; e += kBlock) {
+    const int y = e % rb, i = e / rb;
+    if (i >= y) Mb[y + (size_t)pb[i] * K] = Gb[y + (size_t)i * K];
+  }
+  __syncthreads();
+  // core (ra x rb, ld ra) = R~a R~b^T, kept in Cin for the right side
+  for (int e = threadIdx.x; e < ra * rb; e += kBlock) {
+    const int x = e % ra, y = e / ra;
+    double s = 0.0;
+    for (int c = 0; c < K; ++c) s += Ma[x + (size_t)c * K] * Mb[y + (size_t)c * K];
+    Cm[e] = s;
+    Cin[e] = s;
+  }
+  __syncthreads();
+  if (rb <= kWaveJacobiMax) {
+    if (wid == 0) wave_jacobi_svd(Cm, ra, ra, rb, nullptr, 0);
+    __syncthreads();
+  } else {
+    jacobi_svd(Cm, ra, ra, rb, nullptr, 0, flag);
+  }
+  sorted_norms(Cm, ra, ra, rb, sig, order);
+  int nk = find_k(sig, order, rb, eps);
+  if (nk > tg.cap) {
+    if (threadIdx.x == 0) atomicOr(overflow, 1);
+    nk = tg.cap;
+  }
+  // left U sqrt(S) (ra x nk) and right V sqrt(S) = Cin^T U / sqrt(S) (rb x nk), ld K
+  for (int e = threadIdx.x; e < ra * nk; e += kBlock) {
+    const int x = e % ra, oc = order[e / ra];
+    Ma[x + (size_t)(e / ra) * K] = Cm[x + (size_t)oc * ra] / sqrt(sig[oc]);
+  }
+  for (int e = threadIdx.x; e < rb * nk; e += kBlock) {
+    const int y = e % rb, oc = order[e / rb];
+    double s = 0.0;
+    for (int x = 0; x < ra; ++x) s += Cin[x + (size_t)y * ra] * Cm[x + (size_t)oc * ra];
+    Mb[y + (size_t)(e / rb) * K] = s / (sig[oc] * sqrt(sig[oc]));
+  }
+  __syncthreads();
+  // R11^-1 of each side, one column per thread (A: threads 0.., B: 128..)
+  const int half = kBlock / 2;
+  {
+    const bool bside = threadIdx.x >= half;
+    const int c = bside ? threadIdx.x - half : threadIdx.x;
+    const int r = bside ? rb : ra;
+    const double* R = bside ? Gb : Ga;
+    double* X = bside ? Mb : Ma;
+    if (c < nk)
+      for (int x = r - 1; x >= 0; --x) {
+        double v = X[x + (size_t)c * K];
+        for (int i = x + 1; i < r; ++i) v -= R[x + (size_t)i * K] * X[i + (size_t)c * K];
+        X[x + (size_t)c * K] = v / R[x + (size_t)x * K];
+      }
+  }
+  __syncthreads();
+  // scatter to panel order (rows piv[i]) into Cm / Cin, K x nk, ld K
+  for (int e = threadIdx.x; e < K * nk; e += kBlock) Cm[e] = Cin[e] = 0.0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < ra * nk; e += kBlock)
+    Cm[pa[e % ra] + (size_t)(e / ra) * K] = Ma[e % ra + (size_t)(e / ra) * K];
+  for (int e = threadIdx.x; e < rb * nk; e += kBlock)
+    Cin[pb[e % rb] + (size_t)(e / rb) * K] = Mb[e % rb + (size_t)(e / rb) * K];
+  __syncthreads();
+  rows_times_small<double, T>(Va, m, m, K, Cm, K, nk, (T*)tg.a, tg.la);
+  rows_times_small<double, T>(Vb, n, n, K, Cin, K, nk, (T*)tg.b, tg.lb);
+  if (threadIdx.x == 0) *tg.rank = nk;
+}
+
+// ---------------------------------------------------------------------------
 // FADD: RkMatrix::formattedAddParts (rk_matrix.cpp:846-981)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1208,6 +1391,12 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
     dense_to_rk<T>(G, gr, gc, tr, V, V + (size_t)gc * gc, (int*)(V + (size_t)gc * gc + gc),
                    fabs(t.eps), tg, overflow, &flag);
     return;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (K <= kGramMaxK) {
+      gram_recompress<T>(t, kt, K, gws, sh, fabs(t.eps), overflow, &flag);
+      return;
+    }
   }
   // QR path (truncate, rk_matrix.cpp:490-585) on the concatenated panels
   // [target, alpha_i parts].  Three placements:
