@@ -404,7 +404,8 @@ constexpr int kJacobiMaxCols = 256;  // columns whose norms the Jacobi keeps in 
 // group.  Squared column norms are kept in 'nrm' (cols doubles of shared
 // scratch), recomputed at the start of every sweep and updated exactly by
 // each rotation (a' = a - t g, b' = b + t g), so a pair costs one reduction.
-// A pair is rotated while g^2 > 1e-26 a b and |g| > 1e-30 ||G||_F^2 (columns
+// A pair is rotated while g^2 > tol2 a b (tol2 1e-26 by default) and
+// |g| > 1e-30 ||G||_F^2 (columns
 // at rounding level of ||G|| carry nothing for the truncation thresholds used
 // here).  Returns the number of sweeps.  'sh2' is 2 + kWaves shared doubles of
 // scratch (workgroup team only); 'nrm' holds cols doubles or is null.
@@ -435,7 +436,7 @@ __device__ inline void jacobi_rot(double a, double b, double c2, double& t, doub
 
 template <int kLanes>
 __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double* V, int ldv,
-                                  double* sh2, double* nrm, int max_sweeps) {
+                                  double* sh2, double* nrm, int max_sweeps, double tol2 = 1e-26) {
   const int tid = team_tid<kLanes>();
   if (V != nullptr)
     for (int e = tid; e < cols * cols; e += kLanes) {
@@ -446,7 +447,6 @@ __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double
     team_sync<kLanes>();
     return 0;
   }
-  const double tol2 = 1e-26;
   const int np = (cols + 1) & ~1, rounds = np - 1, half = np / 2;
   const int g = group_lanes(half, kLanes), ngrp = kLanes / g, grp = tid / g, sub = tid % g;
   // without scratch for every column the norms are recomputed pair by pair
@@ -577,7 +577,7 @@ __device__ inline double group_sum_c(double v) {
 // singular vectors from the input, as G_in^T U.
 template <int kLanes, int R, int kG, bool kV>
 __device__ inline int team_jacobi_rows(double* G, int ldg, int rows, int cols, double* V, int ldv,
-                                       double* sh2, double* nrm, int max_sweeps) {
+                                       double* sh2, double* nrm, int max_sweeps, double tol2 = 1e-26) {
   const int tid = team_tid<kLanes>();
   if constexpr (kV) {
     for (int e = tid; e < cols * cols; e += kLanes) {
@@ -589,7 +589,6 @@ __device__ inline int team_jacobi_rows(double* G, int ldg, int rows, int cols, d
     team_sync<kLanes>();
     return 0;
   }
-  const double tol2 = 1e-26;
   const int np = (cols + 1) & ~1, rounds = np - 1, half = np / 2;
   constexpr int ngrp = kLanes / kG;
   const int grp = tid / kG, sub = tid % kG;
@@ -712,14 +711,14 @@ __device__ inline int team_jacobi_rows(double* G, int ldg, int rows, int cols, d
 // accumulation of V.  Shapes outside the instantiated ones use team_jacobi.
 template <int kLanes, bool kV>
 __device__ inline int team_jacobi_sel(double* G, int ldg, int rows, int cols, double* V, int ldv,
-                                      double* sh2, double* nrm, int max_sweeps) {
+                                      double* sh2, double* nrm, int max_sweeps, double tol2 = 1e-26) {
   const int half = ((cols + 1) & ~1) / 2;
   const int g = group_lanes(half, kLanes);
   const int span = kV ? (rows > cols ? rows : cols) : rows;
   const int need = (span + g - 1) / g;
 #define HMX_JR(G_, R_)                                                                             \
   if (g == G_ && need <= R_)                                                                       \
-    return team_jacobi_rows<kLanes, R_, G_, kV>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+    return team_jacobi_rows<kLanes, R_, G_, kV>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps, tol2);
   if constexpr (kLanes == kWave) {
     HMX_JR(16, 1)
     HMX_JR(8, 2)
@@ -730,34 +729,35 @@ __device__ inline int team_jacobi_sel(double* G, int ldg, int rows, int cols, do
     HMX_JR(8, 8)
   }
 #undef HMX_JR
-  return team_jacobi<kLanes>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+  return team_jacobi<kLanes>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps, tol2);
 }
 
 template <int kLanes>
 __device__ inline int team_jacobi_any(double* G, int ldg, int rows, int cols, double* V, int ldv,
-                                      double* sh2, double* nrm, int max_sweeps) {
+                                      double* sh2, double* nrm, int max_sweeps, double tol2 = 1e-26) {
   if (nrm != nullptr && cols <= kJacobiMaxCols && cols >= 2 && rows >= 1) {
-    if (V != nullptr) return team_jacobi_sel<kLanes, true>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
-    return team_jacobi_sel<kLanes, false>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+    if (V != nullptr)
+      return team_jacobi_sel<kLanes, true>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps, tol2);
+    return team_jacobi_sel<kLanes, false>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps, tol2);
   }
-  return team_jacobi<kLanes>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+  return team_jacobi<kLanes>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps, tol2);
 }
 
 // Workgroup version used by the existing call sites (cols <= kJacobiMaxCols);
 // V == nullptr leaves V out.
 __device__ inline void jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
-                                  int* /*flag*/, int max_sweeps = 40) {
+                                  int* /*flag*/, int max_sweeps = 40, double tol2 = 1e-26) {
   __shared__ double sh2[2 + kWaves];
   __shared__ double nrm[kJacobiMaxCols];
-  team_jacobi_any<kBlock>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps);
+  team_jacobi_any<kBlock>(G, ldg, rows, cols, V, ldv, sh2, nrm, max_sweeps, tol2);
 }
 
 // Single-wavefront version (G, V in LDS; cols <= kJacobiMaxCols); V == nullptr
 // leaves V out.
 __device__ inline int wave_jacobi_svd(double* G, int ldg, int rows, int cols, double* V, int ldv,
-                                      int max_sweeps = 40) {
+                                      int max_sweeps = 40, double tol2 = 1e-26) {
   __shared__ double nrm[kJacobiMaxCols];
-  return team_jacobi_any<kWave>(G, ldg, rows, cols, V, ldv, nullptr, nrm, max_sweeps);
+  return team_jacobi_any<kWave>(G, ldg, rows, cols, V, ldv, nullptr, nrm, max_sweeps, tol2);
 }
 
 // Column norms of G into sig[0..cols), then order[] = permutation sorting

@@ -139,6 +139,16 @@ class Matrix {
     int64_t x11, kk;  // offsets (elements) into hodlr_buf
   };
   std::vector<HodlrNode> hodlr_nodes;
+  // dense inverses of the LLt / LDLt factor's diagonal blocks of at most
+  // kInvMaxRows rows (built at the first solve, in the factor's precision):
+  // the solves multiply by them instead of substituting leaf by leaf
+  struct InvBlock {
+    const Block* b;
+    int64_t off;  // elements into inv_buf; m x m, ld m
+  };
+  std::vector<InvBlock> inv_blocks;
+  std::map<const Block*, int> inv_index;
+  DevBuf inv_buf;
   std::map<const Block*, int> hodlr_index;
   DevBuf hodlr_buf, hodlr_logdet;
   int overflow = 0;

@@ -47,6 +47,12 @@ constexpr int64_t kDiagBuf = kTempBase + (int64_t(1) << 31);
 constexpr int kShadowLeaf = 1 << 28;
 // the r x r blocks of the HODLR nodes (two per node)
 constexpr int64_t kHodlrBase = int64_t(1) << 38;
+// the dense inverses of the factor's diagonal blocks, one id per block
+constexpr int64_t kInvBase = kTempBase + (int64_t(1) << 32);
+// largest diagonal block of a factor that the solves apply through its
+// dense inverse (at N = 200k, leaf 128: blocks of 781 rows, 256 per sweep
+// instead of 2048 leaf substitutions)
+constexpr int kInvMaxRows = 1024;
 
 // Dense view; n may be dynamic (rank slot dn).
 struct DV {
@@ -1204,8 +1210,92 @@ class Exec {
   }
 
   // ---- triangular solves -----------------------------------------------------------
+  // ---- dense inverses of diagonal blocks (solves) ------------------------------------
+  DV inv_dv(int q) const {
+    const Matrix::InvBlock& ib = M_.inv_blocks[q];
+    const int m = ib.b->m();
+    DV d;
+    d.p = M_.inv_buf.template as<T>() + ib.off;
+    d.ld = std::max(m, 1);
+    d.m = m;
+    d.n = m;
+    d.buf = kInvBase + q;
+    d.row0 = ib.b->rows->offset;
+    return d;
+  }
+  // L_B^{-1} of every listed block B: the block's own forward substitution
+  // on the identity (blocks are independent, so their tasks share waves)
+  void build_inverses() {
+    use_inv_ = false;
+    for (size_t q = 0; q < M_.inv_blocks.size(); ++q) {
+      const Block* b = M_.inv_blocks[q].b;
+      DV I = inv_dv((int)q);
+      FTask t{};
+      t.type = T_ZERO;
+      t.mode = 1;
+      t.m = dim(I.m);
+      t.n = dim(I.n);
+      t.C = I.p;
+      t.ldc = I.ld;
+      wr(I);
+      emit(t);
+      solve_lower_left(b, I, dim(b->m()));
+      maybe_flush();
+    }
+  }
+  void use_inverses(bool v) { use_inv_ = v; }
+  // X <- L_B^{-1} X (trans: L_B^{-T} X) through the dense inverse: partial
+  // products over 256-column slices of the inverse into a temporary, summed
+  // into X (the sum runs after every product has read X)
+  void inverse_apply(int q, const DV& X, Dim ncol, bool trans) {
+    DV A = inv_dv(q);
+    A.trans = trans;
+    const int m = A.m, nc = std::max(ncol.s, 1);
+    constexpr int kSlice = 256;
+    const int nch = (m + kSlice - 1) / kSlice;
+    DV tmp = temp_dense(m, nc * nch, -1);
+    std::vector<DV> slice(nch);
+    for (int k = 0; k < nch; ++k) {
+      const int p0 = k * kSlice, pc = std::min(kSlice, m - p0);
+      DV As = A;
+      As.p = (T*)A.p + (trans ? (size_t)p0 : (size_t)p0 * A.ld);
+      DV Xs = rows_of(X, p0, pc);
+      slice[k] = tmp;
+      slice[k].p = (T*)tmp.p + (size_t)k * m * nc;
+      slice[k].n = nc;
+      slice[k].buf = kTempBase + fs_.next_temp++;
+      gemm_task(slice[k], false, 1.0, As, Xs, dim(m), ncol, dim(pc), A, Xs, slice[k]);
+    }
+    FTask t{};
+    t.type = T_SUMK;
+    t.mode = 0;
+    t.m = dim(m);
+    t.n = ncol;
+    t.p = dim(nch);
+    t.A = tmp.p;
+    t.lda = m;
+    t.ldb = m * nc;
+    t.C = X.p;
+    t.ldc = X.ld;
+    for (const DV& d : slice) rd(d);
+    rd(X);
+    wr(X);
+    emit(t);
+    release();
+  }
+  int inv_index(const Block* b) const {
+    if (!use_inv_) return -1;
+    auto it = M_.inv_index.find(b);
+    return it == M_.inv_index.end() ? -1 : it->second;
+  }
+
   // HMatrix::solveLowerTriangularLeft(ScalarArray) h_matrix.cpp:2338-2372
   void solve_lower_left(const Block* L, const DV& X, Dim ncol) {
+    const int q = inv_index(L);
+    if (q >= 0) {
+      inverse_apply(q, X, ncol, false);
+      return;
+    }
     if (L->is_leaf()) {
       trsm(1, L, X, ncol);
       return;
@@ -1228,6 +1318,11 @@ class Exec {
 
   // HMatrix::solveUpperTriangularLeft(ScalarArray) h_matrix.cpp:2504-2540 (Uplo LOWER)
   void solve_upper_left(const Block* L, const DV& X, Dim ncol) {
+    const int q = inv_index(L);
+    if (q >= 0) {
+      inverse_apply(q, X, ncol, true);
+      return;
+    }
     if (L->is_leaf()) {
       trsm(2, L, X, ncol);
       return;
@@ -2108,6 +2203,7 @@ class Exec {
   int max_wave_ = 0;
   PlanList* record_ = nullptr;
   bool lu_mode_ = false, ldlt_mode_ = false;
+  bool use_inv_ = false;
   double t_alloc_ = 0.0, t_seg_ = 0.0;
   struct Shadow {
     DV d;
@@ -2135,6 +2231,9 @@ void Matrix::factorize(int algo, bool fp32) {
        algo == hmat_factorization_hodlrsym) && !lower_sym)
     throw std::runtime_error("factorize: LLt, LDLt and HODLR need lower-symmetric storage");
   if (algo == hmat_factorization_hodlrsym) hodlr_layout();
+  inv_blocks.clear();
+  inv_index.clear();
+  inv_buf.release();
   if (algo == hmat_factorization_lu && lower_sym)
     throw std::runtime_error("factorize: LU needs non-symmetric storage");
   if (algo == hmat_factorization_lu && rows->n != cols->n)
@@ -2221,12 +2320,36 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part)
   auto it = fs.solve_plans.find(key);
   if (it == fs.solve_plans.end()) fs.solve_rhs.erase(key);
   DevBuf& rhs = fs.solve_rhs[key];
+  const bool inverses = factorization == hmat_factorization_llt || factorization == hmat_factorization_ldlt;
+  if (it == fs.solve_plans.end() && inverses && inv_blocks.empty()) {
+    // the dense inverses of the diagonal blocks of at most kInvMaxRows rows
+    std::function<void(const Block*)> pick = [&](const Block* b) {
+      if (b->m() <= kInvMaxRows || b->is_leaf()) {
+        inv_index[b] = (int)inv_blocks.size();
+        inv_blocks.push_back(InvBlock{b, 0});
+        return;
+      }
+      for (int i = 0; i < b->nrow; ++i) pick(b->get(i, i));
+    };
+    pick(bt->root);
+    int64_t off = 0;
+    for (InvBlock& ib : inv_blocks) {
+      ib.off = off;
+      off += (int64_t)ib.b->m() * ib.b->m();
+    }
+    inv_buf.alloc(sizeof(T) * (size_t)std::max<int64_t>(off, 1));
+    Exec<T> bx(*this, const_cast<Store<T>&>(st), fs);
+    bx.build_inverses();
+    bx.flush(false);
+    bx.check_flags();
+  }
   if (it == fs.solve_plans.end()) {
     // record the forward + backward substitution once for this nrhs
     fs.solve_rhs[key].ensure(bytes);
     PlanList plans;
     Exec<T> ex(*this, const_cast<Store<T>&>(st), fs);
     ex.record_into(&plans);
+    ex.use_inverses(inverses);
     DV X;
     X.p = fs.solve_rhs[key].p;
     X.ld = n;

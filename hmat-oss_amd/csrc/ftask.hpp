@@ -20,7 +20,7 @@ enum TaskType : int {
   T_FADD_DENSE,   // Rk target += alpha * dense part, eval + SVD (mode 1: target := alpha * part)
   T_RKRK,         // Rk product core: (A1 B1^T)(A2 B2^T)
   T_ACAFULL,      // dense -> Rk by full-pivot ACA
-  T_ZERO,         // C = 0
+  T_ZERO,         // C = 0 (mode 1: C = I)
   T_AXPY,         // C += alpha op(A) (mode 1: C = alpha op(A))
   T_GETRF,        // A <- P L U in place (L unit lower), row interchanges into B (int*)
   T_SUMK,         // C (+)= sum over p.s slices of A (ld lda, slice stride ldb): split-K GEMM

@@ -846,7 +846,9 @@ __global__ void k_zero(const FTask* tasks, const int* fail) {
   const FTask& t = tasks[blockIdx.x];
   T* C = (T*)t.C;
   const int m = t.m.s, n = t.n.s;
-  for (int e = threadIdx.x; e < m * n; e += blockDim.x) C[e % m + (size_t)(e / m) * t.ldc] = (T)0;
+  // mode 1: the identity
+  for (int e = threadIdx.x; e < m * n; e += blockDim.x)
+    C[e % m + (size_t)(e / m) * t.ldc] = (t.mode == 1 && e % m == e / m) ? (T)1 : (T)0;
 }
 
 template <typename T>
@@ -1141,6 +1143,7 @@ __device__ void qr_finish(const double* Va, int lda, int m, const double* Vb, in
 // ---------------------------------------------------------------------------
 constexpr int kGramMaxK = 32;
 constexpr double kGramTol = 1e-13;
+constexpr double kGramJacobiTol2 = 1e-16;
 
 // Pivoted Cholesky of G (K x K, ld K, symmetric, K <= 64) by one wavefront:
 // returns the rank r; rows 0..r-1 of the upper triangle hold R in the pivoted
@@ -1255,11 +1258,13 @@ __device__ void gram_recompress(const FTask& t, int kt, int K, double* gws, doub
     Cin[e] = s;
   }
   __syncthreads();
+  // columns orthogonal to 1e-8 of their norms: far below the FP32 data's
+  // rounding once scaled by the eps cut
   if (rb <= kWaveJacobiMax) {
-    if (wid == 0) wave_jacobi_svd(Cm, ra, ra, rb, nullptr, 0);
+    if (wid == 0) wave_jacobi_svd(Cm, ra, ra, rb, nullptr, 0, 40, kGramJacobiTol2);
     __syncthreads();
   } else {
-    jacobi_svd(Cm, ra, ra, rb, nullptr, 0, flag);
+    jacobi_svd(Cm, ra, ra, rb, nullptr, 0, flag, 40, kGramJacobiTol2);
   }
   sorted_norms(Cm, ra, ra, rb, sig, order);
   int nk = find_k(sig, order, rb, eps);
