@@ -180,7 +180,11 @@ long g_type_launches[32];
 template <typename T>
 void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool capturing = false,
                   bool single_stream = false) {
-  fs.ws.ensure((size_t)std::max<int64_t>(P.ws, 1) * sizeof(double));
+  const size_t ws_bytes = (size_t)std::max<int64_t>(P.ws, 1) * sizeof(double);
+  // segments run asynchronously: a workspace still used by the previous one
+  // is only replaced once the stream has drained
+  if (ws_bytes > fs.ws.bytes && fs.ws.p && !capturing) HMX_CHECK(hipStreamSynchronize(s));
+  fs.ws.ensure(ws_bytes);
   // a fill kernel rather than a memset node, so that the plan can be captured
   if (P.ranks_used) launch_fill_int(fs.ranks.as<int>(), (size_t)P.ranks_used, 0, s);
   const FTask* dt = P.tasks.as<FTask>();
@@ -277,6 +281,10 @@ class Exec {
     HMX_CHECK(hipMemsetAsync(fs_.flags.p, 0, 2 * sizeof(int), M.stream));
     reset_temps();
     fs_.rank_used = 0;
+  }
+
+  ~Exec() {
+    if (!inflight_.empty()) (void)hipStreamSynchronize(M_.stream);
   }
 
   void reset_temps() {
@@ -2069,7 +2077,6 @@ class Exec {
       throw std::runtime_error("task segment limit reached while temporaries are live");
     if (tasks_.empty()) return;
     run_segment();
-    check_flags();  // stop at the first failing pivot, as potrf's info > 0 does
     tasks_.clear();
     waves_.clear();
     units_.clear();
@@ -2138,9 +2145,16 @@ class Exec {
       ws_max = std::max(ws_max, L.overlap ? wave_ws : ws);
       launches.push_back(L);
     }
-    upload(plan->tasks, sorted.data(), n * sizeof(FTask), s);
+    // the task lists go up with a synchronous copy on the null stream, which
+    // does not wait for the matrix's (non-blocking) stream: the host builds the
+    // next segment while the device still runs this one
+    auto put = [](DevBuf& b, const void* src, size_t bytes) {
+      b.ensure(bytes);
+      HMX_CHECK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    put(plan->tasks, sorted.data(), n * sizeof(FTask));
     if (!tiles.empty()) {
-      upload(plan->tiles, tiles.data(), tiles.size() * sizeof(GemmTile), s);
+      put(plan->tiles, tiles.data(), tiles.size() * sizeof(GemmTile));
     } else {
       plan->tiles.ensure(sizeof(GemmTile));
       HMX_CHECK(hipMemsetAsync(plan->tiles.p, 0, sizeof(GemmTile), s));
@@ -2150,8 +2164,12 @@ class Exec {
     plan->ntasks = (long)n;
     plan->waves = W;
     execute_plan<T>(*plan, fs_, s, false, M_.single_stream);
-    HMX_CHECK(hipStreamSynchronize(s));
-    if (record_) record_->push_back(plan);
+    if (record_) {
+      HMX_CHECK(hipStreamSynchronize(s));
+      record_->push_back(plan);
+    } else {
+      inflight_.push_back(plan);  // kept until check_flags() has waited for the stream
+    }
     M_.stats.tasks += (long)n;
     M_.stats.waves += W;
     M_.stats.launches += (long)launches.size();
@@ -2159,7 +2177,12 @@ class Exec {
     M_.stats.temp_bytes = std::max(M_.stats.temp_bytes, fs_.temp_used);
   }
 
+  // Waits for the segments in flight; a failing pivot (potrf's info > 0) or a
+  // capacity overflow seen by any of them raises here.  Launches after a
+  // failing pivot return at once (the failure flag), as the reference stops.
   void check_flags() {
+    HMX_CHECK(hipStreamSynchronize(M_.stream));
+    inflight_.clear();
     int f[2] = {0, 0};
     HMX_CHECK(hipMemcpy(f, fs_.flags.p, sizeof(f), hipMemcpyDeviceToHost));
     if (f[1] && lu_mode_) throw std::runtime_error("luDecomposition: getrf found a zero pivot (singular matrix)");
@@ -2188,7 +2211,10 @@ class Exec {
   }
 
  private:
-  static constexpr size_t kMaxTasks = 3000000;
+  // segments of at most this many tasks, so that the device starts early and
+  // the host builds the next segment while the device runs the current one
+  static constexpr size_t kMaxTasks = 250000;
+  std::vector<std::shared_ptr<SegmentPlan>> inflight_;
   Matrix& M_;
   Store<T>& st_;
   FactorState& fs_;
