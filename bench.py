@@ -81,6 +81,44 @@ def cpu_baseline_factor_solve(n=20000, height=1.6, nrhs=8):
         return time.time() - t
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def gpu_at_cpu_sample(n=20000, height=1.6, nrhs=8):
+    """The GPU on the CPU baseline's own sample (same points, kernel and
+    right-hand-side count): plain FP64 factor + solve, and FP32 factor +
+    refinement to 1e-10, assembly excluded."""
+    import torch
+    import hmat_amd
+    from oracle.hmat_oracle import cylinder_points
+    pts = cylinder_points(n, 1.0, height, 7)
+    tree = hmat_amd.ClusterTree(pts, 128)
+    b = np.random.default_rng(0).standard_normal((n, nrhs))
+    out = {}
+    for mixed in (False, True):
+        h = hmat_amd.HMatrix(tree)
+        h.assemble(length=0.5, nugget=1e-2, eps=1e-4)
+        bd = torch.tensor(b.T.copy(), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t = time.time()
+        h.factorize(hmat_amd.FACT_LLT, mixed=mixed)
+        if mixed:
+            h.solve_refined_device(bd.data_ptr(), nrhs, tol=1e-10, max_iter=40)
+        else:
+            h.solve_device(bd.data_ptr(), nrhs)
+        torch.cuda.synchronize()
+        out["fp32_ir_s" if mixed else "fp64_s"] = time.time() - t
+        h.close()
+    return out
+
+
 def time_gemv(h, n, reps):
     """Average duration of one FP64 H-gemv with one right-hand side (its four
     launches), timed with events on the stream the engine launches on."""
@@ -300,12 +338,16 @@ def main():
     cpu = None
     if not args.no_cpu:
         t_cpu = cpu_baseline_factor_solve(nrhs=nrhs)
-        cpu = {"value": t_cpu, "unit": "s", "cores": 1, "kind": "port",
+        g = gpu_at_cpu_sample(nrhs=nrhs)
+        cpu = {"value": t_cpu, "unit": "s", "cores": 1, "kind": "port", "cpu_model": cpu_model(),
                "sample": "oracle (numpy restatement, 1 BLAS thread) H-LLt factor + solve of 8 "
                          "RHS at N=20,000 on a cylinder of radius 1 and height 1.6 (the same "
                          "point density as the N=200k workload), same kernel, leaf 128, eta 2, "
-                         "eps 1e-4: a bounded sample, not the N=200k workload"}
-        log(f"cpu baseline sample {t_cpu:.1f}s")
+                         "eps 1e-4: a bounded sample, not the N=200k workload",
+               "gpu_same_sample": g,
+               "cpu_over_gpu_fp64": t_cpu / g["fp64_s"]}
+        log(f"cpu baseline sample {t_cpu:.1f}s; GPU on the same sample: FP64 {g['fp64_s']:.3f}s, "
+            f"FP32+IR {g['fp32_ir_s']:.3f}s")
 
     mean_s = float(np.mean(step_s))
     out = {
