@@ -50,9 +50,10 @@ constexpr int64_t kHodlrBase = int64_t(1) << 38;
 // the dense inverses of the factor's diagonal blocks, one id per block
 constexpr int64_t kInvBase = kTempBase + (int64_t(1) << 32);
 // largest diagonal block of a factor that the solves apply through its
-// dense inverse (at N = 200k, leaf 128: blocks of 781 rows, 256 per sweep
-// instead of 2048 leaf substitutions)
-constexpr int kInvMaxRows = 1024;
+// dense inverse (at N = 200k, leaf 128: blocks of 1,562 rows, 128 per sweep
+// instead of 2,048 leaf substitutions; measured against 1024: 0.1 s less
+// per FP32 + refinement step)
+constexpr int kInvMaxRows = 2048;
 
 // Dense view; n may be dynamic (rank slot dn).
 struct DV {
