@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "factor_kernels.hpp"
 #include "hmat/hmat.h"
 #include "hmat/hmat_mi355x.h"
 
@@ -815,5 +816,57 @@ int hmx_set_stream(hmat_matrix_t* h, void* s) {
 }
 
 void hmx_set_rank_capacity(int cap) { g_rank_cap = cap > 0 ? cap : 160; }
+
+int hmx_ftask_size(void) { return (int)sizeof(hmx::FTask); }
+
+int hmx_run_tasks(int precision, int type, const void* tasks, int count, int* flags_out) {
+  return guard([&] {
+    if (count <= 0) return;
+    if (type < 0 || type >= hmx::T_NTYPES) throw std::runtime_error("hmx_run_tasks: unknown task type");
+    std::vector<hmx::FTask> t((const hmx::FTask*)tasks, (const hmx::FTask*)tasks + count);
+    std::vector<hmx::GemmTile> tiles;
+    int64_t ws = 0;
+    for (int i = 0; i < count; ++i) {
+      hmx::FTask& x = t[i];
+      int64_t need = 0;
+      if (type == hmx::T_FADD) {
+        int kmax = x.tgt.cap;
+        for (int q = 0; q < x.nparts; ++q) kmax += x.parts[q].rk.cap;
+        need = hmx::fadd_workspace(x.tgt.m, x.tgt.n, kmax);
+      } else if (type == hmx::T_FADD_DENSE) {
+        need = hmx::fadd_dense_workspace(x.tgt.m, x.tgt.n);
+      } else if (type == hmx::T_RKRK) {
+        need = hmx::rkrk_workspace(x.parts[0].rk.cap, x.parts[1].rk.cap);
+      } else if (type == hmx::T_HODLR) {
+        need = hmx::hodlr_workspace(x.lda);
+      } else if (type == hmx::T_GEMM) {
+        for (int a = 0; a < (x.m.s + 63) / 64; ++a)
+          for (int b = 0; b < (x.n.s + 63) / 64; ++b) tiles.push_back(hmx::GemmTile{i, a, b});
+      }
+      x.ws = ws;
+      ws += need;
+    }
+    hipStream_t s = nullptr;
+    HMX_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{s};
+    hmx::DevBuf dt, dtl, dws, dfl;
+    hmx::upload(dt, t.data(), t.size() * sizeof(hmx::FTask), s);
+    if (!tiles.empty()) hmx::upload(dtl, tiles.data(), tiles.size() * sizeof(hmx::GemmTile), s);
+    dws.alloc(sizeof(double) * (size_t)std::max<int64_t>(ws, 1));
+    dfl.alloc(2 * sizeof(int));
+    HMX_CHECK(hipMemsetAsync(dfl.p, 0, 2 * sizeof(int), s));
+    if (precision == 0)
+      hmx::launch_task_kernel<double>(type, dt.as<hmx::FTask>(), count, dtl.as<hmx::GemmTile>(),
+                                      (int)tiles.size(), dws.as<double>(), dfl.as<int>(), s);
+    else
+      hmx::launch_task_kernel<float>(type, dt.as<hmx::FTask>(), count, dtl.as<hmx::GemmTile>(),
+                                     (int)tiles.size(), dws.as<double>(), dfl.as<int>(), s);
+    HMX_CHECK(hipStreamSynchronize(s));
+    if (flags_out) HMX_CHECK(hipMemcpy(flags_out, dfl.p, 2 * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
 
 }  // extern "C"

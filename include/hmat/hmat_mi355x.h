@@ -129,6 +129,15 @@ HMAT_API int hmx_set_stream(hmat_matrix_t *m, void *stream);
 /* Upper bound on Rk panel capacity used for new matrices (default 160). */
 HMAT_API void hmx_set_rank_capacity(int cap);
 
+/* Test hooks of the task kernels (no counterpart in hmat-oss): the size of
+ * one task record (hmx::FTask of the engine's ftask.hpp, which the tests
+ * mirror), and one batched launch of `count` records of one task type on
+ * device operands, as the factorisation executor launches them (GEMM tasks
+ * in 64 x 64 tiles, workspace placed per task).  precision 0: FP64 kernels,
+ * 1: FP32.  flags_out[0..1]: Rk capacity overflow, failing pivot. */
+HMAT_API int hmx_ftask_size(void);
+HMAT_API int hmx_run_tasks(int precision, int type, const void *tasks, int count, int *flags_out);
+
 #ifdef __cplusplus
 }
 #endif
