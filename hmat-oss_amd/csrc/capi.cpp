@@ -118,11 +118,25 @@ int assemble_generic(hmat_matrix_t* h, hmat_assemble_context_t* ctx) {
     spec.length = k->length;
     spec.nugget = k->nugget;
     spec.shift = k->kind == HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL ? k->shift : 0.0;
-    if (k->kind < 0 || k->kind > HMAT_MI355X_KERNEL_GAUSSIAN_DLENGTH)
+    if (k->kind < 0 || k->kind > HMAT_MI355X_KERNEL_HELMHOLTZ)
       throw std::runtime_error("hmat_mi355x_kernel_t: unknown kernel kind");
     spec.dim = k->dimension;
-    spec.n = k->n;
-    spec.points.assign(k->points, k->points + (size_t)k->n * k->dimension);
+    if (m->embedded) {
+      // point u is DoF 2u + c (c = 0, 1) of the embedded numbering
+      if (k->kind == HMAT_MI355X_KERNEL_HELMHOLTZ && !(k->regularization > 0.0))
+        throw std::runtime_error("hmat_mi355x_kernel_t: the Helmholtz kernel needs regularization > 0");
+      spec.wavenumber = k->wavenumber;
+      spec.reg = k->regularization;
+      spec.n = 2 * k->n;
+      spec.points.resize((size_t)spec.n * k->dimension);
+      for (int u = 0; u < k->n; ++u)
+        for (int c = 0; c < 2; ++c)
+          std::copy(k->points + (size_t)u * k->dimension, k->points + (size_t)(u + 1) * k->dimension,
+                    spec.points.begin() + (size_t)(2 * u + c) * k->dimension);
+    } else {
+      spec.n = k->n;
+      spec.points.assign(k->points, k->points + (size_t)k->n * k->dimension);
+    }
     m->assemble(spec, cs);
     if (ctx->factorization != hmat_factorization_none) m->factorize(ctx->factorization, false);
   });
@@ -425,6 +439,126 @@ void fill(hmat_interface_t* i) {
   i->extract_diagonal_block = u_diag_block;
 }
 
+// ---- complex (Z) entries --------------------------------------------------------
+// The engine's matrix is the 2 x 2 real representation of the complex one
+// (Matrix::create_complex).  A complex array of n entries (interleaved re, im)
+// is the real array of the embedded numbering, so vectors pass through as they
+// are; complex scalars with a nonzero imaginary part are applied on the host
+// (gemv) or refused where the real engine would have to take them.
+using zd = std::complex<double>;
+
+double real_scalar(const void* a, const char* what) {
+  const zd v = *static_cast<const zd*>(a);
+  if (v.imag() != 0.0)
+    throw std::runtime_error(std::string(what) +
+                             ": a complex scalar with a nonzero imaginary part is not provided");
+  return v.real();
+}
+
+Matrix* MZ(hmat_matrix_t* h) {
+  Matrix* m = M(h);
+  if (!m->embedded) throw std::runtime_error("not a complex matrix");
+  return m;
+}
+
+hmat_matrix_t* create_empty_z(const hmat_cluster_tree_t* r, const hmat_cluster_tree_t* c,
+                              int lower_sym, hmat_admissibility_t* cond) {
+  try {
+    if (lower_sym)
+      throw std::runtime_error("complex matrices are stored whole (H-LU); lower-symmetric storage "
+                               "of complex matrices is not provided");
+    hmx::AdmissibilityParams p;
+    if (cond) p = reinterpret_cast<Admissibility*>(cond)->p;
+    auto* m = Matrix::create_complex(reinterpret_cast<const hmx::ClusterTree*>(r),
+                                     reinterpret_cast<const hmx::ClusterTree*>(c), p, g_rank_cap);
+    return reinterpret_cast<hmat_matrix_t*>(m);
+  } catch (const std::exception& e) {
+    fail(e);
+    return nullptr;
+  }
+}
+
+int factorize_z(hmat_matrix_t* h, hmat_factorization_context_t* ctx) {
+  return guard([&] {
+    if (ctx->factorization != hmat_factorization_lu)
+      throw std::runtime_error("complex matrices: only hmat_factorization_lu is provided");
+    MZ(h)->factorize(ctx->factorization, false);
+  });
+}
+
+// y = alpha A x + beta y with complex alpha, beta: real ones go to the device
+// as they are, others through t = A x on the device and the combination on
+// the host
+void gemv_z_run(Matrix* m, char trans, const void* alpha, void* x, const void* beta, void* y,
+                int nrhs, bool permute) {
+  if (trans != 'N') throw std::runtime_error("gemv: only 'N' for complex matrices");
+  const zd a = *static_cast<const zd*>(alpha), b = *static_cast<const zd*>(beta);
+  if (a.imag() == 0.0 && b.imag() == 0.0) {
+    const double ar = a.real(), br = b.real();
+    m->gemv_host('N', &ar, x, &br, y, nrhs, permute);
+    return;
+  }
+  std::vector<zd> t((size_t)m->zrows->n * nrhs);
+  const double one = 1.0, zero = 0.0;
+  m->gemv_host('N', &one, x, &zero, t.data(), nrhs, permute);
+  zd* yz = static_cast<zd*>(y);
+  for (size_t i = 0; i < t.size(); ++i) yz[i] = a * t[i] + b * yz[i];
+}
+int gemv_z(char trans, void* alpha, hmat_matrix_t* h, void* x, void* beta, void* y, int nrhs) {
+  return guard([&] { gemv_z_run(MZ(h), trans, alpha, x, beta, y, nrhs, true); });
+}
+int gemm_scalar_z(char trans, void* alpha, hmat_matrix_t* h, void* x, void* beta, void* y, int nrhs) {
+  return guard([&] { gemv_z_run(MZ(h), trans, alpha, x, beta, y, nrhs, false); });
+}
+int gemm_dense_z(char trans_b, char trans_x, char side, const void* alpha, hmat_matrix_t* h,
+                 void* x, const void* beta, void* y, int nrhs) {
+  return guard([&] {
+    if (side != 'L' || trans_x != 'N')
+      throw std::runtime_error("gemm_dense: only side 'L' with trans_x 'N' is supported");
+    gemv_z_run(MZ(h), trans_b, alpha, x, beta, y, nrhs, false);
+  });
+}
+int scale_z(void* alpha, hmat_matrix_t* h) {
+  return guard([&] { MZ(h)->scale(real_scalar(alpha, "scale")); });
+}
+int add_identity_z(hmat_matrix_t* h, void* alpha) {
+  return guard([&] { MZ(h)->add_identity(real_scalar(alpha, "add_identity")); });
+}
+int axpy_z(void* a, hmat_matrix_t* x, hmat_matrix_t* y) {
+  return guard([&] { MZ(y)->axpy_from(real_scalar(a, "axpy"), *MZ(x)); });
+}
+int gemm_z(char ta, char tb, const void* alpha, hmat_matrix_t* a, hmat_matrix_t* b,
+           const void* beta, hmat_matrix_t* c) {
+  return guard([&] {
+    if (ta != 'N' || tb != 'N') throw std::runtime_error("gemm: only 'N', 'N' for complex matrices");
+    MZ(c)->gemm_from('N', 'N', real_scalar(alpha, "gemm"), *MZ(a), *MZ(b), real_scalar(beta, "gemm"));
+  });
+}
+int get_cluster_trees_z(hmat_matrix_t* h, const hmat_cluster_tree_t** r, const hmat_cluster_tree_t** c) {
+  if (r) *r = reinterpret_cast<const hmat_cluster_tree_t*>(M(h)->zrows);
+  if (c) *c = reinterpret_cast<const hmat_cluster_tree_t*>(M(h)->zcols);
+  return 0;
+}
+int destroy_z(hmat_matrix_t* h) {
+  Matrix* m = M(h);
+  const hmx::ClusterTree* r = m->own_rows ? m->zrows : nullptr;
+  const hmx::ClusterTree* c = m->own_cols ? m->zcols : nullptr;
+  delete m;
+  if (r) delete r;
+  if (c && c != r) delete c;
+  return 0;
+}
+int u_logdet_z(hmat_matrix_t*, double*) { return unsupported("logdet of a complex matrix"); }
+int u_diag_z(hmat_matrix_t*, void*, int) { return unsupported("extract_diagonal of a complex matrix"); }
+int u_tri_z(hmat_matrix_t*, int, void*, int) {
+  return unsupported("solve_lower_triangular of a complex matrix (LLt only)");
+}
+hmat_matrix_t* u_read_z(hmat_iostream, void*) {
+  unsupported("read_struct of a complex matrix");
+  return nullptr;
+}
+void u_write_z(hmat_matrix_t*, hmat_iostream, void*) { unsupported("serialization of a complex matrix"); }
+
 }  // namespace
 
 extern "C" {
@@ -433,13 +567,40 @@ void hmat_init_mi355x_interface(hmat_interface_t* i, hmat_value_t type) {
   switch (type) {
     case HMAT_SIMPLE_PRECISION: fill<HMAT_SIMPLE_PRECISION, float>(i); break;
     case HMAT_DOUBLE_PRECISION: fill<HMAT_DOUBLE_PRECISION, double>(i); break;
+    case HMAT_DOUBLE_COMPLEX:
+      // the real FP64 table, with the entries that see complex scalars,
+      // vectors or trees replaced
+      fill<HMAT_DOUBLE_PRECISION, double>(i);
+      i->value_type = type;
+      i->create_empty_hmatrix_admissibility = create_empty_z;
+      i->factorize_generic = factorize_z;
+      i->destroy = destroy_z;
+      i->gemv = gemv_z;
+      i->gemm_scalar = gemm_scalar_z;
+      i->gemm_dense = gemm_dense_z;
+      i->scale = scale_z;
+      i->add_identity = add_identity_z;
+      i->axpy = axpy_z;
+      i->gemm = gemm_z;
+      i->vector_reorder = vector_order<zd, false>;
+      i->vector_restore = vector_order<zd, true>;
+      i->get_cluster_trees = get_cluster_trees_z;
+      i->logdet = u_logdet_z;
+      i->extract_diagonal = u_diag_z;
+      i->solve_lower_triangular = u_tri_z;
+      i->solve_lower_triangular_dense = u_tri_z;
+      i->read_struct = u_read_z;
+      i->write_struct = u_write_z;
+      i->read_data = u_write_z;
+      i->write_data = u_write_z;
+      break;
     default:
-      // complex types: the table is filled, operations report the limitation
+      // single-precision complex: the table is filled, creation reports the limitation
       fill<HMAT_DOUBLE_PRECISION, double>(i);
       i->value_type = type;
       i->create_empty_hmatrix_admissibility = [](const hmat_cluster_tree_t*, const hmat_cluster_tree_t*, int,
                                                  hmat_admissibility_t*) -> hmat_matrix_t* {
-        unsupported("complex arithmetic");
+        unsupported("single-precision complex arithmetic");
         return nullptr;
       };
   }
@@ -451,7 +612,13 @@ void hmat_mi355x_kernel_compute(void* ctx, int row, int col, void* result) {
   for (int q = 0; q < k->dimension && q < 3; ++q)
     d[q] = k->points[(size_t)row * k->dimension + q] - k->points[(size_t)col * k->dimension + q];
   hmx::KernelParams kp{k->kind, k->length, k->nugget,
-                      k->kind == HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL ? k->shift : 0.0};
+                      k->kind == HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL ? k->shift : 0.0,
+                      k->wavenumber, k->regularization};
+  if (k->kind == HMAT_MI355X_KERNEL_HELMHOLTZ) {
+    // a complex value: (re, im)
+    hmx::helmholtz_value(kp, d[0], d[1], d[2], row == col, (double*)result, (double*)result + 1);
+    return;
+  }
   *(double*)result = hmx::kernel_value(kp, d[0], d[1], d[2], row == col);
 }
 

@@ -29,15 +29,44 @@ enum KernelKind : int {
   kGaussian = 1,
   kShiftedExponential = 2,
   kExponentialDLength = 3,  // d/dl of exp(-r / l): (r / l^2) exp(-r / l)
-  kGaussianDLength = 4      // d/dl of exp(-r^2 / l^2): (2 r^2 / l^3) exp(-r^2 / l^2)
+  kGaussianDLength = 4,     // d/dl of exp(-r^2 / l^2): (2 r^2 / l^3) exp(-r^2 / l^2)
+  kHelmholtz = 5            // complex, see helmholtz_embedded
 };
 
 struct KernelParams {
   int kind;
   double length;
   double nugget;
-  double shift;  // kShiftedExponential: z offset of the source (non-symmetric kernel)
+  double shift;             // kShiftedExponential: z offset of the source (non-symmetric kernel)
+  double wavenumber = 0.0;  // kHelmholtz: k
+  double reg = 0.0;         // kHelmholtz: h, the distance regularisation
 };
+
+// Regularised Helmholtz kernel z = exp(i k r) / (4 pi r), r = sqrt(d^2 + h^2),
+// and 1 / (4 pi h) + nugget on the diagonal (same point).
+__host__ __device__ inline void helmholtz_value(const KernelParams& k, double dx, double dy,
+                                                double dz, bool same, double* re, double* im) {
+  constexpr double kFourPi = 12.566370614359172;
+  if (same) {
+    *re = 1.0 / (kFourPi * k.reg) + k.nugget;
+    *im = 0.0;
+    return;
+  }
+  const double r = sqrt(dx * dx + dy * dy + dz * dz + k.reg * k.reg);
+  const double s = 1.0 / (kFourPi * r);
+  *re = cos(k.wavenumber * r) * s;
+  *im = sin(k.wavenumber * r) * s;
+}
+
+// Entry (2u + a, 2v + b) of the 2 x 2 real representation [[Re z, -Im z],
+// [Im z, Re z]] of the complex entry z = K(u, v): ur = 2u + a, uc = 2v + b.
+__host__ __device__ inline double helmholtz_embedded(const KernelParams& k, double dx, double dy,
+                                                     double dz, int ur, int uc) {
+  double re, im;
+  helmholtz_value(k, dx, dy, dz, (ur >> 1) == (uc >> 1), &re, &im);
+  const int a = ur & 1, b = uc & 1;
+  return a == b ? re : (a ? im : -im);
+}
 
 __host__ __device__ inline double kernel_value(const KernelParams& k, double dx, double dy,
                                                double dz, bool same) {

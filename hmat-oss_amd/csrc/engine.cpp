@@ -60,6 +60,25 @@ Matrix::Matrix(const ClusterTree* r, const ClusterTree* c, bool sym, std::unique
   build_storage(&cap_hint);
 }
 
+Matrix* Matrix::create_complex(const ClusterTree* r, const ClusterTree* c,
+                               const AdmissibilityParams& adm, int rank_cap,
+                               std::shared_ptr<ClusterTree> er, std::shared_ptr<ClusterTree> ec) {
+  if (!er) er = std::shared_ptr<ClusterTree>(embed_cluster_tree(*r));
+  if (!ec) ec = c == r ? er : std::shared_ptr<ClusterTree>(embed_cluster_tree(*c));
+  auto zt = build_block_tree(*r, *c, adm, false);
+  auto et = embed_block_tree(*zt, *r, *er, *c, *ec);
+  // a complex rank k is a real rank 2k in the embedding
+  auto* m = new Matrix(er.get(), ec.get(), false, std::move(et), HMAT_DOUBLE_PRECISION,
+                       2 * rank_cap, {});
+  m->adm = adm;
+  m->embedded = true;
+  m->zrows = r;
+  m->zcols = c;
+  m->erows = er;
+  m->ecols = ec;
+  return m;
+}
+
 void Matrix::device_init() {
   if (own_stream) return;
   HMX_CHECK(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
@@ -151,7 +170,10 @@ void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
   if (cols != rows) upload_points(k, *cols, pts_cols, user_cols, stream);
   PointSet rp{pts_rows.as<double>(), user_rows.as<int>(), rows->n};
   PointSet cp = (cols == rows) ? rp : PointSet{pts_cols.as<double>(), user_cols.as<int>(), cols->n};
-  KernelParams kp{k.kind, k.length, k.nugget, k.shift};
+  KernelParams kp{k.kind, k.length, k.nugget, k.shift, k.wavenumber, k.reg};
+  if (embedded != (k.kind == kHelmholtz))
+    throw std::runtime_error(embedded ? "assembly: a complex matrix needs the complex (Helmholtz) kernel"
+                                      : "assembly: the Helmholtz kernel needs a complex matrix");
   init_store(s64);
   HMX_CHECK(hipMemsetAsync(overflow_.p, 0, sizeof(int), stream));
 
@@ -161,6 +183,12 @@ void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
     const LeafDesc& L = s64.hdesc[i];
     if (L.kind == 0) {
       full.push_back((int)i);
+    } else if (embedded) {
+      // complex blocks (twice the rows and columns in the embedding) go through
+      // ACA and the truncation at any size
+      if (comp.method == 0)
+        throw std::runtime_error("SVD compression of complex blocks is not provided; use ACA");
+      aca.push_back((int)i);
     } else if (std::max(L.m, L.n) < 100 || comp.method == 0 /*svd*/) {
       if (std::max(L.m, L.n) >= 100)
         throw std::runtime_error("SVD compression is implemented for blocks below 100 rows/cols");
@@ -232,15 +260,17 @@ void Matrix::finish_assembly(double t0) {
 
 namespace {
 // One leaf block (rows R, cols C) from the caller's host functions, column-major
-// with ld m (SimpleFunction::assemble / BlockFunction::assemble, assembly.cpp:75-185).
+// with ld m (SimpleFunction::assemble / BlockFunction::assemble, assembly.cpp:75-185);
+// w doubles per entry (2: complex).
 void evaluate_block(const HostFunctions& f, const ClusterTree& rt, const ClusterTree& ct,
-                    const Cluster* R, const Cluster* C, double* out) {
+                    const Cluster* R, const Cluster* C, double* out, int w = 1) {
   const int m = R->size, n = C->size;
   if (f.simple) {
     const int* ri = rt.indices.data() + R->offset;
     const int* ci = ct.indices.data() + C->offset;
     for (int j = 0; j < n; ++j)
-      for (int i = 0; i < m; ++i) f.simple(f.user_context, ri[i], ci[j], out + i + (size_t)j * m);
+      for (int i = 0; i < m; ++i)
+        f.simple(f.user_context, ri[i], ci[j], out + w * (i + (size_t)j * m));
     return;
   }
   hmat_block_info_t info;
@@ -255,7 +285,7 @@ void evaluate_block(const HostFunctions& f, const ClusterTree& rt, const Cluster
   prepare();
   if (info.needed_memory != 0) prepare();  // BlockFunction::prepareBlock calls it again
   if (info.block_type == hmat_block_null) {
-    std::memset(out, 0, sizeof(double) * (size_t)m * n);
+    std::memset(out, 0, sizeof(double) * w * (size_t)m * n);
   } else if (f.advanced) {
     hmat_block_compute_context_t ac;
     ac.block = out;
@@ -270,6 +300,31 @@ void evaluate_block(const HostFunctions& f, const ClusterTree& rt, const Cluster
     f.block(info.user_data, 0, m, 0, n, out);
   }
   if (info.release_user_data) info.release_user_data(info.user_data);
+}
+
+// The block of a complex matrix: the caller's complex block over the complex
+// clusters (half the embedded offsets and sizes), expanded to its 2 x 2 real
+// representation (2m x 2n, ld 2m).
+void evaluate_block_complex(const HostFunctions& f, const ClusterTree& zr, const ClusterTree& zc,
+                            const Cluster* R, const Cluster* C, double* out,
+                            std::vector<double>& z) {
+  Cluster r, c;
+  r.offset = R->offset / 2;
+  r.size = R->size / 2;
+  c.offset = C->offset / 2;
+  c.size = C->size / 2;
+  const int m = r.size, n = c.size;
+  z.resize(2 * (size_t)m * n);
+  evaluate_block(f, zr, zc, &r, &c, z.data(), 2);
+  const size_t ld = 2 * (size_t)m;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < m; ++i) {
+      const double re = z[2 * (i + (size_t)j * m)], im = z[2 * (i + (size_t)j * m) + 1];
+      out[2 * i + 2 * j * ld] = re;
+      out[2 * i + 1 + 2 * j * ld] = im;
+      out[2 * i + (2 * j + 1) * ld] = -im;
+      out[2 * i + 1 + (2 * j + 1) * ld] = re;
+    }
 }
 }  // namespace
 
@@ -287,7 +342,7 @@ void Matrix::assemble_host(const HostFunctions& f, const CompressionSpec& comp) 
   const int64_t kStageElems = (int64_t)1 << 26;  // 512 MB of doubles
   const int64_t aca_budget = (int64_t)4 << 27;   // doubles of ACA workspace per launch
   const int nleaf = (int)bt->leaves.size();
-  std::vector<double> host;
+  std::vector<double> host, zbuf;
   DevBuf table, ws, dl, doff, dt;
   int l0 = 0;
   while (l0 < nleaf) {
@@ -307,10 +362,19 @@ void Matrix::assemble_host(const HostFunctions& f, const CompressionSpec& comp) 
     for (int l = l0; l < l1; ++l) {
       const Block* b = bt->leaves[l];
       const LeafDesc& L = s64.hdesc[l];
-      evaluate_block(f, *rows, *cols, b->rows, b->cols, host.data() + off);
+      if (embedded)
+        evaluate_block_complex(f, *zrows, *zcols, b->rows, b->cols, host.data() + off, zbuf);
+      else
+        evaluate_block(f, *rows, *cols, b->rows, b->cols, host.data() + off);
       if (L.kind == 0) {
         full.push_back(l);
         full_off.push_back(off);
+      } else if (embedded) {
+        if (comp.method == 0)
+          throw std::runtime_error("SVD compression of complex blocks is not provided; use ACA");
+        AcaTask t{l, std::min(std::min(L.m, L.n), 1024), 0};
+        t.tab = off;
+        aca.push_back(t);
       } else if (std::max(L.m, L.n) < 100 || comp.method == 0 /*svd*/) {
         if (std::max(L.m, L.n) >= 100)
           throw std::runtime_error("SVD compression is implemented for blocks below 100 rows/cols");
@@ -614,6 +678,19 @@ void Matrix::info(hmat_info_t& out) const {
   };
   rec(bt->root);
   out.nr_block_clusters = count_nodes;
+  if (embedded) {
+    // counts of the complex matrix: a complex entry is 2 x 2 real entries, a
+    // complex rank k a real rank 2k
+    out.full_size /= 4;
+    out.rk_size /= 4;
+    out.uncompressed_size /= 4;
+    out.compressed_size /= 4;
+    out.largest_rk_dim_rows /= 2;
+    out.largest_rk_dim_cols /= 2;
+    out.largest_rk_mem_rows /= 2;
+    out.largest_rk_mem_cols /= 2;
+    out.largest_rk_mem_rank = (out.largest_rk_mem_rank + 1) / 2;
+  }
 }
 
 }  // namespace hmx

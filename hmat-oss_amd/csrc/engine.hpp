@@ -66,6 +66,8 @@ struct KernelSpec {
   double length = 1.0;
   double nugget = 0.0;
   double shift = 0.0;
+  double wavenumber = 0.0;  // kHelmholtz
+  double reg = 0.0;         // kHelmholtz
   int dim = 3;
   int n = 0;
   std::vector<double> points;  // user numbering
@@ -99,6 +101,20 @@ class Matrix {
          const std::vector<int>& cap_hint);
   AdmissibilityParams adm;
   ~Matrix();
+
+  // A complex (Z) matrix over the caller's trees r x c: the engine stores the
+  // 2 x 2 real representation [[Re, -Im], [Im, Re]] of every entry over the
+  // embedded trees (embed_cluster_tree), so that every kernel runs in real
+  // FP64 arithmetic and a complex vector (interleaved re, im) is a real vector
+  // of the embedded numbering.  er / ec: embedded trees to share (clones).
+  static Matrix* create_complex(const ClusterTree* r, const ClusterTree* c,
+                                const AdmissibilityParams& adm, int rank_cap,
+                                std::shared_ptr<ClusterTree> er = nullptr,
+                                std::shared_ptr<ClusterTree> ec = nullptr);
+  bool embedded = false;                    // a complex matrix (see create_complex)
+  const ClusterTree* zrows = nullptr;       // the caller's complex trees
+  const ClusterTree* zcols = nullptr;
+  std::shared_ptr<ClusterTree> erows, ecols;  // rows / cols point to these
 
   int value_type;        // hmat_value_t
   const ClusterTree* rows;

@@ -2247,6 +2247,8 @@ class Exec {
 void Matrix::factorize(int algo, bool fp32) {
   if (!assembled) throw std::runtime_error("factorize: matrix not assembled");
   if (factorization != -1) throw std::runtime_error("factorize: matrix already factorised");
+  if (embedded && algo != hmat_factorization_lu)
+    throw std::runtime_error("factorize: complex matrices are factorised by hmat_factorization_lu");
   if (algo == hmat_factorization_hodlr)
     throw std::runtime_error("factorize: the non-symmetric HODLR factorisation is not provided "
                              "(hmat_factorization_hodlrsym is)");

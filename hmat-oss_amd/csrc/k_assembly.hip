@@ -13,6 +13,7 @@ __device__ inline double entry(const PointSet& rp, const PointSet& cp, const Ker
   const double dx = rp.xyz[i] - cp.xyz[j];
   const double dy = rp.xyz[rp.n + i] - cp.xyz[cp.n + j];
   const double dz = rp.xyz[2 * rp.n + i] - cp.xyz[2 * cp.n + j];
+  if (kp.kind == kHelmholtz) return helmholtz_embedded(kp, dx, dy, dz, rp.user[i], cp.user[j]);
   return kernel_value(kp, dx, dy, dz, rp.user[i] == cp.user[j]);
 }
 

@@ -33,7 +33,8 @@ std::vector<int> leaves_where(const BlockTree& bt, bool diagonal_only) {
 
 Matrix* Matrix::clone(bool structure_only) {
   if (!assembled && !structure_only) throw std::runtime_error("copy: matrix not assembled");
-  auto* c = new Matrix(rows, cols, lower_sym, adm, value_type, rk_cap_max);
+  auto* c = embedded ? create_complex(zrows, zcols, adm, rk_cap_max / 2, erows, ecols)
+                     : new Matrix(rows, cols, lower_sym, adm, value_type, rk_cap_max);
   try {
     c->lowrank_eps = lowrank_eps;
     c->device_init();
@@ -105,7 +106,8 @@ double Matrix::norm() {
     const Block* b = bt->leaves[list[i]];
     r += (lower_sym && b->mirrored() ? 2.0 : 1.0) * h[i];
   }
-  return std::sqrt(r);
+  // the 2 x 2 real representation holds every complex entry twice in modulus
+  return std::sqrt(embedded ? 0.5 * r : r);
 }
 
 void Matrix::scale(double alpha) {

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <functional>
 #include <stdexcept>
+#include <unordered_map>
 #include <utility>
 
 namespace hmx {
@@ -276,6 +277,69 @@ std::unique_ptr<BlockTree> block_tree_from(const ClusterTree& rows, const Cluste
   int id = 0;
   b.number(bt->root, id);
   return bt;
+}
+
+namespace {
+// pool node of `from` -> the node at the same pool position of `to`
+std::unordered_map<const Cluster*, const Cluster*> pool_map(const ClusterTree& from,
+                                                            const ClusterTree& to) {
+  std::unordered_map<const Cluster*, const Cluster*> m;
+  for (size_t k = 0; k < from.pool.size(); ++k) m[&from.pool[k]] = &to.pool[k];
+  return m;
+}
+}  // namespace
+
+std::unique_ptr<ClusterTree> embed_cluster_tree(const ClusterTree& t) {
+  auto e = std::make_unique<ClusterTree>();
+  e->n = 2 * t.n;
+  e->dim = t.dim;
+  e->max_leaf = 2 * t.max_leaf;
+  e->coords.resize((size_t)e->n * t.dim);
+  e->indices.resize(e->n);
+  e->indices_rev.resize(e->n);
+  for (int u = 0; u < t.n; ++u)
+    for (int c = 0; c < 2; ++c) {
+      for (int d = 0; d < t.dim; ++d)
+        e->coords[(size_t)(2 * u + c) * t.dim + d] = t.coords[(size_t)u * t.dim + d];
+      e->indices[2 * u + c] = 2 * t.indices[u] + c;
+      e->indices_rev[2 * u + c] = 2 * t.indices_rev[u] + c;
+    }
+  for (const Cluster& c : t.pool) {
+    e->pool.push_back(c);
+    e->pool.back().offset = 2 * c.offset;
+    e->pool.back().size = 2 * c.size;
+  }
+  auto m = pool_map(t, *e);
+  auto to = [&](const Cluster* c) { return c ? const_cast<Cluster*>(m.at(c)) : nullptr; };
+  for (Cluster& c : e->pool) {
+    c.father = to(c.father);
+    for (Cluster*& ch : c.children) ch = to(ch);
+  }
+  e->root = to(t.root);
+  for (const Cluster* l : t.leaves) e->leaves.push_back(to(l));
+  return e;
+}
+
+std::unique_ptr<BlockTree> embed_block_tree(const BlockTree& bt, const ClusterTree& rows,
+                                            const ClusterTree& er, const ClusterTree& cols,
+                                            const ClusterTree& ec) {
+  auto mr = pool_map(rows, er);
+  auto mc = pool_map(cols, ec);
+  auto e = std::make_unique<BlockTree>();
+  e->dim = bt.dim;
+  std::unordered_map<const Block*, Block*> mb;
+  for (const Block& b : bt.pool) {
+    e->pool.push_back(b);
+    Block& d = e->pool.back();
+    d.rows = mr.at(b.rows);
+    d.cols = mc.at(b.cols);
+    mb[&b] = &d;
+  }
+  for (Block& d : e->pool)
+    for (Block*& ch : d.child) ch = ch ? mb.at(ch) : nullptr;
+  e->root = mb.at(bt.root);
+  for (const Block* l : bt.leaves) e->leaves.push_back(mb.at(l));
+  return e;
 }
 
 std::unique_ptr<BlockTree> build_block_tree(const ClusterTree& rows, const ClusterTree& cols,

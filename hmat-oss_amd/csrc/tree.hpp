@@ -108,4 +108,17 @@ struct BlockNodeDesc {
 std::unique_ptr<BlockTree> block_tree_from(const ClusterTree& rows, const ClusterTree& cols,
                                            const std::vector<BlockNodeDesc>& nodes);
 
+// The 2 x 2 real representation of a complex problem: cluster [o, o + s) of
+// t becomes [2o, 2o + 2s), and DoF 2u + c (c = 0 real part, 1 imaginary part)
+// of the new tree is component c of DoF u.  Coordinates, bounding boxes and
+// the node order are those of t, so a complex vector of n entries stored as
+// interleaved (re, im) pairs is a real vector in the new tree's numbering.
+std::unique_ptr<ClusterTree> embed_cluster_tree(const ClusterTree& t);
+
+// bt (built over rows x cols) over the embedded trees er x ec: the same
+// blocks, kinds and leaf order, each block covering twice the rows and columns.
+std::unique_ptr<BlockTree> embed_block_tree(const BlockTree& bt, const ClusterTree& rows,
+                                            const ClusterTree& er, const ClusterTree& cols,
+                                            const ClusterTree& ec);
+
 }  // namespace hmx

@@ -19,11 +19,15 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libhmat_mi355x.so")
 
-HMAT_SIMPLE_PRECISION, HMAT_DOUBLE_PRECISION = 0, 1
+HMAT_SIMPLE_PRECISION, HMAT_DOUBLE_PRECISION, HMAT_SIMPLE_COMPLEX, HMAT_DOUBLE_COMPLEX = 0, 1, 2, 3
+_NP_TYPE = {HMAT_SIMPLE_PRECISION: np.float32, HMAT_DOUBLE_PRECISION: np.float64,
+            HMAT_DOUBLE_COMPLEX: np.complex128}
 FACT_NONE, FACT_LU, FACT_LDLT, FACT_LLT, FACT_HODLR, FACT_HODLRSYM = -1, 0, 1, 2, 3, 4
 KERNEL_EXPONENTIAL, KERNEL_GAUSSIAN, KERNEL_SHIFTED_EXPONENTIAL = 0, 1, 2
 # derivatives with respect to the correlation length (for the GP likelihood gradient)
 KERNEL_EXPONENTIAL_DLENGTH, KERNEL_GAUSSIAN_DLENGTH = 3, 4
+# complex (HMAT_DOUBLE_COMPLEX): exp(i k r) / (4 pi r), r = sqrt(|x - y|^2 + h^2)
+KERNEL_HELMHOLTZ = 5
 
 _vp = C.c_void_p
 
@@ -76,7 +80,8 @@ class hmat_solve_context_t(C.Structure):
 class hmat_mi355x_kernel_t(C.Structure):
     _fields_ = [("kind", C.c_int), ("dimension", C.c_int), ("n", C.c_int),
                 ("points", C.POINTER(C.c_double)), ("length", C.c_double),
-                ("nugget", C.c_double), ("shift", C.c_double)]
+                ("nugget", C.c_double), ("shift", C.c_double), ("wavenumber", C.c_double),
+                ("regularization", C.c_double)]
 
 
 _CREATE = C.CFUNCTYPE(_vp, _vp, _vp, C.c_int, _vp)
@@ -288,8 +293,17 @@ class HMatrix:
     def n(self):
         return self.tree.n
 
+    @property
+    def is_complex(self):
+        return self.value_type == HMAT_DOUBLE_COMPLEX
+
+    @property
+    def dtype(self):
+        return _NP_TYPE[self.value_type]
+
     def assemble_context(self, kind=KERNEL_EXPONENTIAL, length=0.5, nugget=1e-2, eps=1e-4,
-                         method="aca_plus", factorization=FACT_NONE, shift=0.0):
+                         method="aca_plus", factorization=FACT_NONE, shift=0.0, wavenumber=0.0,
+                         reg=0.0):
         """An hmat_assemble_context_t for this matrix (the kernel as data, see
         hmat_mi355x_kernel_t), plus the compression object to delete after use."""
         lib = load()
@@ -297,6 +311,7 @@ class HMatrix:
         k.kind, k.dimension, k.n = kind, self.tree.dim, self.tree.n
         k.points = self.tree.points.ctypes.data_as(C.POINTER(C.c_double))
         k.length, k.nugget, k.shift = float(length), float(nugget), float(shift)
+        k.wavenumber, k.regularization = float(wavenumber), float(reg)
         self._kernel = k
         ctx = hmat_assemble_context_t()
         lib.hmat_assemble_context_init(C.byref(ctx))
@@ -311,8 +326,9 @@ class HMatrix:
         return ctx, comp
 
     def assemble(self, kind=KERNEL_EXPONENTIAL, length=0.5, nugget=1e-2, eps=1e-4,
-                 method="aca_plus", factorization=FACT_NONE, shift=0.0):
-        ctx, comp = self.assemble_context(kind, length, nugget, eps, method, factorization, shift)
+                 method="aca_plus", factorization=FACT_NONE, shift=0.0, wavenumber=0.0, reg=0.0):
+        ctx, comp = self.assemble_context(kind, length, nugget, eps, method, factorization, shift,
+                                          wavenumber, reg)
         rc = self.iface.assemble_generic(self.handle, C.byref(ctx))
         load().hmat_delete_compression(comp)
         _check(rc, "assemble_generic")
@@ -359,7 +375,11 @@ class HMatrix:
         cnt = lib.hmx_partition(self.handle, None)
         out = np.zeros(cnt * 5, dtype=np.int32)
         lib.hmx_partition(self.handle, out.ctypes.data)
-        return out.reshape(cnt, 5)
+        out = out.reshape(cnt, 5)
+        if self.is_complex:
+            # the engine's blocks cover the 2 x 2 real representation
+            out[:, :4] //= 2
+        return out
 
     def leaf_table(self, precision=0):
         """(count, 6) int array: m, n, row offset, col offset, kind, rank."""
@@ -389,8 +409,20 @@ class HMatrix:
                 _check(lib.hmx_leaf_data(self.handle, leaf, precision, a.ctypes.data, b.ctypes.data), "leaf")
                 yield m, n, r0, c0, kind, rank, a[:m * k].reshape(k, m).T, b[:n * k].reshape(k, n).T
 
-    def to_dense(self, precision=0, mirror=True):
-        """Dense matrix (hmat numbering) from the stored leaves."""
+    def to_dense(self, precision=0, mirror=True, embedded=False):
+        """Dense matrix (hmat numbering) from the stored leaves.  A complex
+        matrix is stored as the 2 x 2 real representation [[Re, -Im], [Im, Re]]
+        of its entries, which compression keeps to the tolerance only: with
+        embedded=True the stored real 2N x 2N matrix (hmat row / column 2p + c
+        is component c of hmat DoF p), otherwise its nearest complex matrix
+        (Re and Im averaged over the two places that hold them)."""
+        if self.is_complex:
+            d = np.zeros((2 * self.n, 2 * self.n))
+            for m, n, r0, c0, kind, rank, a, b in self.leaves(precision):
+                d[r0:r0 + m, c0:c0 + n] = a if kind == 0 else (a @ b.T if rank > 0 else 0.0)
+            if embedded:
+                return d
+            return 0.5 * (d[0::2, 0::2] + d[1::2, 1::2]) + 0.5j * (d[1::2, 0::2] - d[0::2, 1::2])
         d = np.zeros((self.n, self.n))
         for m, n, r0, c0, kind, rank, a, b in self.leaves(precision):
             blk = a if kind == 0 else (a @ b.T if rank > 0 else np.zeros((m, n)))
@@ -401,7 +433,7 @@ class HMatrix:
 
     # host-pointer API entries (user numbering unless stated)
     def gemv(self, x, alpha=1.0, beta=0.0, y=None, permute=True):
-        dt = np.float64 if self.value_type else np.float32
+        dt = self.dtype
         x = np.asfortranarray(x, dtype=dt)
         nrhs = 1 if x.ndim == 1 else x.shape[1]
         if x.shape[0] != self.n:
@@ -411,8 +443,7 @@ class HMatrix:
         elif y.shape != x.shape:
             raise ValueError(f"gemv: y has shape {y.shape}, expected {x.shape}")
         out = np.asfortranarray(y, dtype=dt)   # a converted copy when y's dtype/layout differs
-        T = C.c_double if self.value_type else C.c_float
-        a, b = T(alpha), T(beta)
+        a, b = self._scalar(alpha), self._scalar(beta)
         fn = self.iface.gemv if permute else self.iface.gemm_scalar
         _check(fn(b"N", C.cast(C.byref(a), _vp), self.handle, x.ctypes.data,
                   C.cast(C.byref(b), _vp), out.ctypes.data, nrhs), "gemv")
@@ -421,7 +452,7 @@ class HMatrix:
         return out
 
     def solve(self, b, permute=True):
-        x = np.array(b, dtype=np.float64 if self.value_type else np.float32, order="F", copy=True)
+        x = np.array(b, dtype=self.dtype, order="F", copy=True)
         nrhs = 1 if x.ndim == 1 else x.shape[1]
         fn = self.iface.solve_systems if permute else self.iface.solve_dense
         _check(fn(self.handle, x.ctypes.data, nrhs), "solve")
@@ -429,6 +460,8 @@ class HMatrix:
 
     # ---- whole-matrix slots ----------------------------------------------------
     def _scalar(self, v):
+        if self.is_complex:
+            return (C.c_double * 2)(complex(v).real, complex(v).imag)
         return (C.c_double if self.value_type else C.c_float)(v)
 
     def copy(self, structure_only=False):

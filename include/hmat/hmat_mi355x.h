@@ -26,7 +26,14 @@
 extern "C" {
 #endif
 
-/* Replaces hmat_init_default_interface (c_default_interface.cpp:242). */
+/* Replaces hmat_init_default_interface (c_default_interface.cpp:242).
+ * HMAT_DOUBLE_COMPLEX matrices are held as the 2 x 2 real representation
+ * [[Re, -Im], [Im, Re]] of every entry (real FP64 kernels throughout); complex
+ * arrays are passed as the reference passes them (interleaved re, im).  They
+ * are stored whole and factorised by H-LU; logdet, extract_diagonal,
+ * serialization and complex scalars with a nonzero imaginary part in scale /
+ * add_identity / axpy / gemm are not provided for them.  HMAT_SIMPLE_COMPLEX
+ * is not provided. */
 HMAT_API void hmat_init_mi355x_interface(hmat_interface_t *i, hmat_value_t type);
 
 typedef enum {
@@ -36,8 +43,13 @@ typedef enum {
   /* exp(-|x_i - x_j - shift e_z| / length) + nugget * delta_ij: non-symmetric */
   HMAT_MI355X_KERNEL_EXPONENTIAL_DLENGTH = 3,
   /* d/dlength of the exponential: (r / length^2) exp(-r / length) + nugget * delta_ij */
-  HMAT_MI355X_KERNEL_GAUSSIAN_DLENGTH = 4
+  HMAT_MI355X_KERNEL_GAUSSIAN_DLENGTH = 4,
   /* d/dlength of the Gaussian: (2 r^2 / length^3) exp(-r^2 / length^2) + nugget * delta_ij */
+  HMAT_MI355X_KERNEL_HELMHOLTZ = 5
+  /* complex (HMAT_DOUBLE_COMPLEX matrices): exp(i k r) / (4 pi r) with
+   * r = sqrt(|x_i - x_j|^2 + h^2), k = wavenumber, h = regularization;
+   * 1 / (4 pi h) + nugget on the diagonal (the interaction of
+   * examples/c-cylinder.c:68-78 with a regularised distance) */
 } hmat_mi355x_kernel_kind_t;
 
 typedef struct {
@@ -48,10 +60,13 @@ typedef struct {
   double length;         /* correlation length */
   double nugget;         /* added on the diagonal (user indices equal) */
   double shift;          /* HMAT_MI355X_KERNEL_SHIFTED_EXPONENTIAL only */
+  double wavenumber;     /* HMAT_MI355X_KERNEL_HELMHOLTZ only */
+  double regularization; /* HMAT_MI355X_KERNEL_HELMHOLTZ only (> 0) */
 } hmat_mi355x_kernel_t;
 
 /* hmat_interaction_func_t for an hmat_mi355x_kernel_t user context (the
- * simple_compute callback of assembly.cpp:75-91); writes a double. */
+ * simple_compute callback of assembly.cpp:75-91); writes a double, or a
+ * complex double (re, im) for HMAT_MI355X_KERNEL_HELMHOLTZ. */
 HMAT_API void hmat_mi355x_kernel_compute(void *user_context, int row, int col, void *result);
 
 /* ---- flat helpers (no counterpart in hmat-oss; used by the test-suite, the

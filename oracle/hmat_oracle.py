@@ -25,6 +25,9 @@ below (paths are relative to the hmat-oss source tree):
                               scalar_array.cpp:737-779
 * H-LU                       recursion.cpp:238-287, h_matrix.cpp:2567, 2772-2779,
                               scalar_array.cpp:694-705, 852-874
+* complex (Z) scalars        the same routines with complex128 temporaries
+                              (``scalar_type``), conjugated dot products in the
+                              ACA norm estimates (blas_overloads.hpp:446-463)
 * logdet                     h_matrix.cpp:2925-2936
 * info counters              h_matrix.cpp:464-502
 * structure serialization    serialization.cpp:34-130 (MatrixStructMarshaller)
@@ -42,6 +45,31 @@ import numpy as np
 # settings (hmat_cpp_interface.hpp:88-95, h_matrix.cpp:151, c_default_interface.cpp:195-200)
 # ---------------------------------------------------------------------------
 COMPRESSION_MIN_LEAF_SIZE = 100
+
+# Scalar type of the temporaries the H-arithmetic creates (float64, or
+# complex128 for the complex (Z) type: the reference templates every routine on
+# its scalar type, data_types.hpp:32-35).
+_DTYPE = [np.float64]
+
+
+class scalar_type:
+    """``with scalar_type(np.complex128): ...`` runs assembly and H-arithmetic
+    with complex temporaries."""
+
+    def __init__(self, dtype):
+        self.dtype = dtype
+
+    def __enter__(self):
+        self.old = _DTYPE[0]
+        _DTYPE[0] = self.dtype
+        return self
+
+    def __exit__(self, *a):
+        _DTYPE[0] = self.old
+
+
+def _zeros(shape):
+    return np.zeros(shape, dtype=_DTYPE[0])
 
 
 # ---------------------------------------------------------------------------
@@ -336,6 +364,27 @@ class GaussKernel(ExpKernel):
         return k
 
 
+class HelmholtzKernel:
+    """Regularised Helmholtz kernel (complex): K_ij = exp(i k r) / (4 pi r) with
+    r = sqrt(|x_i - x_j|^2 + h^2) off the diagonal, 1 / (4 pi h) + nugget on it
+    (user indices equal); the complex interaction of examples/c-cylinder.c:68-78
+    with the distance regularised by h."""
+
+    def __init__(self, points, wavenumber, reg, nugget):
+        self.p = np.ascontiguousarray(points, dtype=np.float64)
+        self.k = float(wavenumber)
+        self.h = float(reg)
+        self.nugget = float(nugget)
+        self.dtype = np.complex128
+
+    def block(self, ri, ci):
+        d = self.p[ri][:, None, :] - self.p[ci][None, :, :]
+        r = np.sqrt((d * d).sum(axis=2) + self.h * self.h)
+        v = np.exp(1j * self.k * r) / (4.0 * np.pi * r)
+        same = ri[:, None] == ci[None, :]
+        return np.where(same, 1.0 / (4.0 * np.pi * self.h) + self.nugget, v)
+
+
 # ---------------------------------------------------------------------------
 # dense helpers
 # ---------------------------------------------------------------------------
@@ -359,7 +408,9 @@ def truncated_svd(m, eps):
     if k == 0:
         return Rk(None, None)
     sq = np.sqrt(s[:k])
-    return Rk(u[:, :k] * sq, vt[:k].T.conj() * sq if np.iscomplexobj(m) else vt[:k].T * sq)
+    # the reference transposes LAPACK's V^H without conjugating it
+    # (scalar_array.cpp:1010): A B^T = U S V^H
+    return Rk(u[:, :k] * sq, vt[:k].T * sq)
 
 
 def rk_truncate(rk, m, n, eps):
@@ -398,9 +449,9 @@ def aca_full(m, eps):
         m -= np.outer(va, vb)
         new = 0.0
         for l in range(nu - 1):   # (sic) the reference sums l < nu-1
-            new += float(np.real(np.dot(va, acols[l]) * np.dot(vb, bcols[l])))
+            new += float(np.real(np.vdot(va, acols[l]) * np.vdot(vb, bcols[l])))
         est += 2.0 * new
-        ab = float(np.dot(va, va) * np.dot(vb, vb))
+        ab = float(np.real(np.vdot(va, va)) * np.real(np.vdot(vb, vb)))
         est += ab
         if ab < eps * eps * est:
             break                 # the pair that meets the criterion is dropped (resize(nu))
@@ -417,6 +468,7 @@ def aca_full(m, eps):
 class _BlockFn:
     def __init__(self, kernel, ri, ci):
         self.k, self.ri, self.ci = kernel, ri, ci
+        self.dtype = getattr(kernel, "dtype", np.float64)
 
     def row(self, i):
         return self.k.block(self.ri[i:i + 1], self.ci)[0]
@@ -447,7 +499,7 @@ def aca_partial(blk, m, n, eps):
     est = 0.0
     row_index, J, k, count = 0, 0, 0, 0
     while True:
-        b = blk.row(row_index).astype(np.float64)
+        b = blk.row(row_index).astype(blk.dtype)
         _update(b, row_index, bcols, acols)
         row_free[row_index] = False
         mx = 0.0
@@ -473,7 +525,7 @@ def aca_partial(blk, m, n, eps):
                     mx, row_index = v, i
             new = 0.0
             for l in range(k):
-                new += float(np.real(np.dot(a, acols[l]) * np.dot(b, bcols[l])))
+                new += float(np.real(np.vdot(a, acols[l]) * np.vdot(b, bcols[l])))
             est += 2.0 * new
             ab = float(np.real(np.vdot(a, a)) * np.real(np.vdot(b, b)))
             est += ab
@@ -501,7 +553,7 @@ def aca_plus(blk, m, n, eps):
     def find_col():
         for j in range(n):
             if col_free[j]:
-                c = blk.col(j).astype(np.float64)
+                c = blk.col(j).astype(blk.dtype)
                 col_free[j] = False
                 if np.any(c != 0):
                     return j, c
@@ -517,7 +569,7 @@ def aca_plus(blk, m, n, eps):
                         i_ref, mn = i, v
             if i_ref == -1:
                 return -1, None
-            r = blk.row(i_ref).astype(np.float64)
+            r = blk.row(i_ref).astype(blk.dtype)
             _update(r, i_ref, bcols, acols)
             row_free[i_ref] = False
             if np.any(r != 0):
@@ -533,7 +585,7 @@ def aca_plus(blk, m, n, eps):
                         j_ref, mn = j, v
             if j_ref == -1:
                 return -1, None
-            c = blk.col(j_ref).astype(np.float64)
+            c = blk.col(j_ref).astype(blk.dtype)
             _update(c, j_ref, acols, bcols)
             col_free[j_ref] = False
             if np.any(c != 0):
@@ -544,26 +596,26 @@ def aca_plus(blk, m, n, eps):
         return Rk(None, None)
     i_ref, b_ref = find_min_row(a_ref)
     if b_ref is None:
-        b_ref = np.zeros(n)
+        b_ref = np.zeros(n, dtype=blk.dtype)
     k = 0
     while True:
         i_star = _amax(a_ref)
         j_star = _amax(b_ref)
         if abs(a_ref[i_star]) ** 2 > abs(b_ref[j_star]) ** 2:
-            b = blk.row(i_star).astype(np.float64)
+            b = blk.row(i_star).astype(blk.dtype)
             _update(b, i_star, bcols, acols)
             j_star = _amax(b)
             piv = b[j_star]
-            a = blk.col(j_star).astype(np.float64)
+            a = blk.col(j_star).astype(blk.dtype)
             _update(a, j_star, acols, bcols)
             if piv != 0:
                 a = a * (1.0 / piv)
         else:
-            a = blk.col(j_star).astype(np.float64)
+            a = blk.col(j_star).astype(blk.dtype)
             _update(a, j_star, acols, bcols)
             i_star = _amax(a)
             piv = a[i_star]
-            b = blk.row(i_star).astype(np.float64)
+            b = blk.row(i_star).astype(blk.dtype)
             _update(b, i_star, bcols, acols)
             if piv != 0:
                 b = b * (1.0 / piv)
@@ -573,9 +625,9 @@ def aca_plus(blk, m, n, eps):
         bcols.append(b)
         new = 0.0
         for l in range(k):
-            new += float(np.dot(a, acols[l]) * np.dot(b, bcols[l]))
+            new += float(np.real(np.vdot(a, acols[l]) * np.vdot(b, bcols[l])))
         est += 2.0 * new
-        ab = float(np.dot(a, a) * np.dot(b, b))
+        ab = float(np.real(np.vdot(a, a)) * np.real(np.vdot(b, b)))
         est += ab
         k += 1
         if ab < eps * eps * est:
@@ -726,7 +778,7 @@ def gemv(h, trans, alpha, x, beta, y):
 # ---------------------------------------------------------------------------
 def _eval(h):
     """Dense value of a (sub)block with its own stored children only."""
-    d = np.zeros((h.m, h.n))
+    d = _zeros((h.m, h.n))
     if h.is_leaf():
         if h.kind == FULL:
             if h.full is not None:
@@ -796,8 +848,8 @@ def formatted_add_parts(target, parts, eps):
             full[r0:r0 + rr.size, c0:c0 + cc.size] += al * (rk.a @ rk.b.T)
         target.rk = truncated_svd(full, abs(eps))
         return
-    A = np.zeros((m, total))
-    B = np.zeros((n, total))
+    A = _zeros((m, total))
+    B = _zeros((n, total))
     off = 0
     for (al, rr, cc, rk) in used:
         r0 = rr.offset - target.rows.offset
@@ -813,7 +865,7 @@ def formatted_add_parts(target, parts, eps):
 
 def _eval_rk_block(b):
     if b.rk is None or b.rk.rank == 0:
-        return np.zeros((b.m, b.n))
+        return _zeros((b.m, b.n))
     return b.rk.a @ b.rk.b.T
 
 
@@ -853,7 +905,7 @@ def multiply_rk(eps, ta, tb, a, b):
             return ar, bc, Rk(None, None)
         # multiplyRkH rk_matrix.cpp:1127-1189: newB = op(H)^T B
         x = rk.b
-        y = np.zeros((bc.size, x.shape[1]))
+        y = _zeros((bc.size, x.shape[1]))
         gemv(b, "T" if tb == "N" else "N", 1.0, x, 0.0, y)
         return ar, bc, Rk(rk.a.copy(), y)
     if not a.is_leaf() and b.kind == RK:
@@ -862,7 +914,7 @@ def multiply_rk(eps, ta, tb, a, b):
             return ar, bc, Rk(None, None)
         # multiplyHRk rk_matrix.cpp:1192-1243: newA = op(H) A
         x = rk.a
-        y = np.zeros((ar.size, x.shape[1]))
+        y = _zeros((ar.size, x.shape[1]))
         gemv(a, ta, 1.0, x, 0.0, y)
         return ar, bc, Rk(y, rk.b.copy())
     if a.kind == RK and b.kind == RK:
@@ -900,7 +952,7 @@ def multiply_full(ta, tb, a, b):
         if b.full is None:
             return None
         x = _op_full(b.full, tb)
-        y = np.zeros((ar.size, bc.size))
+        y = _zeros((ar.size, bc.size))
         gemv(a, ta, 1.0, x, 0.0, y)
         return y
     if a.kind == FULL and not b.is_leaf():
@@ -908,7 +960,7 @@ def multiply_full(ta, tb, a, b):
             return None
         # multiplyFullH: result^T = op(H)^T op(M)^T
         x = _op_full(a.full, ta).T
-        y = np.zeros((bc.size, ar.size))
+        y = _zeros((bc.size, ar.size))
         gemv(b, "T" if tb == "N" else "N", 1.0, np.ascontiguousarray(x), 0.0, y)
         return y.T
     if a.kind == FULL and b.kind == FULL:
@@ -1032,7 +1084,7 @@ def leaf_gemm(c, ta, tb, alpha, a, b, eps):
         if (a.kind == RK and _is_null(a)) or (b.kind == RK and _is_null(b)):
             return
         _, _, rk = multiply_rk(eps, ta, tb, a, b)
-        f = rk.eval(c.m, c.n, np.float64)
+        f = rk.eval(c.m, c.n, _DTYPE[0])
     elif a.is_leaf() and b.is_leaf():
         if a.full is None or b.full is None:
             return
@@ -1059,7 +1111,7 @@ def full_hh_gemm(c, ta, tb, alpha, a, b, eps):
                         full_hh_gemm(c, tA, tB, alpha, ca, cb, eps)
     elif not _recursively_null(a) and not _recursively_null(b):
         if c.full is None:
-            c.full = np.zeros((c.m, c.n))
+            c.full = _zeros((c.m, c.n))
         hgemm(c, ta, tb, alpha, a, b, eps)
 
 

@@ -106,3 +106,21 @@ def test_full_size_partition_matches(n, height, seed):
     oidx, opart = oracle_partition(pts, 128)
     assert np.array_equal(idx, oidx)
     assert part == opart
+
+
+@pytest.mark.parametrize("n,leaf", [(1500, 64), (2050, 128)])
+def test_complex_partition_matches(n, leaf):
+    """A complex matrix keeps the complex partition: its blocks over the 2 x 2
+    real representation cover exactly twice the rows and columns."""
+    pts = ho.cylinder_points(n, 1.0, 1.0, 5)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    h = hmat_amd.HMatrix(t, lower_symmetric=False, value_type=hmat_amd.HMAT_DOUBLE_COMPLEX)
+    part = [(int(a), int(b), int(c), int(d), KIND[int(k)]) for a, b, c, d, k in h.partition()]
+    oidx, opart = oracle_partition(pts, leaf, sym=False)
+    assert np.array_equal(t.indices, oidx)
+    assert part == opart
+    info = h.info()
+    assert info["uncompressed_size"] == n * n
+    h.close()
+    with pytest.raises(hmat_amd.EngineError, match="stored whole"):
+        hmat_amd.HMatrix(t, lower_symmetric=True, value_type=hmat_amd.HMAT_DOUBLE_COMPLEX)
