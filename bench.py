@@ -189,6 +189,132 @@ def batch_bench(count, threads, seed=3):
                                  **out}}), flush=True)
 
 
+def helmholtz_bench(n, height, nrhs, gemv_reps, sample=4096, factor=True):
+    """The complex workload: N seeded points on a cylinder of radius 1 and the
+    given height, the regularised Helmholtz kernel exp(i k r) / (4 pi r) with
+    r = sqrt(|x - y|^2 + h^2), h the mean nearest-neighbour spacing, k = 2 pi /
+    (10 h), diagonal 1 / (4 pi h) + 1; leaf 128, eta 2, partial ACA eps 1e-4;
+    gemv_reps H-gemv with one complex right-hand side, then H-LU factor + solve
+    of nrhs complex-normal right-hand sides, the residual checked against the
+    exact kernel on a seeded sample of rows."""
+    import torch
+    import hmat_amd
+    from scipy.spatial import cKDTree
+    from oracle.hmat_oracle import cylinder_points  # input generator only
+    t0 = time.time()
+    pts = cylinder_points(n, 1.0, height, 1)
+    d, _ = cKDTree(pts).query(pts, 2)
+    reg = float(d[:, 1].mean())
+    wk = 2 * np.pi / (10 * reg)
+    tree = hmat_amd.ClusterTree(pts, 128)
+    idx = tree.indices
+    log(f"helmholtz: N={n}, h={reg:.5f}, k={wk:.3f}; points + tree {time.time() - t0:.1f}s")
+    H = hmat_amd.HMatrix(tree, eta=2.0, lower_symmetric=False,
+                         value_type=hmat_amd.HMAT_DOUBLE_COMPLEX)
+    torch.cuda.synchronize()
+    t = time.time()
+    H.assemble(kind=hmat_amd.KERNEL_HELMHOLTZ, wavenumber=wk, reg=reg, nugget=1.0, eps=1e-4,
+               method="aca_partial")
+    t_asm = time.time() - t
+    info = H.info()
+    tab = H.leaf_table()
+    ranks = tab[:, 5].astype(np.int64)
+    log(f"helmholtz: assembly {t_asm:.2f}s, compression {info['compressed_size'] / info['uncompressed_size']:.4f}, "
+        f"largest complex rank {int(ranks.max() + 1) // 2}")
+    # bytes one H-gemv reads and writes: the stored real representation (8-byte
+    # entries, full m x n and Rk (m + n) rank, embedded sizes), x per leaf
+    # column block, y per leaf row block read and written; the complex
+    # matrix's own storage is half of the matrix part
+    m_, n_, kind = tab[:, 0].astype(np.int64), tab[:, 1].astype(np.int64), tab[:, 4]
+    mat = np.where(kind == 0, m_ * n_, (m_ + n_) * np.maximum(ranks, 0)).sum()
+    vec = int(n_.sum() + 2 * m_.sum())
+    bytes_engine = 8 * int(mat) + 8 * vec
+    bytes_complex = 8 * int(mat) // 2 + 8 * vec
+    stream = torch.cuda.current_stream()
+    H.set_stream(stream.cuda_stream)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    xg = torch.randn(1, n, dtype=torch.complex128, device="cuda", generator=g)
+    yg = torch.zeros(1, n, dtype=torch.complex128, device="cuda")
+    for _ in range(3):
+        H.gemv_device(xg.data_ptr(), yg.data_ptr(), 1)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t = time.time()
+    e0.record(stream)
+    for _ in range(gemv_reps):
+        H.gemv_device(xg.data_ptr(), yg.data_ptr(), 1)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_gemv_wall = time.time() - t
+    H.set_stream(None)
+    gemv_ms = e0.elapsed_time(e1) / gemv_reps
+    log(f"helmholtz: {gemv_reps} H-gemv {t_gemv_wall:.2f}s, {gemv_ms:.3f} ms each, "
+        f"{bytes_engine / (gemv_ms * 1e-3) / 1e9:.0f} GB/s")
+    if not factor:
+        H.close()
+        print(json.dumps({
+            "metric": "complex Helmholtz H-gemv GB/s", "value": bytes_engine / (gemv_ms * 1e-3) / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "complex128 as 2x2 real f64",
+            "config": {"workload": f"Helmholtz, N={n}, cylinder r=1 h={height}, leaf 128, eta 2, "
+                                   f"partial ACA eps 1e-4, {gemv_reps} H-gemv (1 complex RHS)",
+                       "assembly_s": t_asm, "gemv_ms": gemv_ms,
+                       "gemv_total_s": gemv_ms * gemv_reps / 1e3, "gemv_wall_s": t_gemv_wall,
+                       "gemv_GBps_engine_bytes": bytes_engine / (gemv_ms * 1e-3) / 1e9,
+                       "gemv_GBps_complex_bytes": bytes_complex / (gemv_ms * 1e-3) / 1e9,
+                       "gemv_bytes_engine": bytes_engine, "gemv_bytes_complex": bytes_complex,
+                       "compressed_ratio": info["compressed_size"] / info["uncompressed_size"],
+                       "largest_complex_rank": int(ranks.max() + 1) // 2,
+                       "wavenumber": wk, "h": reg}}), flush=True)
+        return
+    rng = np.random.default_rng(16)
+    b = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))   # hmat order
+    bd = torch.tensor(b.T.copy(), dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+    t = time.time()
+    H.factorize(hmat_amd.FACT_LU)
+    torch.cuda.synchronize()
+    t_fac = time.time() - t
+    st = H.stats()
+    log(f"helmholtz: H-LU {t_fac:.2f}s ({st.get('tasks', 0):.0f} tasks)")
+    t = time.time()
+    H.solve_device(bd.data_ptr(), nrhs)
+    torch.cuda.synchronize()
+    t_sol = time.time() - t
+    x = bd.cpu().numpy().T
+    # residual against the exact kernel on a seeded row sample (hmat order)
+    rows = np.sort(np.random.default_rng(7).choice(n, min(sample, n), replace=False))
+    P = torch.tensor(pts[idx], dtype=torch.float64, device="cuda")
+    X = torch.tensor(x, dtype=torch.complex128, device="cuda")
+    ax = []
+    for c0 in range(0, len(rows), 256):
+        rr = torch.tensor(rows[c0:c0 + 256], device="cuda")
+        r = torch.sqrt(torch.cdist(P[rr], P) ** 2 + reg * reg)
+        K = torch.exp(1j * wk * r) / (4 * np.pi * r)
+        K[torch.arange(len(rr), device="cuda"), rr] = 1.0 / (4 * np.pi * reg) + 1.0
+        ax.append(K @ X)
+    ax = torch.cat(ax).cpu().numpy()
+    bs = b[rows]
+    resid = float((np.linalg.norm(bs - ax, axis=0) / np.linalg.norm(bs, axis=0)).max())
+    log(f"helmholtz: solve {t_sol:.3f}s, sample residual {resid:.2e}")
+    H.close()
+    print(json.dumps({
+        "metric": "complex Helmholtz H-LU factor + solve s (16 RHS) and H-gemv GB/s",
+        "value": t_fac + t_sol, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "complex128 as 2x2 real f64",
+        "config": {"workload": f"Helmholtz, N={n}, cylinder r=1 h={height}, leaf 128, eta 2, "
+                               f"partial ACA eps 1e-4, {nrhs} complex RHS, {gemv_reps} H-gemv",
+                   "assembly_s": t_asm, "factor_s": t_fac, "solve_s": t_sol,
+                   "residual_sample_max": resid, "sample_rows": len(rows),
+                   "gemv_ms": gemv_ms, "gemv_total_s": gemv_ms * gemv_reps / 1e3,
+                   "gemv_GBps_engine_bytes": bytes_engine / (gemv_ms * 1e-3) / 1e9,
+                   "gemv_GBps_complex_bytes": bytes_complex / (gemv_ms * 1e-3) / 1e9,
+                   "gemv_bytes_engine": bytes_engine, "gemv_bytes_complex": bytes_complex,
+                   "compressed_ratio": info["compressed_size"] / info["uncompressed_size"],
+                   "largest_complex_rank": int(ranks.max() + 1) // 2,
+                   "wavenumber": wk, "h": reg}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +332,12 @@ def main():
                     help="run this many independent systems through hmx_run_batch instead "
                          "(the batching workload; prints its own JSON line)")
     ap.add_argument("--batch-threads", type=int, default=8)
+    ap.add_argument("--helmholtz", action="store_true",
+                    help="the complex workload instead (N=--n, default 100,000, height 8): "
+                         "H-gemv x --gemv-reps (default 1000) and H-LU + solve of 16 complex "
+                         "RHS; prints its own JSON line")
+    ap.add_argument("--no-factor", action="store_true",
+                    help="with --helmholtz: assembly and H-gemv only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 2 GiB device to device and exit: the known-size launch that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -228,6 +360,12 @@ def main():
 
     if args.batch:
         batch_bench(args.batch, args.batch_threads)
+        return
+
+    if args.helmholtz:
+        n = args.n if args.n != 200000 else 100000
+        helmholtz_bench(n, 8.0 if args.height == 16.0 else args.height, 16,
+                        args.gemv_reps if args.gemv_reps != 20 else 1000, factor=not args.no_factor)
         return
 
     import hmat_amd

@@ -448,7 +448,12 @@ __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double
     return 0;
   }
   const int np = (cols + 1) & ~1, rounds = np - 1, half = np / 2;
-  const int g = group_lanes(half, kLanes), ngrp = kLanes / g, grp = tid / g, sub = tid % g;
+  // long columns: at least 16 lanes per pair, so that a pair's loads are
+  // coalesced and split 16 ways (fewer pairs side by side, each much faster
+  // than a pair walked by one or two lanes)
+  int g0 = group_lanes(half, kLanes);
+  if (kLanes >= kWave && rows >= 128 && g0 < 16) g0 = 16;
+  const int g = g0, ngrp = kLanes / g, grp = tid / g, sub = tid % g;
   // without scratch for every column the norms are recomputed pair by pair
   const bool track = nrm != nullptr && cols <= kJacobiMaxCols;
   int sweep = 0;
@@ -496,8 +501,7 @@ __device__ inline int team_jacobi(double* G, int ldg, int rows, int cols, double
         double* gq = G + (size_t)q * ldg;
         double c2 = 0.0, a, b;
         if (track) {
-          for (int i = sub; i < rows; i += g) c2 += gp[i] * gq[i];
-          c2 = group_sum(c2, g);
+          c2 = group_sum(strided_dot(gp, gq, sub, rows, g), g);
           a = nrm[p];
           b = nrm[q];
         } else {

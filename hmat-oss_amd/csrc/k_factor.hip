@@ -1080,16 +1080,26 @@ __device__ void wy_apply_out(const double* V, int ldv, int rows, int K, const do
 
 // After QR of both panels (reflectors Va, Vb, factors ta, tb) and G = Ra Rb^T
 // in w.g: SVD of G, cut at eps, write Qa U sqrt(S) and Qb V sqrt(S) to out.
+// stage: free LDS of kShDoubles doubles when the core lives in global memory
+// (nullptr otherwise): a core of at most kShDoubles entries is rotated there.
 template <typename T>
 __device__ void qr_finish(const double* Va, int lda, int m, const double* Vb, int ldb, int n, int K,
                           const QrScratch& w, bool wave_core, double eps, RkRef out, int* overflow,
-                          int* flag, int* sweeps_out) {
+                          int* flag, int* sweeps_out, double* stage = nullptr) {
   __shared__ int sweeps;
   // one-sided Jacobi on G without V: G_in (kept in w.v) gives the right side
   // afterwards as G_in^T U, so the rounds only rotate G
   for (int e = threadIdx.x; e < K * K; e += kBlock) w.v[e] = w.g[e];
   __syncthreads();
-  if (wave_core) {
+  if (stage != nullptr && K * K <= kShDoubles) {
+    // the rounds' loads and stores hit LDS instead of the L2
+    for (int e = threadIdx.x; e < K * K; e += kBlock) stage[e] = w.g[e];
+    __syncthreads();
+    jacobi_svd(stage, K, K, K, nullptr, K, flag);
+    for (int e = threadIdx.x; e < K * K; e += kBlock) w.g[e] = stage[e];
+    if (threadIdx.x == 0) sweeps = 0;
+    __syncthreads();
+  } else if (wave_core) {
     if (threadIdx.x < kWave) {
       const int sw = wave_jacobi_svd(w.g, K, K, K, nullptr, K);
       if (threadIdx.x == 0) sweeps = sw;
@@ -1509,7 +1519,7 @@ __global__ void __launch_bounds__(kBlock) k_fadd(const FTask* tasks, double* wsb
   // the core SVD runs on one wavefront when its scratch is in LDS
   const bool core_lds = mode == 5 || mode == 6 || cr == sh;
   qr_finish<T>(Va, lda, m, Vb, ldb, n, K, w, core_lds && K <= kWaveJacobiMax, fabs(t.eps), tg, overflow, &flag,
-               &sweeps);
+               &sweeps, core_lds ? nullptr : sh);
 }
 
 // FADD_DENSE: RkMatrix::formattedAddParts with a full part (rk_matrix.cpp:984-1010)
