@@ -620,12 +620,47 @@ void gemv_host_t(Matrix& M, const void* alpha, void* x, const void* beta, void* 
   else
     std::memcpy(y, hy.data(), hy.size() * sizeof(T));
 }
+
+// y = alpha op(F) x + beta y for a factorised matrix (F: see factor_gemv_device)
+template <typename T>
+void factor_gemv_host_t(Matrix& M, char trans, const void* alpha, void* x, const void* beta,
+                        void* y, int nrhs, bool permute) {
+  const int n = M.n_rows();
+  std::vector<T> hx((size_t)n * nrhs), ht((size_t)n * nrhs);
+  if (permute)
+    permute_in((const T*)x, hx.data(), M.cols->indices, n, nrhs);
+  else
+    std::memcpy(hx.data(), x, hx.size() * sizeof(T));
+  DevBuf dx, dt;
+  upload(dx, hx.data(), hx.size() * sizeof(T), M.stream);
+  dt.alloc(hx.size() * sizeof(T));
+  M.factor_gemv_device<T>(trans, dx.as<T>(), dt.as<T>(), nrhs, M.store<T>());
+  HMX_CHECK(hipMemcpy(hx.data(), dt.p, hx.size() * sizeof(T), hipMemcpyDeviceToHost));
+  if (permute)
+    permute_out(hx.data(), ht.data(), M.rows->indices, n, nrhs);
+  else
+    ht.swap(hx);
+  const T a = *(const T*)alpha, b = *(const T*)beta;
+  T* yy = (T*)y;
+  for (size_t i = 0; i < ht.size(); ++i) yy[i] = a * ht[i] + (b == (T)0 ? (T)0 : b * yy[i]);
+}
 }  // namespace
 
 void Matrix::gemv_host(char trans, const void* alpha, void* x, const void* beta, void* y, int nrhs,
                        bool permute) {
   if (!assembled) throw std::runtime_error("gemv: matrix not assembled");
-  if (factorization != -1) throw std::runtime_error("gemv: matrix has been factorised");
+  if (factorization != -1) {
+    // HMatrix::gemv of the factor (DefaultEngine::gemv default_engine.cpp:133-140)
+    if (factor32 && value_type == HMAT_DOUBLE_PRECISION)
+      throw std::runtime_error("gemv: the matrix holds a mixed-precision factor");
+    if (value_type == HMAT_DOUBLE_PRECISION)
+      factor_gemv_host_t<double>(*this, trans, alpha, x, beta, y, nrhs, permute);
+    else if (value_type == HMAT_SIMPLE_PRECISION)
+      factor_gemv_host_t<float>(*this, trans, alpha, x, beta, y, nrhs, permute);
+    else
+      throw std::runtime_error("gemv: complex types are not supported by this engine");
+    return;
+  }
   if (trans != 'N' && !lower_sym) throw std::runtime_error("gemv: only 'N' for non-symmetric matrices");
   if (value_type == HMAT_DOUBLE_PRECISION)
     gemv_host_t<double>(*this, alpha, x, beta, y, nrhs, permute);

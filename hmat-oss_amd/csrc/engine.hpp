@@ -200,6 +200,11 @@ class Matrix {
   template <typename T>
   void solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part = 0);
   void solve_host(void* b, int nrhs, bool permute, int part = 0);
+  // y <- op(F) x for a factorised matrix: F = L of an LLt factor, L W of a
+  // symmetric HODLR factor (device pointers, hmat numbering, ld = N; y is
+  // overwritten)
+  template <typename T>
+  void factor_gemv_device(char trans, const T* x, T* y, int nrhs, const Store<T>& st);
   double logdet();
   // Iterative refinement (FP32 factor, FP64 residual with the FP64 matrix).
   int solve_refined(double* b_dev, int nrhs, int ldb, double tol, int max_iter, double* resid_out);

@@ -2001,6 +2001,98 @@ class Exec {
     hodlr_upper(m->get(1, 1), X1, ncol);
   }
 
+  // C (+)= alpha A, dense views of equal shape (a T_AXPY task; set: C = alpha A)
+  void dense_axpy(const DV& C, double alpha, const DV& A, Dim m, Dim n, bool set) {
+    FTask t{};
+    t.type = T_AXPY;
+    t.mode = set ? 1 : 0;
+    t.m = m;
+    t.n = n;
+    t.alpha = alpha;
+    t.A = A.p;
+    t.lda = A.ld;
+    t.C = C.p;
+    t.ldc = C.ld;
+    rd(A);
+    if (!set) rd(C);
+    wr(C);
+    emit(t);
+  }
+
+  // Y += op(L) X for the stored blocks of a factorised matrix, without the
+  // mirrored upper blocks of symmetric storage: HMatrix::gemv of an LLt factor
+  // (isTriLower, isLower off: getChildForGEMM returns the absent upper children
+  // as null, h_matrix.cpp:771-788, 790-857); the diagonal leaves hold L with
+  // their upper triangle zeroed (scalar_array.cpp:822-826, as POTRF leaves it).
+  void tri_gemv(const Block* h, char tr, const DV& X, const DV& Y, Dim ncol) {
+    if (h->is_leaf()) {
+      hgemv(h, tr, 1.0, X, Y, ncol);
+      return;
+    }
+    for (int i = 0; i < h->nrow; ++i)
+      for (int j = 0; j < h->ncol; ++j) {
+        const Block* ch = h->get(i, j);
+        if (!ch) continue;
+        int co = ch->cols->offset - h->cols->offset, cs = ch->n();
+        int ro = ch->rows->offset - h->rows->offset, rs = ch->m();
+        if (tr != 'N') {
+          std::swap(co, ro);
+          std::swap(cs, rs);
+        }
+        tri_gemv(ch, tr, rows_of(X, co, cs), rows_of(Y, ro, rs), ncol);
+      }
+  }
+
+  // Y += op(L W) X for a symmetric HODLR factor (HODLR<T>::gemv, hodlr.cpp:
+  // 325-378): per node, with a10 = a b^T, W11 = I + a X11 a^T,
+  //   'N': y1 += L1 (x1 + a (b^T x0 + X11 a^T x1)),  y0 += L0 x0;
+  //   'T': y0 += L0^T x0 + b a^T L1^T x1,  y1 += L1^T x1 + a X11^T a^T L1^T x1.
+  void hodlr_gemv(const Block* m, char tr, const DV& X, const DV& Y, Dim ncol) {
+    if (m->is_leaf()) {
+      hgemv(m, tr, 1.0, X, Y, ncol);
+      return;
+    }
+    const int n0 = m->get(0, 0)->m(), n1 = m->get(1, 1)->m();
+    DV X0 = rows_of(X, 0, n0), X1 = rows_of(X, n0, n1);
+    DV Y0 = rows_of(Y, 0, n0), Y1 = rows_of(Y, n0, n1);
+    RV r = rk_of(m->get(1, 0));
+    const Dim rk = dim(r.cap, r.rank);
+    const int w = std::max(ncol.s, 1);
+    DV A = dv_panel(r.a, r.la, r.m, r.cap, r.bufa, r.row0);
+    DV B = dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0);
+    DV At = A, Bt = B;
+    At.trans = Bt.trans = true;
+    DV X11 = hodlr_block(M_.hodlr_index.at(m), 0);
+    if (tr == 'N') {
+      DV t = temp_dense(r.cap, w, -1), u = temp_dense(r.cap, w, -1), x1b = temp_dense(n1, w, -1);
+      gemm(t, false, 1.0, At, X1, rk, ncol, dim(n1));
+      gemm(u, false, 1.0, X11, t, rk, ncol, rk);
+      gemm(u, true, 1.0, Bt, X0, rk, ncol, dim(n0));
+      dense_axpy(x1b, 1.0, X1, dim(n1), ncol, true);
+      gemm(x1b, true, 1.0, A, u, dim(n1), ncol, rk);
+      hodlr_gemv(m->get(1, 1), tr, x1b, Y1, ncol);
+      hodlr_gemv(m->get(0, 0), tr, X0, Y0, ncol);
+      release();
+      release();
+      release();
+      return;
+    }
+    hodlr_gemv(m->get(0, 0), tr, X0, Y0, ncol);
+    DV l = temp_dense(n1, w, -1), t = temp_dense(r.cap, w, -1), u = temp_dense(r.cap, w, -1);
+    zero(l);
+    hodlr_gemv(m->get(1, 1), tr, X1, l, ncol);
+    gemm(t, false, 1.0, At, l, rk, ncol, dim(n1));
+    gemm(Y0, true, 1.0, B, t, dim(n0), ncol, rk);
+    DV X11t = X11;
+    X11t.trans = true;
+    gemm(u, false, 1.0, X11t, t, rk, ncol, rk);
+    gemm(Y1, true, 1.0, A, u, dim(n1), ncol, rk);
+    dense_axpy(Y1, 1.0, l, dim(n1), ncol, false);
+    release();
+    release();
+    release();
+  }
+
   // HMatrix::solveDiagonal h_matrix.cpp:2969-2983 (LDLt): X <- D^{-1} X, one
   // task per diagonal leaf
   void solve_diag(const DV& X, Dim ncol) {
@@ -2455,6 +2547,48 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part)
 }
 template void Matrix::solve_device<double>(double*, int, int, const Store<double>&, int);
 template void Matrix::solve_device<float>(float*, int, int, const Store<float>&, int);
+
+template <typename T>
+void Matrix::factor_gemv_device(char trans, const T* x, T* y, int nrhs, const Store<T>& st) {
+  if (factorization != hmat_factorization_llt && factorization != hmat_factorization_hodlrsym)
+    throw std::runtime_error("gemv of a factorised matrix: provided for LLt and HODLR factors "
+                             "(the product with L, or with L W)");
+  if ((factor32 ? 1 : 0) != (sizeof(T) == 4 ? 1 : 0))
+    throw std::runtime_error("gemv: the factor is stored in the other precision");
+  if (trans != 'N' && trans != 'T' && trans != 'C') throw std::runtime_error("gemv: bad trans");
+  if (nrhs < 1) throw std::runtime_error("gemv: nrhs must be positive");
+  device_init();
+  if (!fstate_) fstate_ = std::make_shared<FactorState>();
+  const int n = rows->n;
+  DevBuf bx, by;
+  bx.alloc(sizeof(T) * (size_t)n * nrhs);
+  by.alloc(sizeof(T) * (size_t)n * nrhs);
+  HMX_CHECK(hipMemcpyAsync(bx.p, x, sizeof(T) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, stream));
+  HMX_CHECK(hipMemsetAsync(by.p, 0, sizeof(T) * (size_t)n * nrhs, stream));
+  HMX_CHECK(hipStreamSynchronize(stream));
+  {
+    Exec<T> ex(*this, const_cast<Store<T>&>(st), *fstate_);
+    DV X, Y;
+    X.p = bx.p;
+    Y.p = by.p;
+    X.ld = Y.ld = X.m = Y.m = n;
+    X.n = Y.n = nrhs;
+    X.buf = kTempBase + (int64_t(1) << 30) + 1;  // tracked per leaf cluster of rows
+    Y.buf = kTempBase + (int64_t(1) << 30) + 2;
+    X.row0 = Y.row0 = 0;
+    const char t = trans == 'N' ? 'N' : 'T';
+    if (factorization == hmat_factorization_hodlrsym)
+      ex.hodlr_gemv(bt->root, t, X, Y, Exec<T>::dim(nrhs));
+    else
+      ex.tri_gemv(bt->root, t, X, Y, Exec<T>::dim(nrhs));
+    ex.flush(false);
+    ex.check_flags();
+  }
+  HMX_CHECK(hipMemcpyAsync(y, by.p, sizeof(T) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, stream));
+  HMX_CHECK(hipStreamSynchronize(stream));
+}
+template void Matrix::factor_gemv_device<double>(char, const double*, double*, int, const Store<double>&);
+template void Matrix::factor_gemv_device<float>(char, const float*, float*, int, const Store<float>&);
 
 void Matrix::solve_host(void* b, int nrhs, bool permute, int part) {
   const int n = rows->n;

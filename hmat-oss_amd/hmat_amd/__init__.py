@@ -432,7 +432,9 @@ class HMatrix:
         return d
 
     # host-pointer API entries (user numbering unless stated)
-    def gemv(self, x, alpha=1.0, beta=0.0, y=None, permute=True):
+    def gemv(self, x, alpha=1.0, beta=0.0, y=None, permute=True, trans="N"):
+        """y = alpha op(A) x + beta y; on an LLt / HODLR-factorised matrix op
+        applies to the factor (L, or L W), as the reference's gemv does."""
         dt = self.dtype
         x = np.asfortranarray(x, dtype=dt)
         nrhs = 1 if x.ndim == 1 else x.shape[1]
@@ -445,7 +447,7 @@ class HMatrix:
         out = np.asfortranarray(y, dtype=dt)   # a converted copy when y's dtype/layout differs
         a, b = self._scalar(alpha), self._scalar(beta)
         fn = self.iface.gemv if permute else self.iface.gemm_scalar
-        _check(fn(b"N", C.cast(C.byref(a), _vp), self.handle, x.ctypes.data,
+        _check(fn(trans.encode(), C.cast(C.byref(a), _vp), self.handle, x.ctypes.data,
                   C.cast(C.byref(b), _vp), out.ctypes.data, nrhs), "gemv")
         if out is not y and isinstance(y, np.ndarray):
             y[...] = out

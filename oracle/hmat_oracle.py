@@ -18,7 +18,7 @@ below (paths are relative to the hmat-oss source tree):
 * H-gemm / axpy              h_matrix.cpp:870-1073, 1202-1620, 1874-1950
 * H-LLt and triangular solves recursion.cpp:86-134, 191-233, 349-444,
                               h_matrix.cpp:2297-2564, 3010-3018
-* HODLR (symmetric)          hodlr.cpp:113-166, 221-312, 365-404,
+* HODLR (symmetric)          hodlr.cpp:113-166, 221-312, 325-378, 365-404,
                               admissibility.cpp:231-241
 * H-LDLt                     recursion.cpp:37-83, h_matrix.cpp:1953-1998,
                               2585-2690, 2747-2769, 2969-3002,
@@ -1462,6 +1462,52 @@ def hodlr_lower(m, x, node):
         return
     x1 -= rk.a @ (rk.b.T @ x0)
     x1 -= rk.a @ (node.kk @ (rk.a.T @ x1))
+
+
+def hodlr_gemv(m, trans, x, y, node):
+    """y += op(L W) x for a symmetric HODLR factor (HODLR<T>::gemv
+    hodlr.cpp:325-378, alpha 1, beta 1): W = [[I, 0], [a b^T, I + a X11 a^T]]
+    per node, L the leaves' Cholesky factors."""
+    if m.is_leaf():
+        y += (m.full if trans == "N" else m.full.T) @ x
+        return
+    n0 = m.get(0, 0).m
+    x0, x1, y0, y1 = x[:n0], x[n0:], y[:n0], y[n0:]
+    rk = m.get(1, 0).rk
+    r = rk.rank
+    if trans == "N":
+        x1b = x1.copy()
+        if r:
+            x1b += rk.a @ (rk.b.T @ x0 + node.x11 @ (rk.a.T @ x1))
+        hodlr_gemv(m.get(1, 1), trans, x1b, y1, node.child1)
+        hodlr_gemv(m.get(0, 0), trans, x0, y0, node.child0)
+        return
+    hodlr_gemv(m.get(0, 0), trans, x0, y0, node.child0)
+    l1 = np.zeros_like(y1)
+    hodlr_gemv(m.get(1, 1), trans, x1, l1, node.child1)
+    if r:
+        t = rk.a.T @ l1
+        y0 += rk.b @ t
+        y1 += rk.a @ (node.x11.T @ t)
+    y1 += l1
+
+
+def factor_gemv(L, trans, x, y):
+    """y += op(L) x for an LLt-factorised tree: HMatrix::gemv of the factor
+    (isTriLower, no mirrored upper blocks; h_matrix.cpp:771-857); diagonal
+    leaves hold L with a zero upper triangle (scalar_array.cpp:822-826)."""
+    for b in leaves(L):
+        r0, c0 = b.rows.offset - L.rows.offset, b.cols.offset - L.cols.offset
+        if b.kind == FULL:
+            f = np.tril(b.full) if b.rows is b.cols else b.full
+        elif b.rk is not None and b.rk.rank:
+            f = b.rk.a @ b.rk.b.T
+        else:
+            continue
+        if trans == "N":
+            y[r0:r0 + b.m] += f @ x[c0:c0 + b.n]
+        else:
+            y[c0:c0 + b.n] += f.T @ x[r0:r0 + b.m]
 
 
 def hodlr_upper(m, x, node):

@@ -130,6 +130,12 @@ def test_hgemv_and_llt_against_dense():
     xs = b.copy()
     ho.solve_llt(bt, xs)
     assert np.linalg.norm(L @ (L.T @ xs) - b) <= 1e-10 * np.linalg.norm(b) * np.linalg.cond(L) ** 0
+    # the gemv of the factor applies L and L^T (not the mirrored matrix)
+    v = np.random.default_rng(2).standard_normal((n, 2))
+    for tr, op in (("N", L), ("T", L.T)):
+        w = np.zeros_like(v)
+        ho.factor_gemv(bt, tr, v, w)
+        assert np.linalg.norm(w - op @ v) <= 1e-13 * np.linalg.norm(op @ v)
     # logdet returns log det L = 0.5 log det A
     assert np.isclose(ho.logdet(bt), np.log(np.diag(L)).sum(), rtol=1e-12)
     assert np.isclose(ho.logdet(bt), 0.5 * np.linalg.slogdet(H)[1], rtol=1e-4)
@@ -342,3 +348,15 @@ def test_hodlr_sym_is_exact_factorisation():
     sign, ld = np.linalg.slogdet(H)
     assert sign > 0 and abs(ho.hodlr_logdet(bt, node) - 0.5 * ld) <= 1e-10 * abs(ld)
     assert np.allclose((y * y).sum(axis=0), (b * xe).sum(axis=0), rtol=1e-10)
+    # the gemv of the factor: (L W) (L W)^T = A exactly, and (L W)^{-1} is the
+    # lower solve (hodlr.cpp:325-378 against 113-156)
+    v = np.random.default_rng(6).standard_normal((n, 2))
+    t = np.zeros_like(v)
+    ho.hodlr_gemv(bt, "T", v, t, node)
+    w = np.zeros_like(v)
+    ho.hodlr_gemv(bt, "N", t, w, node)
+    assert np.linalg.norm(w - H @ v) <= 1e-12 * np.linalg.norm(H @ v)
+    z = np.zeros_like(v)
+    ho.hodlr_gemv(bt, "N", v, z, node)
+    ho.hodlr_lower(bt, z, node)
+    assert np.linalg.norm(z - v) <= 1e-12 * np.linalg.norm(v)
