@@ -149,11 +149,19 @@ class Matrix {
   // HODLR (symmetric) factor: per 2 x 2 node, the r x r blocks X11 and KK of
   // the Woodbury form (hodlr.cpp:163-285) in the factor's precision, and the
   // log of det(M11) (FP64)
+  // Non-symmetric HODLR factor (hodlr.cpp:45-116): per node the 2r x 2r block
+  // KK = I + [[0, b01^T a10], [b10^T a01, 0]] (LU-factorised, interchanges in
+  // hodlr_piv) and the panels a01, b01 of the transposed copy of the Rk block.
   struct HodlrNode {
     const Block* b;   // the 2 x 2 node; b->get(1, 0) is its Rk block
     int cap;          // capacity of that block's panels
-    int64_t x11, kk;  // offsets (elements) into hodlr_buf
+    int64_t x11, kk;  // offsets (elements) into hodlr_buf (symmetric: r x r each;
+                      // non-symmetric: no X11, KK of 2 cap x 2 cap)
+    int64_t a01 = -1, b01 = -1;  // non-symmetric: n0 x cap and n1 x cap panels
+    int64_t piv = -1;            // non-symmetric: 2 cap interchanges in hodlr_piv
   };
+  int64_t hodlr_elems = 0, hodlr_pivs = 0;
+  DevBuf hodlr_piv;
   std::vector<HodlrNode> hodlr_nodes;
   // dense inverses of the LLt / LDLt factor's diagonal blocks of at most
   // kInvMaxRows rows (built at the first solve, in the factor's precision):
@@ -221,7 +229,7 @@ class Matrix {
   void axpy_from(double alpha, Matrix& x);  // A <- A + alpha X, X of the same partition
   // this <- beta this + alpha op(a) op(b) (H-gemm), a and b over the same trees
   void gemm_from(char ta, char tb, double alpha, Matrix& a, Matrix& b, double beta);
-  void hodlr_layout();            // node list of a HODLR factor; checks the structure
+  void hodlr_layout(bool sym);    // node list of a HODLR factor; checks the structure
   void hodlr_alloc(size_t elem);  // its r x r blocks and log-determinants
   // b <- this^{-1} b for a factorised matrix and an H-matrix b (solve_mat);
   // part 1 / 2: L^{-1} b / L^{-T} b only (LLt)

@@ -2093,6 +2093,136 @@ class Exec {
     release();
   }
 
+  // ---- non-symmetric HODLR (hodlr.cpp:45-116) --------------------------------------
+  // The n0 x cap (which 2: a01) or n1 x cap (3: b01) panel of node q.
+  DV hodlr_panel(int q, int which) const {
+    const Matrix::HodlrNode& nd = M_.hodlr_nodes[q];
+    const Block* b10 = nd.b->get(1, 0);
+    DV d;
+    d.p = M_.hodlr_buf.template as<T>() + (which == 2 ? nd.a01 : nd.b01);
+    d.ld = which == 2 ? b10->n() : b10->m();
+    d.m = d.ld;
+    d.n = nd.cap;
+    d.buf = kHodlrBase + (int64_t(1) << 32) + 2 * (int64_t)q + (which - 2);
+    return d;
+  }
+  // KK of node q (2 cap x 2 cap, ld 2 cap)
+  DV hodlr_kk(int q) const {
+    const Matrix::HodlrNode& nd = M_.hodlr_nodes[q];
+    DV d;
+    d.p = M_.hodlr_buf.template as<T>() + nd.kk;
+    d.ld = d.m = d.n = 2 * nd.cap;
+    d.buf = kHodlrBase + 2 * (int64_t)q + 1;
+    return d;
+  }
+  // C <- I (a T_ZERO task, mode 1)
+  void identity(const DV& C) {
+    FTask t{};
+    t.type = T_ZERO;
+    t.mode = 1;
+    t.m = dim(C.m);
+    t.n = dim(C.n);
+    t.C = C.p;
+    t.ldc = C.ld;
+    wr(C);
+    emit(t);
+  }
+  // TRSM (modes and flags as trsm) with a dense triangular operand and its
+  // interchanges
+  void trsm_dense(int mode, const DV& L, int* piv, const DV& X, Dim other) {
+    FTask t{};
+    t.type = T_TRSM;
+    t.mode = mode;
+    t.n = dim(L.m);
+    t.m = other;
+    t.A = L.p;
+    t.lda = L.ld;
+    if (mode & kTrsmPiv) t.B = piv;
+    t.C = X.p;
+    t.ldc = X.ld;
+    rd(L);
+    rd(X);
+    wr(X);
+    emit(t);
+  }
+
+  // HODLR<T>::factorize / factorize hodlr.cpp:81-116: the children (leaves
+  // by LLt), a01 b01^T = (a10 b10^T)^T (desymmetrize), a01 <- M00^{-1} a01,
+  // a10 <- M11^{-1} a10, KK = I + [[0, b01^T a10], [b10^T a01, 0]], LU of KK.
+  void hodlr_factor_ns(const Block* a) {
+    const Block *a00 = a->get(0, 0), *a11 = a->get(1, 1), *a10 = a->get(1, 0);
+    if (a00->is_leaf()) potrf(a00); else hodlr_factor_ns(a00);
+    if (a11->is_leaf()) potrf(a11); else hodlr_factor_ns(a11);
+    const int q = M_.hodlr_index.at(a);
+    const int n0 = a00->m(), n1 = a11->m();
+    RV r = rk_of(a10);
+    const Dim rk = dim(r.cap, r.rank);
+    DV A10 = dv_panel(r.a, r.la, r.m, r.cap, r.bufa, r.row0);
+    DV B10 = dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0);
+    DV A01 = hodlr_panel(q, 2), B01 = hodlr_panel(q, 3);
+    dense_axpy(A01, 1.0, B10, dim(n0), rk, true);
+    dense_axpy(B01, 1.0, A10, dim(n1), rk, true);
+    hodlr_solve_ns(a00, A01, rk);
+    hodlr_solve_ns(a11, A10, rk);
+    DV K = hodlr_kk(q);
+    identity(K);
+    const int cap = r.cap;
+    DV K01 = K, K10 = K;
+    K01.p = (char*)K.p + sizeof(T) * (size_t)cap * K.ld;
+    K10.p = (char*)K.p + sizeof(T) * (size_t)cap;
+    K01.m = K01.n = K10.m = K10.n = cap;
+    DV B10t = B10, B01t = B01;
+    B10t.trans = B01t.trans = true;
+    gemm(K01, true, 1.0, B01t, A10, rk, rk, dim(n1));
+    gemm(K10, true, 1.0, B10t, A01, rk, rk, dim(n0));
+    FTask t{};
+    t.type = T_GETRF;
+    t.m = dim(K.m);
+    t.A = K.p;
+    t.lda = K.ld;
+    t.B = M_.hodlr_piv.template as<int>() + M_.hodlr_nodes[q].piv;
+    rd(K);
+    wr(K);
+    emit(t);
+    maybe_flush();
+  }
+
+  // X <- M^{-1} X with the non-symmetric HODLR factor of m (solve hodlr.cpp:
+  // 45-74): the leaves' LLt solve, the children, then the Woodbury step
+  // X -= U KK^{-1} V^T X with U = diag(a01, a10), V = [b01; b10] crosswise.
+  void hodlr_solve_ns(const Block* m, const DV& X, Dim ncol) {
+    if (m->is_leaf()) {
+      trsm(1, m, X, ncol);
+      trsm(2, m, X, ncol);
+      return;
+    }
+    const int n0 = m->get(0, 0)->m(), n1 = m->get(1, 1)->m();
+    DV X0 = rows_of(X, 0, n0), X1 = rows_of(X, n0, n1);
+    hodlr_solve_ns(m->get(0, 0), X0, ncol);
+    hodlr_solve_ns(m->get(1, 1), X1, ncol);
+    const int q = M_.hodlr_index.at(m);
+    RV r = rk_of(m->get(1, 0));
+    const Dim rk = dim(r.cap, r.rank);
+    const int cap = r.cap, w = std::max(ncol.s, 1);
+    DV A10 = dv_panel(r.a, r.la, r.m, r.cap, r.bufa, r.row0);
+    DV B10 = dv_panel(r.b, r.lb, r.n, r.cap, r.bufb, r.col0);
+    DV A01 = hodlr_panel(q, 2), B01 = hodlr_panel(q, 3);
+    DV B10t = B10, B01t = B01;
+    B10t.trans = B01t.trans = true;
+    DV tv = temp_dense(2 * cap, w, -1);
+    zero(tv);
+    DV tv0 = rows_of(tv, 0, cap), tv1 = rows_of(tv, cap, cap);
+    gemm(tv0, false, 1.0, B01t, X1, rk, ncol, dim(n1));
+    gemm(tv1, false, 1.0, B10t, X0, rk, ncol, dim(n0));
+    DV K = hodlr_kk(q);
+    int* piv = M_.hodlr_piv.template as<int>() + M_.hodlr_nodes[q].piv;
+    trsm_dense(1 | kTrsmUnit | kTrsmPiv, K, piv, tv, ncol);
+    trsm_dense(2 | kTrsmTransA, K, nullptr, tv, ncol);
+    gemm(X0, true, -1.0, A01, tv0, dim(n0), ncol, rk);
+    gemm(X1, true, -1.0, A10, tv1, dim(n1), ncol, rk);
+    release();
+  }
+
   // HMatrix::solveDiagonal h_matrix.cpp:2969-2983 (LDLt): X <- D^{-1} X, one
   // task per diagonal leaf
   void solve_diag(const DV& X, Dim ncol) {
@@ -2341,17 +2471,15 @@ void Matrix::factorize(int algo, bool fp32) {
   if (factorization != -1) throw std::runtime_error("factorize: matrix already factorised");
   if (embedded && algo != hmat_factorization_lu)
     throw std::runtime_error("factorize: complex matrices are factorised by hmat_factorization_lu");
-  if (algo == hmat_factorization_hodlr)
-    throw std::runtime_error("factorize: the non-symmetric HODLR factorisation is not provided "
-                             "(hmat_factorization_hodlrsym is)");
   if (algo != hmat_factorization_llt && algo != hmat_factorization_lu && algo != hmat_factorization_ldlt &&
-      algo != hmat_factorization_hodlrsym)
+      algo != hmat_factorization_hodlrsym && algo != hmat_factorization_hodlr)
     throw std::runtime_error("factorize: only hmat_factorization_llt, hmat_factorization_ldlt, "
-                             "hmat_factorization_hodlrsym and hmat_factorization_lu are provided");
-  if ((algo == hmat_factorization_llt || algo == hmat_factorization_ldlt ||
-       algo == hmat_factorization_hodlrsym) && !lower_sym)
+                             "hmat_factorization_hodlr, hmat_factorization_hodlrsym and "
+                             "hmat_factorization_lu are provided");
+  const bool hodlr_any = algo == hmat_factorization_hodlrsym || algo == hmat_factorization_hodlr;
+  if ((algo == hmat_factorization_llt || algo == hmat_factorization_ldlt || hodlr_any) && !lower_sym)
     throw std::runtime_error("factorize: LLt, LDLt and HODLR need lower-symmetric storage");
-  if (algo == hmat_factorization_hodlrsym) hodlr_layout();
+  if (hodlr_any) hodlr_layout(algo == hmat_factorization_hodlrsym);
   inv_blocks.clear();
   inv_index.clear();
   inv_buf.release();
@@ -2385,6 +2513,9 @@ void Matrix::factorize(int algo, bool fp32) {
     } else if (algo == hmat_factorization_hodlrsym) {
       hodlr_alloc(sizeof(float));
       ex.hodlr_factor(bt->root);
+    } else if (algo == hmat_factorization_hodlr) {
+      hodlr_alloc(sizeof(float));
+      ex.hodlr_factor_ns(bt->root);
     } else {
       ex.llt(bt->root);
     }
@@ -2405,6 +2536,9 @@ void Matrix::factorize(int algo, bool fp32) {
     } else if (algo == hmat_factorization_hodlrsym) {
       hodlr_alloc(sizeof(double));
       ex.hodlr_factor(bt->root);
+    } else if (algo == hmat_factorization_hodlr) {
+      hodlr_alloc(sizeof(double));
+      ex.hodlr_factor_ns(bt->root);
     } else {
       ex.llt(bt->root);
     }
@@ -2434,8 +2568,8 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part)
   if (sweep_env && atoi(sweep_env))
     throw std::runtime_error("solve: the persistent-sweep solve is disabled (it did not complete "
                              "on the device; cause not found)");
-  if (part != 0 && factorization == hmat_factorization_lu)
-    throw std::runtime_error("solve_lower_triangular: provided for LLt, LDLt and HODLR factors only");
+  if (part != 0 && (factorization == hmat_factorization_lu || factorization == hmat_factorization_hodlr))
+    throw std::runtime_error("solve_lower_triangular: provided for LLt, LDLt and symmetric HODLR factors only");
   const auto key = std::make_pair((int)sizeof(T) + 16 * part, nrhs);
   const size_t bytes = (size_t)n * nrhs * sizeof(T);
   auto it = fs.solve_plans.find(key);
@@ -2484,6 +2618,10 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part)
       // HMatrix::solve (LU) h_matrix.cpp:2772-2779
       ex.lu_solve_lower_left(bt->root, X, Exec<T>::dim(nrhs));
       ex.lu_solve_upper_left(bt->root, X, Exec<T>::dim(nrhs));
+    } else if (factorization == hmat_factorization_hodlr) {
+      // HODLR<T>::solve (DefaultEngine::solve default_engine.cpp:203-204,
+      // hodlr.cpp:434-437)
+      ex.hodlr_solve_ns(bt->root, X, Exec<T>::dim(nrhs));
     } else if (factorization == hmat_factorization_hodlrsym) {
       // HODLR<T>::solveSymLower then solveSymUpper (DefaultEngine::solve
       // default_engine.cpp:205-208, hodlr.cpp:455-470)
@@ -2691,10 +2829,10 @@ void Matrix::axpy_from(double alpha, Matrix& x) {
 // 394-404); the matrix must have the HODLR structure: every inner block a
 // lower-symmetric 2 x 2 node whose (1, 0) child is an Rk leaf, every leaf
 // below the diagonal Rk, the diagonal leaves full.
-void Matrix::hodlr_layout() {
+void Matrix::hodlr_layout(bool sym) {
   hodlr_nodes.clear();
   hodlr_index.clear();
-  int64_t off = 0;
+  int64_t off = 0, poff = 0;
   std::function<void(const Block*)> walk = [&](const Block* b) {
     if (b->is_leaf()) {
       if (b->kind != Kind::Full) throw std::runtime_error("Not HODLR matrix");
@@ -2705,19 +2843,32 @@ void Matrix::hodlr_layout() {
       throw std::runtime_error("Not HODLR matrix");
     const int cap = s64.hdesc.at(b10->leaf).cap;
     hodlr_index[b] = (int)hodlr_nodes.size();
-    hodlr_nodes.push_back(HodlrNode{b, cap, off, off + (int64_t)cap * cap});
-    off += 2 * (int64_t)cap * cap;
+    if (sym) {
+      hodlr_nodes.push_back(HodlrNode{b, cap, off, off + (int64_t)cap * cap});
+      off += 2 * (int64_t)cap * cap;
+    } else {
+      HodlrNode nd{b, cap, -1, off};
+      off += 4 * (int64_t)cap * cap;
+      nd.a01 = off;
+      off += (int64_t)b10->n() * cap;
+      nd.b01 = off;
+      off += (int64_t)b10->m() * cap;
+      nd.piv = poff;
+      poff += 2 * (int64_t)cap;
+      hodlr_nodes.push_back(nd);
+    }
     walk(b->get(0, 0));
     walk(b->get(1, 1));
   };
   if (bt->root->is_leaf()) throw std::runtime_error("Not HODLR matrix");
   walk(bt->root);
+  hodlr_elems = off;
+  hodlr_pivs = poff;
 }
 
 void Matrix::hodlr_alloc(size_t elem) {
-  int64_t elems = 1;
-  for (const HodlrNode& nd : hodlr_nodes) elems = std::max(elems, nd.kk + (int64_t)nd.cap * nd.cap);
-  hodlr_buf.alloc(elem * (size_t)elems);
+  hodlr_buf.alloc(elem * (size_t)std::max<int64_t>(hodlr_elems, 1));
+  hodlr_piv.alloc(sizeof(int) * (size_t)std::max<int64_t>(hodlr_pivs, 1));
   hodlr_logdet.alloc(sizeof(double) * std::max<size_t>(hodlr_nodes.size(), 1));
   HMX_CHECK(hipMemsetAsync(hodlr_logdet.p, 0, hodlr_logdet.bytes, stream));
 }
@@ -2767,7 +2918,7 @@ void Matrix::gemm_from(char ta, char tb, double alpha, Matrix& a, Matrix& b, dou
 
 void Matrix::solve_mat(Matrix& b, int part) {
   if (factorization == -1) throw std::runtime_error("solve_mat: matrix not factorised");
-  if (factorization == hmat_factorization_hodlrsym)
+  if (factorization == hmat_factorization_hodlrsym || factorization == hmat_factorization_hodlr)
     throw std::runtime_error("solve_mat: H-matrix right-hand sides are not supported with HODLR factors");
   if (part != 0 && factorization != hmat_factorization_llt)
     throw std::runtime_error("solveLower only supported by LLt factorization.");

@@ -59,7 +59,7 @@ Matrix* Matrix::clone(bool structure_only) {
         c->init_store(c->s32);
         copy_store(c->s32, s32, c->stream);
       }
-      if (factorization == hmat_factorization_hodlrsym)
+      if (factorization == hmat_factorization_hodlrsym || factorization == hmat_factorization_hodlr)
         throw std::runtime_error("copy: HODLR factors cannot be copied");
       c->factorization = factorization;
       c->factor32 = factor32;

@@ -18,7 +18,7 @@ below (paths are relative to the hmat-oss source tree):
 * H-gemm / axpy              h_matrix.cpp:870-1073, 1202-1620, 1874-1950
 * H-LLt and triangular solves recursion.cpp:86-134, 191-233, 349-444,
                               h_matrix.cpp:2297-2564, 3010-3018
-* HODLR (symmetric)          hodlr.cpp:113-166, 221-312, 325-378, 365-404,
+* HODLR                      hodlr.cpp:37-166, 221-312, 325-378, 365-404,
                               admissibility.cpp:231-241
 * H-LDLt                     recursion.cpp:37-83, h_matrix.cpp:1953-1998,
                               2585-2690, 2747-2769, 2969-3002,
@@ -1462,6 +1462,67 @@ def hodlr_lower(m, x, node):
         return
     x1 -= rk.a @ (rk.b.T @ x0)
     x1 -= rk.a @ (node.kk @ (rk.a.T @ x1))
+
+
+class HodlrNodeNs:
+    """Per 2 x 2 node of a non-symmetric HODLR factor: KK = I + [[0, b01^T a10],
+    [b10^T a01, 0]] as an LU factorisation, and the transposed copy (a01, b01)
+    of the node's Rk block (hodlr.cpp:37-43, 81-116, 384-428)."""
+    __slots__ = ("kk", "a01", "b01", "child0", "child1")
+
+    def __init__(self):
+        self.kk = self.a01 = self.b01 = self.child0 = self.child1 = None
+
+
+def hodlr_solve_ns(m, x, node):
+    """x <- M^-1 x with a non-symmetric HODLR factor (solve hodlr.cpp:45-74)."""
+    import scipy.linalg as sla
+    if m.is_leaf():
+        x[:] = _tri_solve(m.full, _tri_solve(m.full, x, lower=True), lower=True, trans=True)
+        return
+    n0 = m.get(0, 0).m
+    x0, x1 = x[:n0], x[n0:]
+    hodlr_solve_ns(m.get(0, 0), x0, node.child0)
+    hodlr_solve_ns(m.get(1, 1), x1, node.child1)
+    rk = m.get(1, 0).rk
+    r = rk.rank
+    if r == 0:
+        return
+    tv = np.vstack([node.b01.T @ x1, rk.b.T @ x0])
+    tv = sla.lu_solve(node.kk, tv)
+    x0 -= node.a01 @ tv[:r]
+    x1 -= rk.a @ tv[r:]
+
+
+def hodlr_factor_ns(a):
+    """HODLR<T>::factorize (factorize hodlr.cpp:81-116) of a lower-stored HODLR
+    tree in place: leaves by Cholesky, a01 b01^T = (a10 b10^T)^T, a01 <- M00^-1
+    a01, a10 <- M11^-1 a10, KK = I + [[0, b01^T a10], [b10^T a01, 0]] by LU."""
+    import scipy.linalg as sla
+    a00, a11, a10 = a.get(0, 0), a.get(1, 1), a.get(1, 0)
+    if a10 is None or a10.kind != RK or a.get(0, 1) is not None:
+        raise ValueError("Not HODLR matrix")
+    node = HodlrNodeNs()
+    for k, ch in ((0, a00), (1, a11)):
+        if ch.is_leaf():
+            ch.full = np.linalg.cholesky(ch.full)
+        elif k == 0:
+            node.child0 = hodlr_factor_ns(ch)
+        else:
+            node.child1 = hodlr_factor_ns(ch)
+    r = a10.rk.rank
+    if r == 0:
+        return node
+    node.a01, node.b01 = a10.rk.b.copy(), a10.rk.a.copy()
+    hodlr_solve_ns(a00, node.a01, node.child0)
+    ua = a10.rk.a.copy()
+    hodlr_solve_ns(a11, ua, node.child1)
+    a10.rk = Rk(ua, a10.rk.b)
+    kk = np.eye(2 * r)
+    kk[:r, r:] = node.b01.T @ ua
+    kk[r:, :r] = a10.rk.b.T @ node.a01
+    node.kk = sla.lu_factor(kk)
+    return node
 
 
 def hodlr_gemv(m, trans, x, y, node):

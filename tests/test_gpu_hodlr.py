@@ -1,7 +1,7 @@
-"""GPU test of the symmetric HODLR factorisation (hmat_create_admissibility_hodlr,
-factorize_generic with hmat_factorization_hodlrsym; hodlr.cpp:113-166,
-221-312): partition, factor, solve, lower / upper solves and logdet against
-the oracle's restatement on the same assembled blocks and against the exact
+"""GPU test of the HODLR factorisations (hmat_create_admissibility_hodlr,
+factorize_generic with hmat_factorization_hodlrsym, hodlr.cpp:113-166,
+221-312, and hmat_factorization_hodlr, hodlr.cpp:45-116): partition, factor,
+solve, lower / upper solves and logdet against the oracle's restatement on the same assembled blocks and against the exact
 dense algebra of the assembled matrix (the HODLR factorisation truncates
 nothing), beside the H-LLt of the same matrix (the reference's
 examples/hodlrvsllt.c comparison)."""
@@ -80,6 +80,41 @@ def test_hodlr_sym_matches_oracle(n, leaf, height):
 
 
 @pytest.mark.gpu
+def test_hodlr_nonsym_matches_oracle():
+    """hmat_factorization_hodlr (HODLR<T>::factorize / solve, hodlr.cpp:45-116):
+    the same lower-stored matrix through the non-symmetric Woodbury form."""
+    n, leaf = 3000, 64
+    say(f"hodlr (non-symmetric) n={n} leaf={leaf}")
+    pts = ho.cylinder_points(n, 1.0, 1.5, 3)
+    root, idx = ho.build_cluster_tree(pts, leaf)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    h = hmat_amd.HMatrix(t, hodlr=True)
+    h.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    H = h.to_dense()
+    h.factorize(hmat_amd.FACT_HODLR)
+    b = np.random.default_rng(4).standard_normal((n, 4))
+    x = h.solve(b)
+    bt = ho.build_block_tree(root, root, 2.0, True, hodlr=True)
+    ho.assemble(bt, idx, ho.ExpKernel(pts, 0.5, 1e-2))
+    node = ho.hodlr_factor_ns(bt)
+    xo = b[idx].copy()
+    ho.hodlr_solve_ns(bt, xo, node)
+    xou = np.empty_like(xo)
+    xou[idx] = xo
+    xe = np.linalg.solve(H, b[idx])
+    rel_o = np.linalg.norm(x - xou) / np.linalg.norm(xou)
+    rel_e = np.linalg.norm(x[idx] - xe) / np.linalg.norm(xe)
+    say(f"solution vs oracle {rel_o:.2e}, vs dense solve of H {rel_e:.2e}")
+    assert rel_o <= 1e-11 and rel_e <= 1e-11
+    # what the reference's non-symmetric HODLR factor does not provide either
+    with pytest.raises(hmat_amd.EngineError, match="logdet"):
+        h.logdet()
+    with pytest.raises(hmat_amd.EngineError, match="solve_lower_triangular"):
+        h.solve_lower(b)
+    h.close()
+
+
+@pytest.mark.gpu
 def test_hodlr_mixed_precision_and_errors():
     import torch
     n, leaf = 2000, 64
@@ -99,8 +134,7 @@ def test_hodlr_mixed_precision_and_errors():
     say(f"{it} sweeps, residual {res.max():.2e}")
     assert res.max() <= 1e-10
     h.close()
-    # a standard-admissibility partition is not HODLR; the non-symmetric HODLR
-    # factorisation is not provided
+    # a standard-admissibility partition is not HODLR
     s = hmat_amd.HMatrix(t)
     s.assemble(length=0.5, nugget=1e-2, eps=EPS)
     with pytest.raises(hmat_amd.EngineError, match="Not HODLR"):

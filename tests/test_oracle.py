@@ -331,6 +331,19 @@ def test_hodlr_partition_rule():
     walk(bt)
 
 
+def test_hodlr_nonsym_is_exact_factorisation():
+    # the non-symmetric HODLR factorisation of the same lower-stored matrix
+    # (Woodbury with an LU-factorised 2r x 2r core) solves exactly too
+    n = 2000
+    bt, H = _hodlr(n, 64)
+    node = ho.hodlr_factor_ns(bt)
+    b = np.random.default_rng(8).standard_normal((n, 3))
+    x = b.copy()
+    ho.hodlr_solve_ns(bt, x, node)
+    xe = np.linalg.solve(H, b)
+    assert np.linalg.norm(x - xe) <= 1e-10 * np.linalg.norm(xe)
+
+
 def test_hodlr_sym_is_exact_factorisation():
     # the HODLR factorisation truncates nothing: its solve is the exact solve
     # of the assembled matrix, its logdet half the log-determinant, and
