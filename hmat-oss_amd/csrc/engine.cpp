@@ -2,6 +2,9 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <utility>
+
+#include "factor_kernels.hpp"
 #include <chrono>
 #include <cstring>
 #include <functional>
@@ -163,6 +166,57 @@ void upload_points(const KernelSpec& k, const ClusterTree& t, DevBuf& pts, DevBu
 }
 }  // namespace
 
+// truncatedSvd (compression.cpp:224-239) of leaves of 100 rows / cols or more:
+// each dense block (column-major, ld m) at dense + off[i] becomes the leaf's
+// Rk panels by a dense recompression task (a one-sided Jacobi SVD of the
+// block, cut at eps) in batches bounded by the tasks' workspace.
+void Matrix::svd_compress_dense(const std::vector<std::pair<int, int64_t>>& blocks,
+                                const double* dense, double eps) {
+  const int64_t budget = (int64_t)1 << 28;  // doubles of workspace per launch (2 GiB)
+  DevBuf dt, dws, dfl;
+  dfl.alloc(2 * sizeof(int));
+  size_t i = 0;
+  while (i < blocks.size()) {
+    std::vector<FTask> tasks;
+    int64_t ws = 0;
+    while (i < blocks.size()) {
+      const LeafDesc& L = s64.hdesc[blocks[i].first];
+      const int64_t need = fadd_dense_workspace(L.m, L.n);
+      if (!tasks.empty() && ws + need > budget) break;
+      FTask t{};
+      t.type = T_FADD_DENSE;
+      t.mode = 1;  // the target's panels are replaced
+      t.eps = eps;
+      t.alpha = 1.0;
+      t.A = const_cast<double*>(dense) + blocks[i].second;
+      t.lda = L.m;
+      t.m = Dim{L.m, nullptr};
+      t.n = Dim{L.n, nullptr};
+      t.tgt.a = s64.data() + L.a;
+      t.tgt.b = s64.data() + L.b;
+      t.tgt.la = L.m;
+      t.tgt.lb = L.n;
+      t.tgt.m = L.m;
+      t.tgt.n = L.n;
+      t.tgt.cap = L.cap;
+      t.tgt.rank = s64.ranks.as<int>() + blocks[i].first;
+      t.ws = ws;
+      ws += need;
+      tasks.push_back(t);
+      ++i;
+    }
+    upload(dt, tasks.data(), tasks.size() * sizeof(FTask), stream);
+    dws.ensure(sizeof(double) * (size_t)std::max<int64_t>(ws, 1));
+    HMX_CHECK(hipMemsetAsync(dfl.p, 0, 2 * sizeof(int), stream));
+    launch_task_kernel<double>(T_FADD_DENSE, dt.as<FTask>(), (int)tasks.size(), nullptr, 0,
+                               dws.as<double>(), dfl.as<int>(), stream);
+    int fl[2];
+    HMX_CHECK(hipMemcpyAsync(fl, dfl.p, sizeof(fl), hipMemcpyDeviceToHost, stream));
+    HMX_CHECK(hipStreamSynchronize(stream));
+    if (fl[0]) throw std::runtime_error("assembly: an Rk rank exceeded the panel capacity");
+  }
+}
+
 void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
   const double t0 = now();
   device_init();
@@ -177,22 +231,23 @@ void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
   init_store(s64);
   HMX_CHECK(hipMemsetAsync(overflow_.p, 0, sizeof(int), stream));
 
-  std::vector<int> full, svd;
+  std::vector<int> full, svd, big_svd;
   std::vector<int> aca;
   for (size_t i = 0; i < bt->leaves.size(); ++i) {
     const LeafDesc& L = s64.hdesc[i];
     if (L.kind == 0) {
       full.push_back((int)i);
     } else if (embedded) {
-      // complex blocks (twice the rows and columns in the embedding) go through
-      // ACA and the truncation at any size
+      // complex blocks (twice the rows and columns in the embedding): ACA and
+      // the truncation at any size, or the dense SVD when asked for
       if (comp.method == 0)
-        throw std::runtime_error("SVD compression of complex blocks is not provided; use ACA");
-      aca.push_back((int)i);
-    } else if (std::max(L.m, L.n) < 100 || comp.method == 0 /*svd*/) {
-      if (std::max(L.m, L.n) >= 100)
-        throw std::runtime_error("SVD compression is implemented for blocks below 100 rows/cols");
+        big_svd.push_back((int)i);
+      else
+        aca.push_back((int)i);
+    } else if (std::max(L.m, L.n) < 100) {
       svd.push_back((int)i);
+    } else if (comp.method == 0 /*svd*/) {
+      big_svd.push_back((int)i);
     } else {
       aca.push_back((int)i);
     }
@@ -208,6 +263,37 @@ void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
   launch_svd_compress<double>(s64.desc.as<LeafDesc>(), dsvd.as<int>(), (int)svd.size(), s64.data(),
                               s64.ranks.as<int>(), overflow_.as<int>(), rp, cp, kp,
                               std::max(comp.eps, lowrank_eps), stream);
+  // larger blocks with SVD compression: evaluated densely into a staging
+  // buffer (the leaf list's descriptors pointed at it), then recompressed
+  {
+    const int64_t kStageElems = (int64_t)1 << 27;  // 1 GiB of doubles
+    size_t i = 0;
+    DevBuf stage, sdesc, slist;
+    while (i < big_svd.size()) {
+      std::vector<LeafDesc> d;
+      std::vector<int> lst;
+      std::vector<std::pair<int, int64_t>> blocks;
+      int64_t off = 0;
+      while (i < big_svd.size()) {
+        LeafDesc L = s64.hdesc[big_svd[i]];
+        const int64_t e = (int64_t)L.m * L.n;
+        if (!blocks.empty() && off + e > kStageElems) break;
+        blocks.push_back({big_svd[i], off});
+        L.a = off;
+        lst.push_back((int)d.size());
+        d.push_back(L);
+        off += (e + 63) / 64 * 64;
+        ++i;
+      }
+      HMX_CHECK(hipStreamSynchronize(stream));
+      stage.ensure(sizeof(double) * (size_t)off);
+      upload(sdesc, d.data(), d.size() * sizeof(LeafDesc), stream);
+      upload(slist, lst.data(), lst.size() * sizeof(int), stream);
+      launch_assemble_full<double>(sdesc.as<LeafDesc>(), slist.as<int>(), (int)lst.size(),
+                                   stage.as<double>(), rp, cp, kp, stream);
+      svd_compress_dense(blocks, stage.as<double>(), std::max(comp.eps, lowrank_eps));
+    }
+  }
   // ACA blocks in batches bounded by a workspace budget.
   const int64_t budget = (int64_t)4 << 27;  // doubles (4 GiB)
   const int partial = comp.method == 2;
@@ -357,6 +443,7 @@ void Matrix::assemble_host(const HostFunctions& f, const CompressionSpec& comp) 
     host.resize((size_t)std::max<int64_t>(used, 1));
     std::vector<int> full, svd;
     std::vector<int64_t> full_off, svd_off;
+    std::vector<std::pair<int, int64_t>> big_svd;
     std::vector<AcaTask> aca;
     int64_t off = 0;
     for (int l = l0; l < l1; ++l) {
@@ -369,17 +456,17 @@ void Matrix::assemble_host(const HostFunctions& f, const CompressionSpec& comp) 
       if (L.kind == 0) {
         full.push_back(l);
         full_off.push_back(off);
+      } else if (embedded && comp.method == 0) {
+        big_svd.push_back({l, off});
       } else if (embedded) {
-        if (comp.method == 0)
-          throw std::runtime_error("SVD compression of complex blocks is not provided; use ACA");
         AcaTask t{l, std::min(std::min(L.m, L.n), 1024), 0};
         t.tab = off;
         aca.push_back(t);
-      } else if (std::max(L.m, L.n) < 100 || comp.method == 0 /*svd*/) {
-        if (std::max(L.m, L.n) >= 100)
-          throw std::runtime_error("SVD compression is implemented for blocks below 100 rows/cols");
+      } else if (std::max(L.m, L.n) < 100) {
         svd.push_back(l);
         svd_off.push_back(off);
+      } else if (comp.method == 0 /*svd*/) {
+        big_svd.push_back({l, off});
       } else {
         AcaTask t{l, std::min(std::min(L.m, L.n), 1024), 0};
         t.tab = off;
@@ -400,6 +487,8 @@ void Matrix::assemble_host(const HostFunctions& f, const CompressionSpec& comp) 
                                 s64.ranks.as<int>(), overflow_.as<int>(), none, none, kp,
                                 std::max(comp.eps, lowrank_eps), stream, doff.as<int64_t>(),
                                 table.as<double>());
+    if (!big_svd.empty())
+      svd_compress_dense(big_svd, table.as<double>(), std::max(comp.eps, lowrank_eps));
     size_t i = 0;
     while (i < aca.size()) {
       std::vector<AcaTask> batch;

@@ -243,6 +243,8 @@ class Matrix {
 
  private:
   void build_storage(const std::vector<int>* cap_hint = nullptr);
+  void svd_compress_dense(const std::vector<std::pair<int, int64_t>>& blocks, const double* dense,
+                          double eps);
   void finish_assembly(double t0);
   template <typename T>
   void init_store(Store<T>& st);

@@ -118,6 +118,34 @@ def test_aca_partial_and_svd_methods():
         h.close()
 
 
+@pytest.mark.gpu
+def test_svd_compression_every_size():
+    """hmat_create_compression_svd (truncatedSvd, compression.cpp:224-239) on
+    blocks of every size: below 100 rows by k_svd_compress, above by a dense
+    recompression task; leaves equal the oracle's truncated SVDs."""
+    n = 3000
+    say(f"svd compression n={n}")
+    pts = ho.cylinder_points(n, 1.0, 1.2, 23)
+    t = hmat_amd.ClusterTree(pts, 128)
+    h = hmat_amd.HMatrix(t)
+    h.assemble(eps=EPS, method="svd")
+    root, idx = ho.build_cluster_tree(pts, 128)
+    bt = ho.build_block_tree(root, root, 2.0, True)
+    ho.assemble(bt, idx, ho.ExpKernel(pts, 0.5, 1e-2), method="svd")
+    worst, mism, big = 0.0, 0, 0
+    for ob, (m, nn, r0, c0, kind, rank, a, b) in zip(ho.leaves(bt), h.leaves()):
+        if kind == 0:
+            continue
+        big += max(m, nn) >= 100
+        g = a @ b.T if rank > 0 else np.zeros((m, nn))
+        o = ob.rk.eval(m, nn, np.float64)
+        worst = max(worst, np.linalg.norm(g - o) / max(np.linalg.norm(o), 1e-300))
+        mism += rank != ob.rk.rank
+    say(f"svd leaves vs oracle: worst {worst:.2e}, rank mismatches {mism}, {big} blocks of 100 rows or more")
+    assert big > 0 and mism == 0 and worst <= 1e-10
+    h.close()
+
+
 # N=6000 on a cylinder of height 8 with leaf 64 has Rk panels of up to 1500
 # rows (oracle partition, eta 2), so the one-RHS z pass sees chunks in every
 # row class (<= 64, 128, 256, 512 and 1024 rows), including full 1024-row

@@ -197,3 +197,26 @@ def test_complex_block_callback_and_device_entries():
     assert abs(h.norm() - 2 * n0) <= 1e-12 * n0
     h.close()
     g.close()
+
+
+@pytest.mark.gpu
+def test_complex_svd_compression():
+    n, leaf = 800, 32
+    say(f"complex svd compression n={n}")
+    pts, hreg, k = problem(n, 0.3, 8)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    h = hmat_amd.HMatrix(t, lower_symmetric=False, value_type=hmat_amd.HMAT_DOUBLE_COMPLEX)
+    h.assemble(kind=hmat_amd.KERNEL_HELMHOLTZ, wavenumber=k, reg=hreg, nugget=1.0, eps=EPS,
+               method="svd")
+    idx = t.indices
+    ker = ho.HelmholtzKernel(pts, k, hreg, 1.0)
+    exact = ker.block(idx, idx)
+    err = np.linalg.norm(h.to_dense() - exact) / np.linalg.norm(exact)
+    root, _ = ho.build_cluster_tree(pts, leaf)
+    bt = ho.build_block_tree(root, root, 2.0, False)
+    with ho.scalar_type(np.complex128):
+        ho.assemble(bt, idx, ker, method="svd", eps=EPS)
+    ratio = h.info()["compressed_size"] / ho.info(bt)["compressed_size"]
+    say(f"complex svd: vs exact kernel {err:.2e}, compressed size / oracle's {ratio:.4f}")
+    assert err <= EPS and abs(ratio - 1) <= 0.02
+    h.close()
