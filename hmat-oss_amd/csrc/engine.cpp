@@ -213,12 +213,53 @@ void Matrix::svd_compress_dense(const std::vector<std::pair<int, int64_t>>& bloc
     int fl[2];
     HMX_CHECK(hipMemcpyAsync(fl, dfl.p, sizeof(fl), hipMemcpyDeviceToHost, stream));
     HMX_CHECK(hipStreamSynchronize(stream));
-    if (fl[0]) throw std::runtime_error("assembly: an Rk rank exceeded the panel capacity");
+    if (fl[0]) HMX_CHECK(hipMemcpy(overflow_.p, &fl[0], sizeof(int), hipMemcpyHostToDevice));
   }
+}
+
+// An Rk rank above the panel capacity during assembly: every leaf's capacity
+// limit doubles (up to its min(m, n)) and the assembly runs again.
+bool Matrix::assembly_overflow() {
+  HMX_CHECK(hipStreamSynchronize(stream));
+  int ov = 0;
+  HMX_CHECK(hipMemcpy(&ov, overflow_.p, sizeof(int), hipMemcpyDeviceToHost));
+  return ov != 0;
+}
+
+void Matrix::grow_rank_capacity() {
+  int widest = 0;
+  for (const LeafDesc& L : s64.hdesc)
+    if (L.kind == 1) widest = std::max(widest, std::min(L.m, L.n));
+  if (rk_cap_max >= widest) throw std::runtime_error("assembly: an Rk rank exceeded the panel capacity");
+  rk_cap_max = std::min(2 * rk_cap_max, widest);
+  s64.arena.release();
+  s64.ranks.release();
+  s64.desc.release();
+  s64.valid = false;
+  build_storage();
 }
 
 void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
   const double t0 = now();
+  for (;;) {
+    assemble_pass(k, comp);
+    if (!assembly_overflow()) break;
+    grow_rank_capacity();
+  }
+  finish_assembly(t0);
+}
+
+void Matrix::assemble_host(const HostFunctions& f, const CompressionSpec& comp) {
+  const double t0 = now();
+  for (;;) {
+    assemble_host_pass(f, comp);
+    if (!assembly_overflow()) break;
+    grow_rank_capacity();
+  }
+  finish_assembly(t0);
+}
+
+void Matrix::assemble_pass(const KernelSpec& k, const CompressionSpec& comp) {
   device_init();
   upload_points(k, *rows, pts_rows, user_rows, stream);
   if (cols != rows) upload_points(k, *cols, pts_cols, user_cols, stream);
@@ -318,14 +359,10 @@ void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
                        s64.ranks.as<int>(), overflow_.as<int>(), ws.as<double>(), rp, cp, kp,
                        comp.eps, lowrank_eps, partial, stream);
   }
-  finish_assembly(t0);
 }
 
 void Matrix::finish_assembly(double t0) {
   HMX_CHECK(hipStreamSynchronize(stream));
-  int ov = 0;
-  HMX_CHECK(hipMemcpy(&ov, overflow_.p, sizeof(int), hipMemcpyDeviceToHost));
-  if (ov) throw std::runtime_error("assembly: an Rk rank exceeded the panel capacity");
   s64.valid = true;
   if (value_type == HMAT_SIMPLE_PRECISION) {
     init_store(s32);
@@ -414,8 +451,7 @@ void evaluate_block_complex(const HostFunctions& f, const ClusterTree& zr, const
 }
 }  // namespace
 
-void Matrix::assemble_host(const HostFunctions& f, const CompressionSpec& comp) {
-  const double t0 = now();
+void Matrix::assemble_host_pass(const HostFunctions& f, const CompressionSpec& comp) {
   if (!f.simple && !((f.block || f.advanced) && f.prepare))
     throw std::runtime_error("assembly: set simple_compute, or block_compute / advanced_compute with prepare");
   device_init();
@@ -512,7 +548,6 @@ void Matrix::assemble_host(const HostFunctions& f, const CompressionSpec& comp) 
     }
     l0 = l1;
   }
-  finish_assembly(t0);
 }
 
 // ---------------------------------------------------------------------------

@@ -243,6 +243,10 @@ class Matrix {
 
  private:
   void build_storage(const std::vector<int>* cap_hint = nullptr);
+  void assemble_pass(const KernelSpec& k, const CompressionSpec& c);
+  void assemble_host_pass(const HostFunctions& f, const CompressionSpec& c);
+  bool assembly_overflow();
+  void grow_rank_capacity();
   void svd_compress_dense(const std::vector<std::pair<int, int64_t>>& blocks, const double* dense,
                           double eps);
   void finish_assembly(double t0);

@@ -211,3 +211,30 @@ def test_gemv_fp32_storage():
     say("fp32 gemv rel err", err)
     assert err < 1e-6
     h.close()
+
+
+@pytest.mark.gpu
+def test_rank_capacity_grows_during_assembly():
+    """A panel capacity below the ranks (hmx_set_rank_capacity(4)): the
+    assembly doubles the capacity limit and runs again until the ranks fit;
+    the leaves equal those of an assembly at the default capacity."""
+    n = 3000
+    say(f"rank capacity growth n={n}")
+    pts = ho.cylinder_points(n, 1.0, 1.2, 29)
+    t = hmat_amd.ClusterTree(pts, 128)
+    lib = hmat_amd.load()
+    lib.hmx_set_rank_capacity(4)
+    try:
+        small = hmat_amd.HMatrix(t)
+        small.assemble(eps=EPS)
+    finally:
+        lib.hmx_set_rank_capacity(0)    # back to the default
+    ref = hmat_amd.HMatrix(t)
+    ref.assemble(eps=EPS)
+    ranks = ref.leaf_table()[:, 5]
+    diff = np.linalg.norm(small.to_dense() - ref.to_dense()) / np.linalg.norm(ref.to_dense())
+    same = np.array_equal(small.leaf_table()[:, 5], ranks)
+    say(f"largest rank {ranks.max()}, ranks equal {same}, dense diff {diff:.2e}")
+    assert ranks.max() > 4 and same and diff <= 1e-14
+    small.close()
+    ref.close()
