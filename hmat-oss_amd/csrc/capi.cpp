@@ -152,8 +152,20 @@ int destroy(hmat_matrix_t* h) {
   return 0;
 }
 
+// factorize_generic c_wrapping.hpp:139-145: the context's progress reporter
+// (HMatInterface::factorize sets its max to N, hmat_cpp_interface.cpp:74-83)
 int factorize_generic(hmat_matrix_t* h, hmat_factorization_context_t* ctx) {
-  return guard([&] { M(h)->factorize(ctx->factorization, false); });
+  return guard([&] {
+    Matrix* m = M(h);
+    hmat_progress_t* saved = m->progress;
+    if (ctx->progress) m->progress = ctx->progress;
+    struct Restore {
+      Matrix* m;
+      hmat_progress_t* p;
+      ~Restore() { m->progress = p; }
+    } restore{m, saved};
+    m->factorize(ctx->factorization, false);
+  });
 }
 
 int logdet(hmat_matrix_t* h, double* res) {
@@ -373,7 +385,8 @@ void read_data(hmat_matrix_t* h, hmat_iostream f, void* ud) {
 void write_data(hmat_matrix_t* h, hmat_iostream f, void* ud) {
   guard([&] { hmx::serialize_write_data(*M(h), f, ud); });
 }
-void u_progress(hmat_matrix_t*, hmat_progress_t*) {}
+// set_progressbar c_wrapping.hpp:949-951: the reporter used by later factorisations
+void set_progressbar(hmat_matrix_t* h, hmat_progress_t* p) { M(h)->progress = p; }
 int u_diag_block(hmat_matrix_t*, int, void*) { return unsupported("extract_diagonal_block"); }
 
 template <int VT, typename T>
@@ -435,7 +448,7 @@ void fill(hmat_interface_t* i) {
   i->write_struct = write_struct;
   i->read_data = read_data;
   i->write_data = write_data;
-  i->set_progressbar = u_progress;
+  i->set_progressbar = set_progressbar;
   i->extract_diagonal_block = u_diag_block;
 }
 
@@ -482,7 +495,15 @@ int factorize_z(hmat_matrix_t* h, hmat_factorization_context_t* ctx) {
   return guard([&] {
     if (ctx->factorization != hmat_factorization_lu)
       throw std::runtime_error("complex matrices: only hmat_factorization_lu is provided");
-    MZ(h)->factorize(ctx->factorization, false);
+    Matrix* m = MZ(h);
+    hmat_progress_t* saved = m->progress;
+    if (ctx->progress) m->progress = ctx->progress;
+    struct Restore {
+      Matrix* m;
+      hmat_progress_t* p;
+      ~Restore() { m->progress = p; }
+    } restore{m, saved};
+    m->factorize(ctx->factorization, false);
   });
 }
 

@@ -143,6 +143,10 @@ class Matrix {
   Store<float> s32;
   bool assembled = false;
   int factorization = -1;    // hmat_factorization_t
+  // progress of a factorisation (hmat_factorization_context_t.progress,
+  // hmat_interface_t.set_progressbar): max = N, current = the end row of the
+  // last diagonal leaf factorised, updated as task segments complete
+  hmat_progress_t* progress = nullptr;
   bool factor32 = false;      // factor stored in s32 (mixed precision)
   DevBuf pivots;              // LU: getrf interchanges of each diagonal leaf, at its rows
   DevBuf ldlt_diag;           // LDLt: D at the matrix's rows, in the factor's precision

@@ -542,6 +542,7 @@ class Exec {
 
   // ---- leaf operations ---------------------------------------------------------
   void potrf(const Block* b) {
+    prog_row_ = std::max(prog_row_, b->rows->offset + b->m());
     DV d = full_of(b);
     FTask t{};
     t.type = T_POTRF;
@@ -556,6 +557,7 @@ class Exec {
   // HMatrix::luDecomposition at a leaf (full_matrix.cpp:230-237): getrf in
   // place, row interchanges into the matrix's pivot array at the leaf's rows.
   void getrf(const Block* b) {
+    prog_row_ = std::max(prog_row_, b->rows->offset + b->m());
     DV d = full_of(b);
     FTask t{};
     t.type = T_GETRF;
@@ -1630,6 +1632,7 @@ class Exec {
   }
 
   void ldlt_leaf(const Block* b) {
+    prog_row_ = std::max(prog_row_, b->rows->offset + b->m());
     DV a = full_of(b);
     DV dg = diag_of(b->rows->offset, b->m());
     FTask t{};
@@ -2290,6 +2293,8 @@ class Exec {
   }
 
   // ---- execution -----------------------------------------------------------------------
+  int prog_row_ = 0;  // end row of the last diagonal leaf factorisation emitted
+
   void maybe_flush() {
     if (live_ != 0) return;
     if (tasks_.size() >= kMaxTasks || fs_.temp_used > fs_.temp_budget / 2) flush(false);
@@ -2300,6 +2305,13 @@ class Exec {
       throw std::runtime_error("task segment limit reached while temporaries are live");
     if (tasks_.empty()) return;
     run_segment();
+    if (M_.progress && M_.progress->update) {
+      // hmat_progress_t (h_matrix.cpp:2554-2557): the end row of the last
+      // diagonal leaf factorised, once the segment holding it is done
+      HMX_CHECK(hipStreamSynchronize(M_.stream));
+      M_.progress->current = M_.embedded ? prog_row_ / 2 : prog_row_;
+      M_.progress->update(M_.progress);
+    }
     tasks_.clear();
     waves_.clear();
     units_.clear();
@@ -2489,6 +2501,10 @@ void Matrix::factorize(int algo, bool fp32) {
     throw std::runtime_error("factorize: LU needs a square matrix");
   device_init();
   const double t0 = now_s();
+  if (progress) {
+    progress->max = embedded ? rows->n / 2 : rows->n;
+    progress->current = 0;
+  }
   if (!fstate_) fstate_ = std::make_shared<FactorState>();
   fstate_->solve_plans.clear();
   fstate_->drop_graphs();

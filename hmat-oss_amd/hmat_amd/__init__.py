@@ -68,6 +68,15 @@ PREPARE_FUNC = C.CFUNCTYPE(None, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C
 COMPUTE_FUNC = C.CFUNCTYPE(None, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp)
 
 
+class hmat_progress_t(C.Structure):
+    pass
+
+
+PROGRESS_FUNC = C.CFUNCTYPE(None, C.POINTER(hmat_progress_t))
+hmat_progress_t._fields_ = [("max", C.c_int), ("current", C.c_int), ("update", PROGRESS_FUNC),
+                            ("user_data", _vp)]
+
+
 class hmat_factorization_context_t(C.Structure):
     _fields_ = [("factorization", C.c_int), ("progress", _vp)]
 
@@ -356,13 +365,22 @@ class HMatrix:
         lib.hmat_delete_compression(comp)
         _check(rc, "assemble_generic")
 
-    def factorize(self, algo=FACT_LLT, mixed=False):
+    def factorize(self, algo=FACT_LLT, mixed=False, progress=None):
+        """factorize_generic; progress(current, max) is called as the
+        factorisation advances (hmat_progress_t of the context)."""
         if mixed:
             _check(load().hmx_factorize_mixed(self.handle, algo), "hmx_factorize_mixed")
             return
         ctx = hmat_factorization_context_t()
         load().hmat_factorization_context_init(C.byref(ctx))
         ctx.factorization = algo
+        if progress is not None:
+            @PROGRESS_FUNC
+            def update(p):
+                progress(p[0].current, p[0].max)
+            prog = hmat_progress_t(0, 0, update, None)
+            self._progress = (prog, update)
+            ctx.progress = C.cast(C.pointer(prog), _vp)
         _check(self.iface.factorize_generic(self.handle, C.byref(ctx)), "factorize_generic")
 
     def info(self):

@@ -92,3 +92,21 @@ def test_hodlr_factor_gemv():
     say(f"(LW)(LW)^T x vs A x {rec:.2e}; L^-1 (L W) x vs x {inv:.2e}; vs oracle {worst:.2e}")
     assert rec <= 1e-12 and inv <= 1e-12 and worst <= 1e-11   # measured 1.6e-15, 1.2e-15, 6.2e-14
     h.close()
+
+
+@pytest.mark.gpu
+def test_factorisation_progress():
+    """hmat_progress_t of the factorisation context: max = N, current = the end
+    row of the last diagonal leaf factorised, non-decreasing, ending at N."""
+    n = 20000
+    say(f"factorisation progress n={n}")
+    pts = ho.cylinder_points(n, 1.0, 1.6, 2)
+    t = hmat_amd.ClusterTree(pts, 128)
+    h = hmat_amd.HMatrix(t)
+    h.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    seen = []
+    h.factorize(hmat_amd.FACT_LLT, progress=lambda cur, mx: seen.append((cur, mx)))
+    say(f"progress calls: {seen}")
+    assert seen and all(mx == n for _, mx in seen)
+    assert seen[-1][0] == n and all(a[0] <= b[0] for a, b in zip(seen, seen[1:]))
+    h.close()
