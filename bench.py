@@ -404,6 +404,7 @@ def main():
     t_asm = None
     x_ir = None
     h = None
+    solve_rows = None   # inverted diagonal block size picked by hmx_autotune_solve in the warm-up
     for s in range(args.warmup + args.steps):
         if h is not None:
             h.close()
@@ -414,10 +415,16 @@ def main():
         t = time.time()
         h.factorize(hmat_amd.FACT_LLT, mixed=True)
         tf = time.time() - t
+        if solve_rows is not None:
+            h.set_solve_block_rows(solve_rows)
         it, rel = h.solve_refined_device(x.data_ptr(), nrhs, tol=1e-10, max_iter=50)
         torch.cuda.synchronize()
         dt = time.time() - t
         st = h.stats()
+        if s == 0 and args.warmup > 0:
+            solve_rows, solve_ms = h.autotune_solve(nrhs)
+            log(f"solve autotune: inverted diagonal blocks of up to {solve_rows} rows, "
+                f"{solve_ms:.1f} ms per {nrhs}-RHS FP32 solve")
         log(f"step {s}: assembly {ta:.2f}s factor {tf:.2f}s factor+solve {dt:.2f}s "
             f"IR sweeps {it} resid {rel:.2e} tasks {st['tasks']:.0f} waves {st['waves']:.0f} "
             f"launches {st['launches']:.0f} segments {st['segments']:.0f}")
@@ -509,7 +516,8 @@ def main():
                    "resid_dense_4096_rows": resid_dense_ir,
                    "compressed_size": info["compressed_size"], "fp64": fp64,
                    "speedup_vs_fp64": (fp64["factor_solve_s"] / mean_s) if fp64 else None,
-                   "hgemv_ms": gemv_ms, "hgemv_GBps": gbs},
+                   "hgemv_ms": gemv_ms, "hgemv_GBps": gbs,
+                   "solve_block_rows": solve_rows or 2048},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs / HBM_PEAK_GBS, "kernel": "H-gemv (FP64)",
                      "traffic": HGEMV_TRAFFIC_BYTES_N200K

@@ -1005,6 +1005,22 @@ int hmx_set_stream(hmat_matrix_t* h, void* s) {
 
 void hmx_set_rank_capacity(int cap) { g_rank_cap = cap > 0 ? cap : 160; }
 
+int hmx_autotune_solve(hmat_matrix_t* h, int nrhs, int* best_rows, double* best_ms) {
+  return guard([&] {
+    const double ms = M(h)->autotune_solve(nrhs, best_rows);
+    if (best_ms) *best_ms = ms;
+  });
+}
+
+int hmx_set_solve_block_rows(hmat_matrix_t* h, int rows) {
+  return guard([&] {
+    if (rows < 1) throw std::runtime_error("hmx_set_solve_block_rows: rows must be positive");
+    Matrix* m = M(h);
+    m->inv_max_rows = rows;
+    m->drop_solve_plans();
+  });
+}
+
 int hmx_ftask_size(void) { return (int)sizeof(hmx::FTask); }
 
 int hmx_run_tasks(int precision, int type, const void* tasks, int count, int* flags_out) {

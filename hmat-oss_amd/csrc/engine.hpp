@@ -175,6 +175,11 @@ class Matrix {
     int64_t off;  // elements into inv_buf; m x m, ld m
   };
   std::vector<InvBlock> inv_blocks;
+  int inv_max_rows = 2048;   // the largest diagonal blocks applied as inverses
+  // times the solve (nrhs right-hand sides) with each candidate inv_max_rows
+  // and keeps the fastest; returns its milliseconds per solve
+  double autotune_solve(int nrhs, int* best_rows);
+  void drop_solve_plans();   // recorded solve plans, their graphs and the inverses
   std::map<const Block*, int> inv_index;
   DevBuf inv_buf;
   std::map<const Block*, int> hodlr_index;

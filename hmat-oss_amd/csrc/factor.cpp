@@ -53,7 +53,7 @@ constexpr int64_t kInvBase = kTempBase + (int64_t(1) << 32);
 // dense inverse (at N = 200k, leaf 128: blocks of 1,562 rows, 128 per sweep
 // instead of 2,048 leaf substitutions; measured against 1024: 0.1 s less
 // per FP32 + refinement step)
-constexpr int kInvMaxRows = 2048;
+constexpr int kInvMaxRows = 2048;  // default of Matrix::inv_max_rows
 
 // Dense view; n may be dynamic (rank slot dn).
 struct DV {
@@ -2593,9 +2593,9 @@ void Matrix::solve_device(T* b, int nrhs, int ldb, const Store<T>& st, int part)
   DevBuf& rhs = fs.solve_rhs[key];
   const bool inverses = factorization == hmat_factorization_llt || factorization == hmat_factorization_ldlt;
   if (it == fs.solve_plans.end() && inverses && inv_blocks.empty()) {
-    // the dense inverses of the diagonal blocks of at most kInvMaxRows rows
+    // the dense inverses of the diagonal blocks of at most inv_max_rows rows
     std::function<void(const Block*)> pick = [&](const Block* b) {
-      if (b->m() <= kInvMaxRows || b->is_leaf()) {
+      if (b->m() <= inv_max_rows || b->is_leaf()) {
         inv_index[b] = (int)inv_blocks.size();
         inv_blocks.push_back(InvBlock{b, 0});
         return;
@@ -2742,6 +2742,78 @@ void Matrix::factor_gemv_device(char trans, const T* x, T* y, int nrhs, const St
   HMX_CHECK(hipStreamSynchronize(stream));
 }
 template void Matrix::factor_gemv_device<double>(char, const double*, double*, int, const Store<double>&);
+
+void Matrix::drop_solve_plans() {
+  if (fstate_) {
+    HMX_CHECK(hipStreamSynchronize(stream));
+    fstate_->drop_graphs();
+    fstate_->solve_graphs.clear();
+    fstate_->solve_plans.clear();
+    fstate_->solve_rhs.clear();
+  }
+  inv_blocks.clear();
+  inv_index.clear();
+  inv_buf.release();
+}
+
+// The solve's schedule picked by measurement: each candidate size of the
+// diagonal blocks the solves apply as dense inverses is built, recorded and
+// replayed on a random right-hand side block; the fastest replay is kept.
+double Matrix::autotune_solve(int nrhs, int* best_rows) {
+  if (factorization != hmat_factorization_llt && factorization != hmat_factorization_ldlt)
+    throw std::runtime_error("autotune_solve: provided for LLt and LDLt factors");
+  if (nrhs < 1) throw std::runtime_error("autotune_solve: nrhs must be positive");
+  device_init();
+  const int n = rows->n;
+  const size_t esz = factor32 ? sizeof(float) : sizeof(double);
+  DevBuf b;
+  b.alloc(esz * (size_t)n * nrhs);
+  HMX_CHECK(hipMemsetAsync(b.p, 0, b.bytes, stream));
+  hipEvent_t e0, e1;
+  HMX_CHECK(hipEventCreate(&e0));
+  HMX_CHECK(hipEventCreate(&e1));
+  struct Events {
+    hipEvent_t a, b;
+    ~Events() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } ev{e0, e1};
+  struct KeepUserStream {
+    bool& flag;
+    bool saved;
+    ~KeepUserStream() { flag = saved; }
+  } keep{user_stream, user_stream};
+  user_stream = true;  // the solves below do not wait on their own
+  double best = -1.0;
+  int best_r = inv_max_rows;
+  for (int r : {512, 1024, 2048, 4096}) {
+    inv_max_rows = r;
+    drop_solve_plans();
+    auto solve = [&] {
+      if (factor32)
+        solve_device<float>(b.as<float>(), nrhs, n, s32);
+      else
+        solve_device<double>(b.as<double>(), nrhs, n, s64);
+    };
+    solve();  // inverses, recording
+    solve();  // graph capture
+    HMX_CHECK(hipEventRecord(e0, stream));
+    for (int q = 0; q < 3; ++q) solve();
+    HMX_CHECK(hipEventRecord(e1, stream));
+    HMX_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HMX_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (best < 0 || ms / 3 < best) {
+      best = ms / 3;
+      best_r = r;
+    }
+  }
+  inv_max_rows = best_r;
+  drop_solve_plans();
+  if (best_rows) *best_rows = best_r;
+  return best;
+}
 template void Matrix::factor_gemv_device<float>(char, const float*, float*, int, const Store<float>&);
 
 void Matrix::solve_host(void* b, int nrhs, bool permute, int part) {

@@ -187,6 +187,8 @@ _EXPORTS = {
     "hmx_leaf_table": (C.c_int, [_vp, C.c_int, _vp]),
     "hmx_stats": (C.c_int, [_vp, C.POINTER(C.c_double)]),
     "hmx_set_rank_capacity": (None, [C.c_int]),
+    "hmx_autotune_solve": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "hmx_set_solve_block_rows": (C.c_int, [_vp, C.c_int]),
     "hmx_set_stream": (C.c_int, [_vp, _vp]),
 }
 
@@ -627,6 +629,16 @@ class HMatrix:
         if it < 0:
             raise EngineError(f"refined solve failed: {load().hmx_last_error().decode()}")
         return it, r.value
+
+    def autotune_solve(self, nrhs=1):
+        """hmx_autotune_solve: (rows of the largest inverse blocks kept, ms per solve)."""
+        r, ms = C.c_int(0), C.c_double(0.0)
+        _check(load().hmx_autotune_solve(self.handle, nrhs, C.byref(r), C.byref(ms)),
+               "hmx_autotune_solve")
+        return r.value, ms.value
+
+    def set_solve_block_rows(self, rows):
+        _check(load().hmx_set_solve_block_rows(self.handle, rows), "hmx_set_solve_block_rows")
 
     def set_stream(self, stream_ptr):
         """Launch on a caller's HIP stream (int handle, e.g.

@@ -144,6 +144,16 @@ HMAT_API int hmx_set_stream(hmat_matrix_t *m, void *stream);
 /* Upper bound on Rk panel capacity used for new matrices (default 160). */
 HMAT_API void hmx_set_rank_capacity(int cap);
 
+/* No counterpart in hmat-oss (the engine's own tuning).  The solves of an
+ * LLt / LDLt factor apply the dense inverses of the diagonal blocks of at most
+ * R rows (default 2048).  hmx_autotune_solve measures the solve of nrhs
+ * right-hand sides with R = 512, 1024, 2048 and 4096 on the factorised matrix
+ * and keeps the fastest (best_rows, best_ms may be NULL);
+ * hmx_set_solve_block_rows sets R directly (for instance the tuned value on a
+ * new matrix of the same structure). */
+HMAT_API int hmx_autotune_solve(hmat_matrix_t *m, int nrhs, int *best_rows, double *best_ms);
+HMAT_API int hmx_set_solve_block_rows(hmat_matrix_t *m, int rows);
+
 /* Test hooks of the task kernels (no counterpart in hmat-oss): the size of
  * one task record (hmx::FTask of the engine's ftask.hpp, which the tests
  * mirror), and one batched launch of `count` records of one task type on

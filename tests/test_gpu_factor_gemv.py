@@ -110,3 +110,29 @@ def test_factorisation_progress():
     assert seen and all(mx == n for _, mx in seen)
     assert seen[-1][0] == n and all(a[0] <= b[0] for a, b in zip(seen, seen[1:]))
     h.close()
+
+
+@pytest.mark.gpu
+def test_solve_autotune():
+    """hmx_autotune_solve times the solve with each size of inverted diagonal
+    blocks and keeps the fastest; the solution does not depend on it beyond
+    rounding."""
+    n = 20000
+    say(f"solve autotune n={n}")
+    pts = ho.cylinder_points(n, 1.0, 1.6, 2)
+    t = hmat_amd.ClusterTree(pts, 128)
+    h = hmat_amd.HMatrix(t)
+    h.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    h.factorize(hmat_amd.FACT_LLT)
+    b = np.random.default_rng(3).standard_normal((n, 8))
+    x0 = h.solve(b)
+    rows, ms = h.autotune_solve(8)
+    x1 = h.solve(b)
+    h.set_solve_block_rows(512)
+    x2 = h.solve(b)
+    d1 = np.linalg.norm(x1 - x0) / np.linalg.norm(x0)
+    d2 = np.linalg.norm(x2 - x0) / np.linalg.norm(x0)
+    say(f"autotune picked {rows} rows ({ms:.2f} ms per 8-RHS solve); solution changes {d1:.2e}, {d2:.2e}")
+    assert rows in (512, 1024, 2048, 4096) and ms > 0
+    assert d1 <= 1e-11 and d2 <= 1e-11
+    h.close()
