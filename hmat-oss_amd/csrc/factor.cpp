@@ -21,6 +21,8 @@
 #include <stdexcept>
 #include <deque>
 #include <functional>
+#include <exception>
+#include <thread>
 #include <unordered_map>
 
 #include "engine.hpp"
@@ -184,8 +186,12 @@ void execute_plan(const SegmentPlan& P, FactorState& fs, hipStream_t s, bool cap
   const size_t ws_bytes = (size_t)std::max<int64_t>(P.ws, 1) * sizeof(double);
   // segments run asynchronously: a workspace still used by the previous one
   // is only replaced once the stream has drained
-  if (ws_bytes > fs.ws.bytes && fs.ws.p && !capturing) HMX_CHECK(hipStreamSynchronize(s));
-  fs.ws.ensure(ws_bytes);
+  if (ws_bytes > fs.ws.bytes) {
+    // grown with headroom (at least twice the old size), so that the stream
+    // drains for a reallocation a few times per factorisation, not per segment
+    if (fs.ws.p && !capturing) HMX_CHECK(hipStreamSynchronize(s));
+    fs.ws.ensure(capturing ? ws_bytes : std::max(ws_bytes, 2 * fs.ws.bytes));
+  }
   // a fill kernel rather than a memset node, so that the plan can be captured
   if (P.ranks_used) launch_fill_int(fs.ranks.as<int>(), (size_t)P.ranks_used, 0, s);
   const FTask* dt = P.tasks.as<FTask>();
@@ -285,7 +291,21 @@ class Exec {
   }
 
   ~Exec() {
+    if (submitter_.joinable()) submitter_.join();
     if (!inflight_.empty()) (void)hipStreamSynchronize(M_.stream);
+  }
+
+  // Waits until the launches of the segment handed to the submission thread
+  // are all queued (the queue's depth holds the host back until the device has
+  // run most of them); rethrows its error.
+  void join_submitter() {
+    if (!submitter_.joinable()) return;
+    submitter_.join();
+    if (submit_err_) {
+      std::exception_ptr e = submit_err_;
+      submit_err_ = nullptr;
+      std::rethrow_exception(e);
+    }
   }
 
   void reset_temps() {
@@ -2308,6 +2328,7 @@ class Exec {
     if (M_.progress && M_.progress->update) {
       // hmat_progress_t (h_matrix.cpp:2554-2557): the end row of the last
       // diagonal leaf factorised, once the segment holding it is done
+      join_submitter();
       HMX_CHECK(hipStreamSynchronize(M_.stream));
       M_.progress->current = M_.embedded ? prog_row_ / 2 : prog_row_;
       M_.progress->update(M_.progress);
@@ -2398,12 +2419,25 @@ class Exec {
     plan->ranks_used = fs_.rank_used;
     plan->ntasks = (long)n;
     plan->waves = W;
-    execute_plan<T>(*plan, fs_, s, false, M_.single_stream);
+    // the previous segment's launches are queued before this one's
+    join_submitter();
     if (record_) {
+      execute_plan<T>(*plan, fs_, s, false, M_.single_stream);
       HMX_CHECK(hipStreamSynchronize(s));
       record_->push_back(plan);
     } else {
       inflight_.push_back(plan);  // kept until check_flags() has waited for the stream
+      // a thread queues the launches, so that the host builds the next
+      // segment meanwhile: queuing ~10k launches waits on the device (a full
+      // queue), and building the next segment after it left the device idle
+      const bool one = M_.single_stream;
+      submitter_ = std::thread([this, plan, s, one] {
+        try {
+          execute_plan<T>(*plan, fs_, s, false, one);
+        } catch (...) {
+          submit_err_ = std::current_exception();
+        }
+      });
     }
     M_.stats.tasks += (long)n;
     M_.stats.waves += W;
@@ -2416,6 +2450,7 @@ class Exec {
   // capacity overflow seen by any of them raises here.  Launches after a
   // failing pivot return at once (the failure flag), as the reference stops.
   void check_flags() {
+    join_submitter();
     HMX_CHECK(hipStreamSynchronize(M_.stream));
     inflight_.clear();
     int f[2] = {0, 0};
@@ -2450,6 +2485,8 @@ class Exec {
   // the host builds the next segment while the device runs the current one
   static constexpr size_t kMaxTasks = 250000;
   std::vector<std::shared_ptr<SegmentPlan>> inflight_;
+  std::thread submitter_;          // queues the launches of the last segment
+  std::exception_ptr submit_err_;
   Matrix& M_;
   Store<T>& st_;
   FactorState& fs_;
