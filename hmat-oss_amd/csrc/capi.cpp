@@ -367,6 +367,60 @@ int u_set_fp2(hmat_matrix_t*, float, int, hmat_FPcompress_t, bool, bool) { retur
 int u_dump(hmat_matrix_t*, char*) { return unsupported("dump_info"); }
 int u_set_ct(hmat_matrix_t*, const hmat_cluster_tree_t*, const hmat_cluster_tree_t*) { return unsupported("set_cluster_trees"); }
 int u_values(struct hmat_get_values_context_t*) { return unsupported("get_block/get_values"); }
+
+// get_block c_wrapping.hpp:832-868 (UncompressedBlock): the rows [row_offset,
+// +row_size) x cols [col_offset, +col_size) block in hmat numbering,
+// column-major, ld row_size; renumber_rows puts the rows of a full-height
+// block in user order; row_indices / col_indices point at the trees'
+// numbering of the block
+template <typename T>
+int get_block(struct hmat_get_values_context_t* ctx) {
+  return guard([&] {
+    Matrix* m = M(ctx->matrix);
+    std::vector<int> rp(ctx->row_size), cp(ctx->col_size);
+    for (int i = 0; i < ctx->row_size; ++i) rp[i] = ctx->row_offset + i;
+    for (int j = 0; j < ctx->col_size; ++j) cp[j] = ctx->col_offset + j;
+    std::vector<double> v((size_t)ctx->row_size * ctx->col_size);
+    m->get_entries(rp, cp, v.data(), ctx->row_size);
+    T* out = static_cast<T*>(ctx->values);
+    if (ctx->renumber_rows) {
+      if (ctx->row_offset != 0 || ctx->row_size != m->n_rows())
+        throw std::runtime_error("Cannot renumber");
+      const int* idx = m->rows->indices.data();
+      for (int j = 0; j < ctx->col_size; ++j)
+        for (int i = 0; i < ctx->row_size; ++i)
+          out[idx[i] + (size_t)j * ctx->row_size] = (T)v[i + (size_t)j * ctx->row_size];
+    } else {
+      for (size_t e = 0; e < v.size(); ++e) out[e] = (T)v[e];
+    }
+    ctx->row_indices = const_cast<int*>(m->rows->indices.data()) + ctx->row_offset;
+    ctx->col_indices = const_cast<int*>(m->cols->indices.data()) + ctx->col_offset;
+  });
+}
+
+// get_values c_wrapping.hpp:870-890 (UncompressedValues): the entries of the
+// 1-based user indices row_indices x col_indices, column-major, ld row_size
+template <typename T>
+int get_values(struct hmat_get_values_context_t* ctx) {
+  return guard([&] {
+    Matrix* m = M(ctx->matrix);
+    std::vector<int> rp(ctx->row_size), cp(ctx->col_size);
+    for (int i = 0; i < ctx->row_size; ++i) {
+      const int u = ctx->row_indices[i] - 1;
+      if (u < 0 || u >= m->n_rows()) throw std::runtime_error("get_values: row index out of range");
+      rp[i] = m->rows->indices_rev[u];
+    }
+    for (int j = 0; j < ctx->col_size; ++j) {
+      const int u = ctx->col_indices[j] - 1;
+      if (u < 0 || u >= m->n_cols()) throw std::runtime_error("get_values: column index out of range");
+      cp[j] = m->cols->indices_rev[u];
+    }
+    std::vector<double> v((size_t)ctx->row_size * ctx->col_size);
+    m->get_entries(rp, cp, v.data(), ctx->row_size);
+    T* out = static_cast<T*>(ctx->values);
+    for (size_t e = 0; e < v.size(); ++e) out[e] = (T)v[e];
+  });
+}
 int u_walk(hmat_matrix_t*, hmat_procedure_t*) { return unsupported("walk"); }
 int u_leaf(hmat_matrix_t*, const hmat_leaf_procedure_t*) { return unsupported("apply_on_leaf"); }
 // read_struct / write_struct / read_data / write_data c_wrapping.hpp:921-946
@@ -438,8 +492,8 @@ void fill(hmat_interface_t* i) {
   i->extract_diagonal = extract_diagonal<T>;
   i->solve_lower_triangular = solve_lower_triangular;
   i->solve_lower_triangular_dense = solve_lower_triangular_dense;
-  i->get_block = u_values;
-  i->get_values = u_values;
+  i->get_block = get_block<T>;
+  i->get_values = get_values<T>;
   i->walk = u_walk;
   i->apply_on_leaf = u_leaf;
   i->value_type = (hmat_value_t)VT;
@@ -610,6 +664,8 @@ void hmat_init_mi355x_interface(hmat_interface_t* i, hmat_value_t type) {
       i->extract_diagonal = u_diag_z;
       i->solve_lower_triangular = u_tri_z;
       i->solve_lower_triangular_dense = u_tri_z;
+      i->get_block = u_values;
+      i->get_values = u_values;
       i->read_struct = u_read_z;
       i->write_struct = u_write_z;
       i->read_data = u_write_z;

@@ -234,6 +234,12 @@ class Matrix {
   void scale(double alpha);                 // A <- alpha A
   void add_identity(double alpha);          // A <- A + alpha I
   void extract_diagonal(double* diag);      // diagonal in hmat numbering (n_rows doubles)
+  // entries (rpos[i], cpos[j]) in hmat numbering, mirrored for lower-symmetric
+  // storage (a factor's stored blocks as they are), into out[i + j ld]
+  void get_entries(const std::vector<int>& rpos, const std::vector<int>& cpos, double* out, int ld);
+  template <typename T>
+  void get_entries_t(const Store<T>& st, const std::vector<int>& rpos, const std::vector<int>& cpos,
+                     double* out, int ld);
   void truncate_all();                      // recompress every Rk leaf at lowrank_eps
   void axpy_from(double alpha, Matrix& x);  // A <- A + alpha X, X of the same partition
   // this <- beta this + alpha op(a) op(b) (H-gemm), a and b over the same trees

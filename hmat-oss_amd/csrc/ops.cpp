@@ -2,10 +2,13 @@
 // and copy_struct (HMatInterface::copy, c_wrapping.hpp:98-108), norm
 // (HMatrix::normSqr h_matrix.cpp:624-646: off-diagonal blocks of symmetric
 // storage count twice), scale (c_wrapping.hpp:431), add_identity
-// (c_wrapping.hpp:393) and extract_diagonal (c_wrapping.hpp:760-773).
+// (c_wrapping.hpp:393), extract_diagonal (c_wrapping.hpp:760-773) and the
+// entries of get_block / get_values (c_wrapping.hpp:832-890).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <utility>
 
 #include "engine.hpp"
 
@@ -171,6 +174,91 @@ void Matrix::extract_diagonal(double* diag) {
   HMX_CHECK(hipMemcpyAsync(diag, dout.p, (size_t)n_rows() * sizeof(double),
                            hipMemcpyDeviceToHost, stream));
   HMX_CHECK(hipStreamSynchronize(stream));
+}
+
+namespace {
+// (position, query index) pairs sorted by position
+std::vector<std::pair<int, int>> sorted_query(const std::vector<int>& pos) {
+  std::vector<std::pair<int, int>> q(pos.size());
+  for (size_t i = 0; i < pos.size(); ++i) q[i] = {pos[i], (int)i};
+  std::sort(q.begin(), q.end());
+  return q;
+}
+// the query entries with positions in [lo, lo + len)
+std::pair<size_t, size_t> query_range(const std::vector<std::pair<int, int>>& q, int lo, int len) {
+  const auto b = std::lower_bound(q.begin(), q.end(), std::make_pair(lo, -1));
+  const auto e = std::lower_bound(q.begin(), q.end(), std::make_pair(lo + len, -1));
+  return {(size_t)(b - q.begin()), (size_t)(e - q.begin())};
+}
+}  // namespace
+
+template <typename T>
+void Matrix::get_entries_t(const Store<T>& st, const std::vector<int>& rpos,
+                           const std::vector<int>& cpos, double* out, int ld) {
+  const auto qr = sorted_query(rpos), qc = sorted_query(cpos);
+  std::vector<int> ranks(st.hdesc.size());
+  HMX_CHECK(hipMemcpy(ranks.data(), st.ranks.p, ranks.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < rpos.size(); ++i)
+    for (size_t j = 0; j < cpos.size(); ++j) out[i + (size_t)j * ld] = 0.0;
+  std::vector<T> a, b;
+  for (const Block* blk : bt->leaves) {
+    const LeafDesc& L = st.hdesc[blk->leaf];
+    // a factor is lower triangular (isLower off after the factorisation,
+    // h_matrix.cpp:2562-2563): its blocks are not mirrored
+    const bool mir = lower_sym && factorization == -1 && blk->mirrored();
+    const auto rr = query_range(qr, L.r0, L.m), cc = query_range(qc, L.c0, L.n);
+    const auto rt = mir ? query_range(qr, L.c0, L.n) : std::make_pair<size_t, size_t>(0, 0);
+    const auto ct = mir ? query_range(qc, L.r0, L.m) : std::make_pair<size_t, size_t>(0, 0);
+    const bool direct = rr.first < rr.second && cc.first < cc.second;
+    const bool mirror = rt.first < rt.second && ct.first < ct.second;
+    if (!direct && !mirror) continue;
+    // the leaf's data on the host (full: m x n; Rk: A m x k, B n x k)
+    const int k = L.kind == 0 ? 0 : std::max(ranks[blk->leaf], 0);
+    if (L.kind == 0) {
+      a.resize((size_t)L.m * L.n);
+      HMX_CHECK(hipMemcpy(a.data(), st.data() + L.a, a.size() * sizeof(T), hipMemcpyDeviceToHost));
+    } else {
+      a.resize((size_t)L.m * k);
+      b.resize((size_t)L.n * k);
+      if (k) {
+        HMX_CHECK(hipMemcpy(a.data(), st.data() + L.a, a.size() * sizeof(T), hipMemcpyDeviceToHost));
+        HMX_CHECK(hipMemcpy(b.data(), st.data() + L.b, b.size() * sizeof(T), hipMemcpyDeviceToHost));
+      }
+    }
+    auto value = [&](int i, int j) {  // leaf-local row i, column j
+      if (L.kind == 0) return (double)a[i + (size_t)j * L.m];
+      double v = 0.0;
+      for (int l = 0; l < k; ++l) v += (double)a[i + (size_t)l * L.m] * (double)b[j + (size_t)l * L.n];
+      return v;
+    };
+    if (direct)
+      for (size_t x = rr.first; x < rr.second; ++x)
+        for (size_t y = cc.first; y < cc.second; ++y)
+          out[qr[x].second + (size_t)qc[y].second * ld] = value(qr[x].first - L.r0, qc[y].first - L.c0);
+    if (mirror)
+      for (size_t x = rt.first; x < rt.second; ++x)
+        for (size_t y = ct.first; y < ct.second; ++y)
+          out[qr[x].second + (size_t)qc[y].second * ld] = value(qc[y].first - L.r0, qr[x].first - L.c0);
+  }
+}
+
+void Matrix::get_entries(const std::vector<int>& rpos, const std::vector<int>& cpos, double* out,
+                         int ld) {
+  if (!assembled) throw std::runtime_error("get_values: matrix not assembled");
+  if (factorization == hmat_factorization_hodlrsym || factorization == hmat_factorization_hodlr)
+    throw std::runtime_error("get_values: Unsupported operation (HODLR factor)");
+  for (int p : rpos)
+    if (p < 0 || p >= n_rows()) throw std::runtime_error("get_values: row index out of range");
+  for (int p : cpos)
+    if (p < 0 || p >= n_cols()) throw std::runtime_error("get_values: column index out of range");
+  device_init();
+  HMX_CHECK(hipStreamSynchronize(stream));
+  // the factor's precision when factorised, else the matrix's
+  const bool use32 = factorization != -1 ? factor32 : !s64.valid;
+  if (use32)
+    get_entries_t<float>(s32, rpos, cpos, out, ld);
+  else
+    get_entries_t<double>(s64, rpos, cpos, out, ld);
 }
 
 }  // namespace hmx

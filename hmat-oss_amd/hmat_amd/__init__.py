@@ -68,6 +68,13 @@ PREPARE_FUNC = C.CFUNCTYPE(None, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C
 COMPUTE_FUNC = C.CFUNCTYPE(None, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp)
 
 
+class hmat_get_values_context_t(C.Structure):
+    _fields_ = [("matrix", _vp), ("values", _vp), ("row_offset", C.c_int), ("col_offset", C.c_int),
+                ("row_size", C.c_int), ("col_size", C.c_int),
+                ("row_indices", C.POINTER(C.c_int)), ("col_indices", C.POINTER(C.c_int)),
+                ("renumber_rows", C.c_int, 1)]
+
+
 class hmat_progress_t(C.Structure):
     pass
 
@@ -119,6 +126,7 @@ IOSTREAM = C.CFUNCTYPE(None, _vp, C.c_size_t, _vp)   # hmat_iostream
 _READ_STRUCT = C.CFUNCTYPE(_vp, IOSTREAM, _vp)
 _WRITE_IO = C.CFUNCTYPE(None, _vp, IOSTREAM, _vp)
 _GET_TREES = C.CFUNCTYPE(C.c_int, _vp, C.POINTER(_vp), C.POINTER(_vp))
+_GET_VALUES = C.CFUNCTYPE(C.c_int, C.POINTER(hmat_get_values_context_t))
 
 # slot order of hmat_interface_t (hmat.h:717-1103); unused slots are plain pointers
 _SLOTS = [
@@ -136,7 +144,8 @@ _SLOTS = [
     ("SetFPCompressionSettings", _vp), ("SetFPCompressionSettingsParams", _vp),
     ("dump_info", _vp), ("get_cluster_trees", _GET_TREES), ("set_cluster_trees", _vp),
     ("extract_diagonal", _DIAG), ("solve_lower_triangular", _SOLVE_TRI),
-    ("solve_lower_triangular_dense", _SOLVE_TRI), ("get_block", _vp), ("get_values", _vp),
+    ("solve_lower_triangular_dense", _SOLVE_TRI), ("get_block", _GET_VALUES),
+    ("get_values", _GET_VALUES),
     ("walk", _vp), ("apply_on_leaf", _vp), ("value_type", C.c_int), ("internal", _vp),
     ("read_struct", _READ_STRUCT), ("write_struct", _WRITE_IO), ("read_data", _WRITE_IO), ("write_data", _WRITE_IO),
     ("set_progressbar", _vp), ("extract_diagonal_block", _vp),
@@ -639,6 +648,32 @@ class HMatrix:
 
     def set_solve_block_rows(self, rows):
         _check(load().hmx_set_solve_block_rows(self.handle, rows), "hmx_set_solve_block_rows")
+
+    def get_block(self, row_offset, row_size, col_offset, col_size, renumber_rows=False):
+        """get_block: the dense block of rows [row_offset, +row_size) x cols
+        [col_offset, +col_size) in hmat numbering (rows in user order with
+        renumber_rows, for a full-height block)."""
+        out = np.zeros((col_size, row_size), dtype=self.dtype)   # column-major storage
+        ctx = hmat_get_values_context_t()
+        ctx.matrix, ctx.values = self.handle, out.ctypes.data
+        ctx.row_offset, ctx.row_size, ctx.col_offset, ctx.col_size = row_offset, row_size, col_offset, col_size
+        ctx.renumber_rows = 1 if renumber_rows else 0
+        _check(self.iface.get_block(C.byref(ctx)), "get_block")
+        return out.T
+
+    def get_values(self, rows, cols):
+        """get_values: the entries at user indices rows x cols (0-based here;
+        the C slot takes them 1-based)."""
+        r = np.ascontiguousarray(np.asarray(rows, dtype=np.int32) + 1)
+        c = np.ascontiguousarray(np.asarray(cols, dtype=np.int32) + 1)
+        out = np.zeros((len(c), len(r)), dtype=self.dtype)
+        ctx = hmat_get_values_context_t()
+        ctx.matrix, ctx.values = self.handle, out.ctypes.data
+        ctx.row_size, ctx.col_size = len(r), len(c)
+        ctx.row_indices = r.ctypes.data_as(C.POINTER(C.c_int))
+        ctx.col_indices = c.ctypes.data_as(C.POINTER(C.c_int))
+        _check(self.iface.get_values(C.byref(ctx)), "get_values")
+        return out.T
 
     def set_stream(self, stream_ptr):
         """Launch on a caller's HIP stream (int handle, e.g.

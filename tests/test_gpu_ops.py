@@ -143,3 +143,37 @@ def test_solve_lower_triangular():
     yh[:] = yu[idx]
     assert np.linalg.norm(yh - y) <= 1e-13 * np.linalg.norm(y)
     h.close()
+
+
+@pytest.mark.gpu
+def test_get_block_and_get_values():
+    """get_block / get_values (c_wrapping.hpp:832-890): blocks in hmat
+    numbering, entries at 1-based user indices, symmetric storage mirrored,
+    a factor's stored blocks as they are."""
+    n = 3000
+    pts = ho.cylinder_points(n, 1.0, 1.2, 31)
+    t = hmat_amd.ClusterTree(pts, 128)
+    h = hmat_amd.HMatrix(t)
+    h.assemble(length=0.5, nugget=1e-2, eps=1e-4)
+    D = h.to_dense()                               # hmat numbering
+    idx = t.indices
+    B = h.get_block(0, n, 0, n)
+    e1 = np.abs(B - D).max()
+    S = h.get_block(700, 900, 1500, 1100)
+    e2 = np.abs(S - D[700:1600, 1500:2600]).max()
+    R = h.get_block(0, n, 100, 50, renumber_rows=True)
+    U = np.empty_like(D)
+    U[idx] = D
+    e3 = np.abs(R - U[:, 100:150]).max()
+    rng = np.random.default_rng(5)
+    rows, cols = rng.choice(n, 37, replace=False), rng.choice(n, 23, replace=False)
+    V = h.get_values(rows, cols)
+    Du = np.empty_like(D)
+    Du[np.ix_(idx, idx)] = D                       # user numbering
+    e4 = np.abs(V - Du[np.ix_(rows, cols)]).max()
+    h.factorize(hmat_amd.FACT_LLT)
+    F = h.get_block(0, n, 0, n)
+    e5 = np.abs(F - h.to_dense(mirror=False)).max()
+    print(f"get_block full {e1:.1e}, sub {e2:.1e}, renumbered {e3:.1e}; get_values {e4:.1e}; factor {e5:.1e}")
+    assert max(e1, e2, e3, e4) <= 1e-14 * np.abs(D).max() and e5 <= 1e-14 * np.abs(F).max()
+    h.close()
