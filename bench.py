@@ -214,7 +214,8 @@ def helmholtz_bench(n, height, nrhs, gemv_reps, sample=4096, factor=True):
     torch.cuda.synchronize()
     t = time.time()
     H.assemble(kind=hmat_amd.KERNEL_HELMHOLTZ, wavenumber=wk, reg=reg, nugget=1.0, eps=1e-4,
-               method="aca_partial")
+               method="aca_partial",
+               progress=lambda c, m: log(f"helmholtz: assembly {c}/{m} leaves at {time.time() - t:.1f}s"))
     t_asm = time.time() - t
     info = H.info()
     tab = H.leaf_table()
@@ -272,7 +273,15 @@ def helmholtz_bench(n, height, nrhs, gemv_reps, sample=4096, factor=True):
     bd = torch.tensor(b.T.copy(), dtype=torch.complex128, device="cuda")
     torch.cuda.synchronize()
     t = time.time()
-    H.factorize(hmat_amd.FACT_LU)
+    last = [t]
+
+    def fac_progress(c, m):
+        # the factorisation's own progress (end row of the last diagonal
+        # leaf), printed at most every 20 s
+        if time.time() - last[0] > 20:
+            last[0] = time.time()
+            log(f"helmholtz: H-LU row {c}/{m} at {time.time() - t:.1f}s")
+    H.factorize(hmat_amd.FACT_LU, progress=fac_progress)
     torch.cuda.synchronize()
     t_fac = time.time() - t
     st = H.stats()

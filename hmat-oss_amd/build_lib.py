@@ -17,7 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HOST_ONLY = ["tree.cpp"]
-HIP_SRC = ["engine.cpp", "factor.cpp", "sweep.cpp", "ops.cpp", "serialize.cpp", "capi.cpp", "k_assembly.hip",
+HIP_SRC = ["engine.cpp", "factor.cpp", "sweep.cpp", "ops.cpp", "serialize.cpp", "capi.cpp", "giant.cpp",
+           "k_assembly.hip",
            "k_gemv.hip", "k_factor.hip", "k_sweep.hip", "k_ops.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
@@ -53,7 +54,10 @@ def build(verbose=False):
                 print(o)
     objs = [j[-1] for j in jobs]
     tmp = OUT + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    # rocBLAS / rocSOLVER: GEMM, QR and SVD of the whole-device compression of
+    # very large blocks
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
+          "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"])
     os.replace(tmp, OUT)
     return OUT
 

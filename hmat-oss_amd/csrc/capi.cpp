@@ -95,6 +95,15 @@ int assemble_generic(hmat_matrix_t* h, hmat_assemble_context_t* ctx) {
       throw std::runtime_error("assembly: C++ assembly objects are not supported by this engine");
     const auto* c = reinterpret_cast<const Compression*>(ctx->compression);
     hmx::CompressionSpec cs{c->method, c->eps};
+    // the context's progress reporter for this assembly (and the
+    // factorisation it may ask for); the matrix's own one afterwards
+    hmat_progress_t* saved = m->progress;
+    if (ctx->progress) m->progress = ctx->progress;
+    struct Restore {
+      Matrix* m;
+      hmat_progress_t* p;
+      ~Restore() { m->progress = p; }
+    } restore{m, saved};
     if (ctx->simple_compute != hmat_mi355x_kernel_compute) {
       // the caller's host functions: blocks evaluated on the host, compressed
       // on the device (only one of advanced / block / simple is used, advanced
@@ -1065,6 +1074,15 @@ int hmx_autotune_solve(hmat_matrix_t* h, int nrhs, int* best_rows, double* best_
   return guard([&] {
     const double ms = M(h)->autotune_solve(nrhs, best_rows);
     if (best_ms) *best_ms = ms;
+  });
+}
+
+int hmx_set_large_block_rows(hmat_matrix_t* h, int rows) {
+  return guard([&] {
+    if (rows < 1) throw std::runtime_error("hmx_set_large_block_rows: rows must be positive");
+    Matrix* m = M(h);
+    // complex rows; the real representation has twice as many
+    m->giant_rows = m->embedded ? 2 * rows : rows;
   });
 }
 

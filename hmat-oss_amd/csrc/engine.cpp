@@ -236,16 +236,18 @@ void Matrix::grow_rank_capacity() {
   s64.ranks.release();
   s64.desc.release();
   s64.valid = false;
-  build_storage();
+  build_storage(cap_hint_.empty() ? nullptr : &cap_hint_);
 }
 
 void Matrix::assemble(const KernelSpec& k, const CompressionSpec& comp) {
   const double t0 = now();
+  giant_.clear();
   for (;;) {
     assemble_pass(k, comp);
     if (!assembly_overflow()) break;
     grow_rank_capacity();
   }
+  giant_.clear();
   finish_assembly(t0);
 }
 
@@ -259,6 +261,34 @@ void Matrix::assemble_host(const HostFunctions& f, const CompressionSpec& comp) 
   finish_assembly(t0);
 }
 
+namespace {
+// ACA panels of one block: room for twice the leaf's Rk capacity (at most
+// 1024 columns).  A block whose ACA reaches that many iterations below its
+// own limit sets the overflow flag, so the assembly runs again with doubled
+// capacities instead of keeping an approximation stopped early.
+int aca_kmax(const LeafDesc& L) {
+  return std::min(std::min(L.m, L.n), std::min(1024, std::max(2 * L.cap, 64)));
+}
+
+// ACA workspace of one launch, in doubles: a quarter of the free device
+// memory, between 4 and 64 GiB (a larger launch runs more blocks side by side;
+// each launch lasts as long as its slowest block)
+int64_t aca_budget_doubles() {
+  size_t freeb = 0, total = 0;
+  HMX_CHECK(hipMemGetInfo(&freeb, &total));
+  const int64_t lo = (int64_t)4 << 30, hi = (int64_t)64 << 30;
+  return std::min(hi, std::max(lo, (int64_t)(freeb / 4))) / (int64_t)sizeof(double);
+}
+
+// the largest blocks first: blocks of similar cost share a launch
+void sort_aca_by_size(std::vector<AcaTask>& tasks, const std::vector<LeafDesc>& desc) {
+  std::stable_sort(tasks.begin(), tasks.end(), [&](const AcaTask& a, const AcaTask& b) {
+    const LeafDesc &x = desc[a.leaf], &y = desc[b.leaf];
+    return (int64_t)(x.m + x.n) * aca_kmax(x) > (int64_t)(y.m + y.n) * aca_kmax(y);
+  });
+}
+}  // namespace
+
 void Matrix::assemble_pass(const KernelSpec& k, const CompressionSpec& comp) {
   device_init();
   upload_points(k, *rows, pts_rows, user_rows, stream);
@@ -269,8 +299,53 @@ void Matrix::assemble_pass(const KernelSpec& k, const CompressionSpec& comp) {
   if (embedded != (k.kind == kHelmholtz))
     throw std::runtime_error(embedded ? "assembly: a complex matrix needs the complex (Helmholtz) kernel"
                                       : "assembly: the Helmholtz kernel needs a complex matrix");
+  // very large complex blocks first: their ranks may raise their leaves'
+  // capacities, which changes the storage layout
+  std::vector<int> giant;
+  if (embedded && comp.method != 0)
+    for (size_t i = 0; i < bt->leaves.size(); ++i) {
+      const LeafDesc& L = s64.hdesc[i];
+      if (L.kind == 1 && std::min(L.m, L.n) >= giant_rows) giant.push_back((int)i);
+    }
+  if (!giant.empty()) {
+    compress_giants(giant, rp, cp, kp, std::max(comp.eps, lowrank_eps));
+    bool grow = false;
+    for (int l : giant) {
+      const int r = giant_.at(l)->rank;
+      if (r > s64.hdesc[l].cap) {
+        if (cap_hint_.size() < bt->leaves.size()) cap_hint_.resize(bt->leaves.size(), 0);
+        cap_hint_[l] = std::max(cap_hint_[l], r);
+        grow = true;
+      }
+    }
+    if (grow) {
+      s64.arena.release();
+      s64.ranks.release();
+      s64.desc.release();
+      s64.valid = false;
+      build_storage(&cap_hint_);
+    }
+  }
   init_store(s64);
   HMX_CHECK(hipMemsetAsync(overflow_.p, 0, sizeof(int), stream));
+  if (!giant.empty()) {
+    std::vector<int> gr;
+    for (int l : giant) {
+      const LeafDesc& L = s64.hdesc[l];
+      const GiantPanels& g = *giant_.at(l);
+      gr.push_back(g.rank);
+      if (g.rank > 0) {
+        HMX_CHECK(hipMemcpyAsync(s64.data() + L.a, g.a.p, sizeof(double) * (size_t)L.m * g.rank,
+                                 hipMemcpyDeviceToDevice, stream));
+        HMX_CHECK(hipMemcpyAsync(s64.data() + L.b, g.b.p, sizeof(double) * (size_t)L.n * g.rank,
+                                 hipMemcpyDeviceToDevice, stream));
+      }
+    }
+    for (size_t q = 0; q < giant.size(); ++q)
+      HMX_CHECK(hipMemcpyAsync(s64.ranks.as<int>() + giant[q], &gr[q], sizeof(int),
+                               hipMemcpyHostToDevice, stream));
+    HMX_CHECK(hipStreamSynchronize(stream));
+  }
 
   std::vector<int> full, svd, big_svd;
   std::vector<int> aca;
@@ -278,6 +353,8 @@ void Matrix::assemble_pass(const KernelSpec& k, const CompressionSpec& comp) {
     const LeafDesc& L = s64.hdesc[i];
     if (L.kind == 0) {
       full.push_back((int)i);
+    } else if (std::binary_search(giant.begin(), giant.end(), (int)i)) {
+      // compressed above
     } else if (embedded) {
       // complex blocks (twice the rows and columns in the embedding): ACA and
       // the truncation at any size, or the dense SVD when asked for
@@ -335,20 +412,25 @@ void Matrix::assemble_pass(const KernelSpec& k, const CompressionSpec& comp) {
       svd_compress_dense(blocks, stage.as<double>(), std::max(comp.eps, lowrank_eps));
     }
   }
-  // ACA blocks in batches bounded by a workspace budget.
-  const int64_t budget = (int64_t)4 << 27;  // doubles (4 GiB)
+  // ACA blocks, the largest first, in launches bounded by a workspace budget.
+  std::vector<AcaTask> acat;
+  for (int l : aca) acat.push_back(AcaTask{l, aca_kmax(s64.hdesc[l]), 0});
+  sort_aca_by_size(acat, s64.hdesc);
+  const int64_t budget = acat.empty() ? 0 : aca_budget_doubles();
   const int partial = comp.method == 2;
   size_t i = 0;
   DevBuf ws, dt;
-  while (i < aca.size()) {
+  const int nleaf = (int)bt->leaves.size();
+  while (i < acat.size()) {
+    HMX_CHECK(hipStreamSynchronize(stream));
+    report_assembly(nleaf - (int)(acat.size() - i), nleaf);
     std::vector<AcaTask> batch;
     int64_t used = 0;
-    while (i < aca.size()) {
-      const LeafDesc& L = s64.hdesc[aca[i]];
-      const int kmax = std::min(std::min(L.m, L.n), 1024);
-      const int64_t need = aca_workspace(L.m, L.n, kmax);
+    while (i < acat.size()) {
+      const LeafDesc& L = s64.hdesc[acat[i].leaf];
+      const int64_t need = aca_workspace(L.m, L.n, acat[i].kmax);
       if (!batch.empty() && used + need > budget) break;
-      batch.push_back(AcaTask{aca[i], kmax, used});
+      batch.push_back(AcaTask{acat[i].leaf, acat[i].kmax, used});
       used += need;
       ++i;
     }
@@ -361,8 +443,18 @@ void Matrix::assemble_pass(const KernelSpec& k, const CompressionSpec& comp) {
   }
 }
 
+// hmat_progress_t of the assembly context: max = the number of leaves,
+// current = the leaves whose blocks are assembled (reported between launches)
+void Matrix::report_assembly(int done, int total) {
+  if (!progress || !progress->update) return;
+  progress->max = total;
+  progress->current = done;
+  progress->update(progress);
+}
+
 void Matrix::finish_assembly(double t0) {
   HMX_CHECK(hipStreamSynchronize(stream));
+  report_assembly((int)bt->leaves.size(), (int)bt->leaves.size());
   s64.valid = true;
   if (value_type == HMAT_SIMPLE_PRECISION) {
     init_store(s32);
@@ -462,7 +554,7 @@ void Matrix::assemble_host_pass(const HostFunctions& f, const CompressionSpec& c
   // leaves in staging batches of at most kStage doubles (a leaf larger than
   // that is a batch of its own)
   const int64_t kStageElems = (int64_t)1 << 26;  // 512 MB of doubles
-  const int64_t aca_budget = (int64_t)4 << 27;   // doubles of ACA workspace per launch
+  const int64_t aca_budget = aca_budget_doubles();  // doubles of ACA workspace per launch
   const int nleaf = (int)bt->leaves.size();
   std::vector<double> host, zbuf;
   DevBuf table, ws, dl, doff, dt;
@@ -495,7 +587,7 @@ void Matrix::assemble_host_pass(const HostFunctions& f, const CompressionSpec& c
       } else if (embedded && comp.method == 0) {
         big_svd.push_back({l, off});
       } else if (embedded) {
-        AcaTask t{l, std::min(std::min(L.m, L.n), 1024), 0};
+        AcaTask t{l, aca_kmax(L), 0};
         t.tab = off;
         aca.push_back(t);
       } else if (std::max(L.m, L.n) < 100) {
@@ -504,7 +596,7 @@ void Matrix::assemble_host_pass(const HostFunctions& f, const CompressionSpec& c
       } else if (comp.method == 0 /*svd*/) {
         big_svd.push_back({l, off});
       } else {
-        AcaTask t{l, std::min(std::min(L.m, L.n), 1024), 0};
+        AcaTask t{l, aca_kmax(L), 0};
         t.tab = off;
         aca.push_back(t);
       }
@@ -525,6 +617,7 @@ void Matrix::assemble_host_pass(const HostFunctions& f, const CompressionSpec& c
                                 table.as<double>());
     if (!big_svd.empty())
       svd_compress_dense(big_svd, table.as<double>(), std::max(comp.eps, lowrank_eps));
+    sort_aca_by_size(aca, s64.hdesc);
     size_t i = 0;
     while (i < aca.size()) {
       std::vector<AcaTask> batch;
@@ -545,6 +638,10 @@ void Matrix::assemble_host_pass(const HostFunctions& f, const CompressionSpec& c
       launch_aca<double>(s64.desc.as<LeafDesc>(), dt.as<AcaTask>(), (int)batch.size(), s64.data(),
                          s64.ranks.as<int>(), overflow_.as<int>(), ws.as<double>(), none, none,
                          kp, comp.eps, lowrank_eps, comp.method == 2, stream, table.as<double>());
+    }
+    if (progress) {
+      HMX_CHECK(hipStreamSynchronize(stream));
+      report_assembly(l1, nleaf);
     }
     l0 = l1;
   }

@@ -145,8 +145,11 @@ class Matrix {
   int factorization = -1;    // hmat_factorization_t
   // progress of a factorisation (hmat_factorization_context_t.progress,
   // hmat_interface_t.set_progressbar): max = N, current = the end row of the
-  // last diagonal leaf factorised, updated as task segments complete
+  // last diagonal leaf factorised, updated as task segments complete; of an
+  // assembly (hmat_assemble_context_t.progress): max = the number of leaves,
+  // current = the leaves assembled, updated between launches
   hmat_progress_t* progress = nullptr;
+  void report_assembly(int done, int total);
   bool factor32 = false;      // factor stored in s32 (mixed precision)
   DevBuf pivots;              // LU: getrf interchanges of each diagonal leaf, at its rows
   DevBuf ldlt_diag;           // LDLt: D at the matrix's rows, in the factor's precision
@@ -175,6 +178,7 @@ class Matrix {
     int64_t off;  // elements into inv_buf; m x m, ld m
   };
   std::vector<InvBlock> inv_blocks;
+  int giant_rows = 8192;      // embedded rows and columns of a block compressed on the whole device
   int inv_max_rows = 2048;   // the largest diagonal blocks applied as inverses
   // times the solve (nrhs right-hand sides) with each candidate inv_max_rows
   // and keeps the fastest; returns its milliseconds per solve
@@ -264,6 +268,18 @@ class Matrix {
   void grow_rank_capacity();
   void svd_compress_dense(const std::vector<std::pair<int, int64_t>>& blocks, const double* dense,
                           double eps);
+  // very large blocks of a complex matrix (at least giant_rows embedded rows
+  // and columns): dense evaluation and a randomized SVD on the whole device
+  // (giant.cpp); the panels are kept for the passes of one assembly, and a
+  // rank above the leaf's capacity raises that leaf's capacity
+  struct GiantPanels {
+    DevBuf a, b;
+    int rank = 0;
+  };
+  std::map<int, std::unique_ptr<GiantPanels>> giant_;
+  std::vector<int> cap_hint_;   // per-leaf capacity floors (empty: none)
+  void compress_giants(const std::vector<int>& list, PointSet rp, PointSet cp, KernelParams kp,
+                       double eps);
   void finish_assembly(double t0);
   template <typename T>
   void init_store(Store<T>& st);

@@ -472,6 +472,10 @@ __global__ void __launch_bounds__(kBlock) k_aca(const LeafDesc* desc, const AcaT
     k = aca_partial(c, w, t.kmax, aca_eps, red, rv, ri, tmp);
   else
     k = aca_plus(c, w, t.kmax, aca_eps, red, rv, ri, tmp);
+  // stopped by the panel room rather than by the criterion: the host grows
+  // the capacities and assembles again
+  const int room = d.m < d.n ? d.m : d.n;
+  if (k >= t.kmax && t.kmax < (room < 1024 ? room : 1024) && threadIdx.x == 0) atomicOr(overflow, 1);
   int nk = 0;
   if (k > 0)
     nk = truncate_panels<T>(w.acols, d.m, w.bcols, d.n, k, w, trunc_eps, arena + d.a,

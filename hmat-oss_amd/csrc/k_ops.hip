@@ -2,7 +2,9 @@
 // squared Frobenius norms of leaves (HMatrix::normSqr h_matrix.cpp:624-646,
 // RkMatrix::normSqr, FullMatrix::normSqr), scaling (HMatrix::scale), adding a
 // multiple of the identity (HMatrix::addIdentity) and reading the diagonal
-// (HMatrix::extractDiagonal).  One workgroup per leaf of a list.
+// (HMatrix::extractDiagonal).  One workgroup per leaf of a list.  Also the
+// element-wise helpers of the whole-device compression of very large blocks
+// (seeded normal numbers, square-root scalings, triangle copies).
 #include "kernels.hpp"
 
 namespace hmx {
@@ -105,6 +107,76 @@ void launch_leaf_get_diag(const LeafDesc* desc, const int* list, int count, cons
                           double* out, hipStream_t s) {
   if (count == 0) return;
   hipLaunchKernelGGL(k_leaf_get_diag<T>, dim3(count), dim3(256), 0, s, desc, list, arena, out);
+  HMX_CHECK(hipGetLastError());
+}
+
+namespace {
+// splitmix64 finaliser (Steele, Lea, Flood: "Fast splittable pseudorandom
+// number generators", OOPSLA 2014)
+__device__ inline uint64_t mix64(uint64_t x) {
+  x ^= x >> 30;
+  x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 27;
+  x *= 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return x;
+}
+
+// standard normal numbers, entry i a function of (seed, i) only: two
+// uniforms from the counter by the finaliser, then Box-Muller
+__global__ void k_gauss(double* out, size_t n, uint64_t seed) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint64_t z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    const uint64_t a = mix64(z), b = mix64(z ^ 0xD1B54A32D192ED03ull);
+    const double u1 = ((double)(a >> 11) + 0.5) * 0x1.0p-53;
+    const double u2 = (double)(b >> 11) * 0x1.0p-53;
+    out[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  }
+}
+
+// v (ld rows) scaled by sqrt(s): rows j < k (by_rows) or columns j < k
+__global__ void k_scale_sqrt(double* v, int ld, int rows, int cols, const double* s, int k,
+                             int by_rows) {
+  const size_t total = (size_t)rows * cols;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % rows), j = (int)(e / rows);
+    const int q = by_rows ? i : j;
+    if (q < k) v[i + (size_t)j * ld] *= sqrt(s[q]);
+  }
+}
+
+// dst (p x p) = the upper triangle of src (ld lds), zeros below
+__global__ void k_upper(const double* src, int lds, double* dst, int p) {
+  const size_t total = (size_t)p * p;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % p), j = (int)(e / p);
+    dst[e] = i <= j ? src[i + (size_t)j * lds] : 0.0;
+  }
+}
+
+unsigned grid_for(size_t n) {
+  const size_t b = (n + 255) / 256;
+  return (unsigned)(b < 65536 ? (b ? b : 1) : 65536);
+}
+}  // namespace
+
+void launch_gauss(double* out, size_t n, uint64_t seed, hipStream_t s) {
+  hipLaunchKernelGGL(k_gauss, dim3(grid_for(n)), dim3(256), 0, s, out, n, seed);
+  HMX_CHECK(hipGetLastError());
+}
+
+void launch_scale_sqrt(double* v, int ld, int rows, int cols, const double* sv, int k, bool by_rows,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_sqrt, dim3(grid_for((size_t)rows * cols)), dim3(256), 0, s, v, ld,
+                     rows, cols, sv, k, by_rows ? 1 : 0);
+  HMX_CHECK(hipGetLastError());
+}
+
+void launch_upper(const double* src, int lds, double* dst, int p, hipStream_t s) {
+  hipLaunchKernelGGL(k_upper, dim3(grid_for((size_t)p * p)), dim3(256), 0, s, src, lds, dst, p);
   HMX_CHECK(hipGetLastError());
 }
 

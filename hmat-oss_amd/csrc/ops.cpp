@@ -40,6 +40,12 @@ Matrix* Matrix::clone(bool structure_only) {
                      : new Matrix(rows, cols, lower_sym, adm, value_type, rk_cap_max);
   try {
     c->lowrank_eps = lowrank_eps;
+    // the same leaf layout (capacities raised by an assembly included)
+    c->cap_hint_ = cap_hint_;
+    c->s64.hdesc = s64.hdesc;
+    c->s32.hdesc = s32.hdesc;
+    c->s64.elems = s64.elems;
+    c->s32.elems = s32.elems;
     c->device_init();
     if (structure_only) {
       // a null matrix of the same structure: zero full leaves, rank-0 Rk leaves

@@ -198,6 +198,7 @@ _EXPORTS = {
     "hmx_set_rank_capacity": (None, [C.c_int]),
     "hmx_autotune_solve": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "hmx_set_solve_block_rows": (C.c_int, [_vp, C.c_int]),
+    "hmx_set_large_block_rows": (C.c_int, [_vp, C.c_int]),
     "hmx_set_stream": (C.c_int, [_vp, _vp]),
 }
 
@@ -345,10 +346,25 @@ class HMatrix:
         ctx.factorization = factorization
         return ctx, comp
 
+    def _progress_reporter(self, progress):
+        """An hmat_progress_t calling progress(current, max), kept alive with
+        the matrix."""
+        @PROGRESS_FUNC
+        def update(p):
+            progress(p[0].current, p[0].max)
+        prog = hmat_progress_t(0, 0, update, None)
+        self._progress = (prog, update)
+        return C.cast(C.pointer(prog), _vp)
+
     def assemble(self, kind=KERNEL_EXPONENTIAL, length=0.5, nugget=1e-2, eps=1e-4,
-                 method="aca_plus", factorization=FACT_NONE, shift=0.0, wavenumber=0.0, reg=0.0):
+                 method="aca_plus", factorization=FACT_NONE, shift=0.0, wavenumber=0.0, reg=0.0,
+                 progress=None):
+        """assemble_generic with the device kernel; progress(current, max) is
+        called with the number of leaves assembled as the assembly advances."""
         ctx, comp = self.assemble_context(kind, length, nugget, eps, method, factorization, shift,
                                           wavenumber, reg)
+        if progress is not None:
+            ctx.progress = self._progress_reporter(progress)
         rc = self.iface.assemble_generic(self.handle, C.byref(ctx))
         load().hmat_delete_compression(comp)
         _check(rc, "assemble_generic")
@@ -386,12 +402,7 @@ class HMatrix:
         load().hmat_factorization_context_init(C.byref(ctx))
         ctx.factorization = algo
         if progress is not None:
-            @PROGRESS_FUNC
-            def update(p):
-                progress(p[0].current, p[0].max)
-            prog = hmat_progress_t(0, 0, update, None)
-            self._progress = (prog, update)
-            ctx.progress = C.cast(C.pointer(prog), _vp)
+            ctx.progress = self._progress_reporter(progress)
         _check(self.iface.factorize_generic(self.handle, C.byref(ctx)), "factorize_generic")
 
     def info(self):
@@ -648,6 +659,11 @@ class HMatrix:
 
     def set_solve_block_rows(self, rows):
         _check(load().hmx_set_solve_block_rows(self.handle, rows), "hmx_set_solve_block_rows")
+
+    def set_large_block_rows(self, rows):
+        """Complex blocks of at least this many rows and columns are
+        compressed on the whole device (randomized SVD) at the next assembly."""
+        _check(load().hmx_set_large_block_rows(self.handle, rows), "hmx_set_large_block_rows")
 
     def get_block(self, row_offset, row_size, col_offset, col_size, renumber_rows=False):
         """get_block: the dense block of rows [row_offset, +row_size) x cols

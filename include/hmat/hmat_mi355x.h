@@ -154,6 +154,15 @@ HMAT_API void hmx_set_rank_capacity(int cap);
 HMAT_API int hmx_autotune_solve(hmat_matrix_t *m, int nrhs, int *best_rows, double *best_ms);
 HMAT_API int hmx_set_solve_block_rows(hmat_matrix_t *m, int rows);
 
+/* No counterpart in hmat-oss (the engine's own scheduling).  Admissible blocks
+ * of a complex matrix with at least `rows` rows and columns (default 4096) are
+ * compressed by a randomized SVD over the whole device (dense evaluation,
+ * GEMM, QR and SVD, cut at eps as the SVD compression cuts) instead of one
+ * workgroup's partial ACA: a block of tens of thousands of rows with a rank
+ * of several hundred otherwise holds one compute unit for minutes.  Takes
+ * effect at the next assembly. */
+HMAT_API int hmx_set_large_block_rows(hmat_matrix_t *m, int rows);
+
 /* Test hooks of the task kernels (no counterpart in hmat-oss): the size of
  * one task record (hmx::FTask of the engine's ftask.hpp, which the tests
  * mirror), and one batched launch of `count` records of one task type on

@@ -133,6 +133,80 @@ def test_helmholtz_assembly_gemv_lu(n, leaf, height):
 
 
 @pytest.mark.gpu
+def test_helmholtz_large_blocks_on_the_whole_device():
+    """Blocks of at least 300 complex rows and columns compressed by the
+    randomized SVD over the whole device instead of partial ACA: each such
+    block equals the exact kernel block to the cut (eps of its largest
+    singular value) with the rank of the exact truncated SVD (numpy), and the
+    matrix, its copy and its H-LU solve behave as after an ACA assembly."""
+    n, leaf, rows = 3000, 64, 300
+    say(f"complex Helmholtz, whole-device compression of blocks >= {rows} rows, n={n}")
+    pts, hreg, k = problem(n, 1.0, 11)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    idx = t.indices
+    ker = ho.HelmholtzKernel(pts, k, hreg, 1.0)
+    ranks = {}
+    dense = {}
+    for big in (False, True):
+        h = hmat_amd.HMatrix(t, lower_symmetric=False, value_type=hmat_amd.HMAT_DOUBLE_COMPLEX)
+        if big:
+            h.set_large_block_rows(rows)
+        h.assemble(kind=hmat_amd.KERNEL_HELMHOLTZ, wavenumber=k, reg=hreg, nugget=1.0, eps=EPS,
+                   method="aca_partial")
+        tab = h.leaf_table()
+        ranks[big] = tab[:, 5].copy()
+        dense[big] = h.to_dense()
+        if big:
+            hb = h
+        else:
+            h.close()
+    tab = hb.leaf_table()
+    sel = [i for i, (m, nn, r0, c0, kind, _) in enumerate(tab)
+           if kind == 1 and min(m, nn) >= 2 * rows]
+    say(f"{len(sel)} blocks compressed on the whole device")
+    assert len(sel) >= 2
+    H = dense[True]
+    worst = 0.0
+    for i in sel:
+        m, nn, r0, c0 = (int(v) for v in tab[i, :4])
+        rs, cs = slice(r0 // 2, (r0 + m) // 2), slice(c0 // 2, (c0 + nn) // 2)
+        ex = ker.block(idx[rs], idx[cs])
+        sv = np.linalg.svd(ex, compute_uv=False)
+        kx = ho.find_k(sv, EPS)
+        err = np.linalg.norm(H[rs, cs] - ex, 2) / sv[0]
+        worst = max(worst, err)
+        say(f"block {m // 2} x {nn // 2}: complex rank {ranks[True][i] / 2:.1f} (exact truncated SVD {kx}, "
+            f"partial ACA {ranks[False][i] / 2:.1f}), spectral error / s0 {err:.2e}")
+        # the real representation doubles every singular value's multiplicity
+        assert abs(int(ranks[True][i]) - 2 * kx) <= 2
+        assert int(ranks[True][i]) <= int(ranks[False][i]) + 2
+        assert err <= 2 * EPS
+    exact = ker.block(idx, idx)
+    err_exact = np.linalg.norm(H - exact) / np.linalg.norm(exact)
+    say(f"whole matrix vs exact kernel {err_exact:.2e} (ACA assembly "
+        f"{np.linalg.norm(dense[False] - exact) / np.linalg.norm(exact):.2e})")
+    assert err_exact <= 2 * EPS
+    rng = np.random.default_rng(3)
+    x = cnormal(rng, (n, 2))
+    D, Du = embedded_user(hb, idx)
+    ax = complex_view(Du @ real_view(x))
+    c = hb.copy()
+    g = np.linalg.norm(hb.gemv(x) - ax) / np.linalg.norm(ax)
+    gc = np.linalg.norm(c.gemv(x) - ax) / np.linalg.norm(ax)
+    c.close()
+    say(f"gemv vs dense product of the stored matrix {g:.2e}, of its copy {gc:.2e}")
+    assert g <= 1e-13 and gc <= 1e-13
+    hb.factorize(hmat_amd.FACT_LU)
+    b = cnormal(rng, (n, 4))
+    xs = hb.solve(b)
+    xe = complex_view(np.linalg.solve(Du, real_view(b)))
+    rel = np.linalg.norm(xs - xe) / np.linalg.norm(xe)
+    say(f"H-LU solve vs dense solve of the stored matrix {rel:.2e}")
+    assert rel <= 1e-3
+    hb.close()
+
+
+@pytest.mark.gpu
 def test_complex_block_callback_and_device_entries():
     import torch
     n, leaf = 2000, 64

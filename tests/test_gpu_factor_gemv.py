@@ -96,14 +96,22 @@ def test_hodlr_factor_gemv():
 
 @pytest.mark.gpu
 def test_factorisation_progress():
-    """hmat_progress_t of the factorisation context: max = N, current = the end
-    row of the last diagonal leaf factorised, non-decreasing, ending at N."""
+    """hmat_progress_t of the assembly context (leaves assembled) and of the
+    factorisation context: max = N, current = the end row of the last diagonal
+    leaf factorised, non-decreasing, ending at N."""
     n = 20000
     say(f"factorisation progress n={n}")
     pts = ho.cylinder_points(n, 1.0, 1.6, 2)
     t = hmat_amd.ClusterTree(pts, 128)
     h = hmat_amd.HMatrix(t)
-    h.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    asm = []
+    h.assemble(length=0.5, nugget=1e-2, eps=EPS, progress=lambda cur, mx: asm.append((cur, mx)))
+    # the assembly's reporter: max = the number of leaves, current = leaves
+    # assembled, non-decreasing, ending at max
+    nleaf = len(h.leaf_table())
+    say(f"assembly progress calls: {asm}, {nleaf} leaves")
+    assert asm and all(mx == nleaf for _, mx in asm)
+    assert asm[-1][0] == nleaf and all(a[0] <= b[0] for a, b in zip(asm, asm[1:]))
     seen = []
     h.factorize(hmat_amd.FACT_LLT, progress=lambda cur, mx: seen.append((cur, mx)))
     say(f"progress calls: {seen}")
