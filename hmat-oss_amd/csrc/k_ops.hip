@@ -4,7 +4,7 @@
 // multiple of the identity (HMatrix::addIdentity) and reading the diagonal
 // (HMatrix::extractDiagonal).  One workgroup per leaf of a list.  Also the
 // element-wise helpers of the whole-device compression of very large blocks
-// (seeded normal numbers, square-root scalings, triangle copies).
+// (seeded normal numbers, scaled eigenvector columns).
 #include "kernels.hpp"
 
 namespace hmx {
@@ -135,25 +135,17 @@ __global__ void k_gauss(double* out, size_t n, uint64_t seed) {
   }
 }
 
-// v (ld rows) scaled by sqrt(s): rows j < k (by_rows) or columns j < k
-__global__ void k_scale_sqrt(double* v, int ld, int rows, int cols, const double* s, int k,
-                             int by_rows) {
-  const size_t total = (size_t)rows * cols;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e % rows), j = (int)(e / rows);
-    const int q = by_rows ? i : j;
-    if (q < k) v[i + (size_t)j * ld] *= sqrt(s[q]);
-  }
-}
-
-// dst (p x p) = the upper triangle of src (ld lds), zeros below
-__global__ void k_upper(const double* src, int lds, double* dst, int p) {
-  const size_t total = (size_t)p * p;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e % p), j = (int)(e / p);
-    dst[e] = i <= j ? src[i + (size_t)j * lds] : 0.0;
+// out (rows x count, ld rows) column c = V column first + count - 1 - c
+// (ld ldv) times lam[first + count - 1 - c]^e: the last `count` columns of an
+// eigenvector matrix (eigenvalues ascending), largest first, scaled
+__global__ void k_pick_scale(const double* V, int ldv, int rows, int first, int count,
+                             const double* lam, double e, double* out) {
+  const size_t total = (size_t)rows * count;
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < total;
+       q += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(q % rows), c = (int)(q / rows);
+    const int j = first + count - 1 - c;
+    out[q] = V[i + (size_t)j * ldv] * pow(lam[j], e);
   }
 }
 
@@ -168,15 +160,10 @@ void launch_gauss(double* out, size_t n, uint64_t seed, hipStream_t s) {
   HMX_CHECK(hipGetLastError());
 }
 
-void launch_scale_sqrt(double* v, int ld, int rows, int cols, const double* sv, int k, bool by_rows,
-                       hipStream_t s) {
-  hipLaunchKernelGGL(k_scale_sqrt, dim3(grid_for((size_t)rows * cols)), dim3(256), 0, s, v, ld,
-                     rows, cols, sv, k, by_rows ? 1 : 0);
-  HMX_CHECK(hipGetLastError());
-}
-
-void launch_upper(const double* src, int lds, double* dst, int p, hipStream_t s) {
-  hipLaunchKernelGGL(k_upper, dim3(grid_for((size_t)p * p)), dim3(256), 0, s, src, lds, dst, p);
+void launch_pick_scale(const double* V, int ldv, int rows, int first, int count,
+                       const double* lam, double e, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_pick_scale, dim3(grid_for((size_t)rows * count)), dim3(256), 0, s, V, ldv,
+                     rows, first, count, lam, e, out);
   HMX_CHECK(hipGetLastError());
 }
 

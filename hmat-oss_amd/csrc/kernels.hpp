@@ -114,12 +114,11 @@ void launch_leaf_get_diag(const LeafDesc* desc, const int* list, int count, cons
                           double* out, hipStream_t s);
 
 // Whole-device compression of very large blocks (giant.cpp): seeded standard
-// normal numbers; v scaled by sqrt(s[j]) in its rows (or columns) j < k; the
-// upper triangle of a p x p block (ld lds) into a p x p matrix, zeros below.
+// normal numbers; the last `count` columns of V (ld ldv, eigenvalues lam
+// ascending), largest first, column j scaled by lam[j]^e, into out (ld rows).
 void launch_gauss(double* out, size_t n, uint64_t seed, hipStream_t s);
-void launch_scale_sqrt(double* v, int ld, int rows, int cols, const double* sv, int k, bool by_rows,
-                       hipStream_t s);
-void launch_upper(const double* src, int lds, double* dst, int p, hipStream_t s);
+void launch_pick_scale(const double* V, int ldv, int rows, int first, int count,
+                       const double* lam, double e, double* out, hipStream_t s);
 
 // Fixed device scratch of the allocation-free H-gemv (64 MiB per GPU); returns
 // its device address on the current device (looked up once per device).
