@@ -637,11 +637,6 @@ int u_diag_z(hmat_matrix_t*, void*, int) { return unsupported("extract_diagonal 
 int u_tri_z(hmat_matrix_t*, int, void*, int) {
   return unsupported("solve_lower_triangular of a complex matrix (LLt only)");
 }
-hmat_matrix_t* u_read_z(hmat_iostream, void*) {
-  unsupported("read_struct of a complex matrix");
-  return nullptr;
-}
-void u_write_z(hmat_matrix_t*, hmat_iostream, void*) { unsupported("serialization of a complex matrix"); }
 
 }  // namespace
 
@@ -675,10 +670,9 @@ void hmat_init_mi355x_interface(hmat_interface_t* i, hmat_value_t type) {
       i->solve_lower_triangular_dense = u_tri_z;
       i->get_block = u_values;
       i->get_values = u_values;
-      i->read_struct = u_read_z;
-      i->write_struct = u_write_z;
-      i->read_data = u_write_z;
-      i->write_data = u_write_z;
+      // the stream holds the complex matrix (unfactorised); the engine reads
+      // it into the real representation
+      i->read_struct = read_struct<HMAT_DOUBLE_COMPLEX>;
       break;
     default:
       // single-precision complex: the table is filled, creation reports the limitation

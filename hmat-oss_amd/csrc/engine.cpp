@@ -879,7 +879,7 @@ void Matrix::gemv_host(char trans, const void* alpha, void* x, const void* beta,
     else if (value_type == HMAT_SIMPLE_PRECISION)
       factor_gemv_host_t<float>(*this, trans, alpha, x, beta, y, nrhs, permute);
     else
-      throw std::runtime_error("gemv: complex types are not supported by this engine");
+      throw std::runtime_error("gemv: unsupported value type");
     return;
   }
   if (trans != 'N' && !lower_sym) throw std::runtime_error("gemv: only 'N' for non-symmetric matrices");
@@ -888,7 +888,7 @@ void Matrix::gemv_host(char trans, const void* alpha, void* x, const void* beta,
   else if (value_type == HMAT_SIMPLE_PRECISION)
     gemv_host_t<float>(*this, alpha, x, beta, y, nrhs, permute);
   else
-    throw std::runtime_error("gemv: complex types are not supported by this engine");
+    throw std::runtime_error("gemv: unsupported value type");
 }
 
 void Matrix::info(hmat_info_t& out) const {

@@ -2884,7 +2884,8 @@ void Matrix::solve_host(void* b, int nrhs, bool permute, int part) {
   } else if (value_type == HMAT_SIMPLE_PRECISION) {
     run((float*)nullptr);
   } else {
-    throw std::runtime_error("solve: complex types are not supported by this engine");
+    // complex matrices are stored as their real representation (double)
+    throw std::runtime_error("solve: unsupported value type");
   }
 }
 

@@ -45,10 +45,12 @@ struct In {
   }
 };
 
+// ScalarTypes data_types.hpp:37-46
 int type_code(int value_type) {
   if (value_type == HMAT_SIMPLE_PRECISION) return 0;
   if (value_type == HMAT_DOUBLE_PRECISION) return 1;
-  throw std::runtime_error("serialization: complex types are not supported by this engine");
+  if (value_type == HMAT_DOUBLE_COMPLEX) return 3;
+  throw std::runtime_error("serialization: single-precision complex matrices are not provided");
 }
 
 // MatrixStructMarshaller::write(ClusterTree) serialization.cpp:34-53
@@ -129,15 +131,30 @@ struct Serializer {
     return r;
   }
 
+  // A complex matrix is held as its real representation (DoF 2u + c of the
+  // embedded numbering is component c of DoF u, every entry a 2 x 2 block
+  // [[Re, -Im], [Im, Re]]).  The stream holds the complex matrix: the caller's
+  // cluster trees, the same block tree, complex leaves.  An embedded Rk leaf
+  // A B^T of rank k is written as the complex Rk (A_even + i A_odd) B_even^T of
+  // rank k (the first column of each 2 x 2 block; the two columns agree to the
+  // compression), a full leaf as the average of the two places holding Re and
+  // Im.  Factors are not written: the row interchanges of the real
+  // representation's LU are not interchanges of complex rows.
+  static void check_complex(const Matrix& M) {
+    if (M.embedded && M.factorization != -1)
+      throw std::runtime_error("serialization of a factorised complex matrix is not provided");
+  }
+
   // MatrixStructMarshaller::write(HMatrix) serialization.cpp:63-130
   static void write_struct(Matrix& M, hmat_iostream f, void* ud) {
     if (!M.assembled) throw std::runtime_error("write_struct: matrix not assembled");
+    check_complex(M);
     Out o{f, ud};
     const int fact = written_factorization(M);
-    o.put<int>(type_code(M.value_type));
+    o.put<int>(type_code(M.embedded ? HMAT_DOUBLE_COMPLEX : M.value_type));
     o.put<int>(fact);
-    write_cluster_tree(o, *M.rows);
-    write_cluster_tree(o, *M.cols);
+    write_cluster_tree(o, M.embedded ? *M.zrows : *M.rows);
+    write_cluster_tree(o, M.embedded ? *M.zcols : *M.cols);
     const std::vector<int> rk = ranks(M);
     // lltDecomposition / ldltDecomposition flags, h_matrix.cpp:2562, 2766
     const bool tri = fact == hmat_factorization_llt || fact == hmat_factorization_ldlt;
@@ -214,11 +231,59 @@ struct Serializer {
     }
   }
 
+  // the complex leaves of an embedded matrix (see check_complex)
+  static void write_complex_leaves(Matrix& M, Out& o, const std::vector<int>& rk) {
+    const Store<double>& st = M.s64;
+    std::vector<double> buf, z;
+    for (const Block* b : M.bt->leaves) {
+      const LeafDesc& L = st.hdesc[b->leaf];
+      const int m = L.m / 2, n = L.n / 2;
+      if (L.kind == 1) {
+        const int k = std::max(rk[b->leaf], 0);
+        o.put<int>(k);
+        if (k > 0) {
+          buf.resize((size_t)L.m * k);
+          HMX_CHECK(hipMemcpy(buf.data(), st.data() + L.a, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+          z.assign(2 * (size_t)m * k, 0.0);
+          for (int l = 0; l < k; ++l)
+            for (int u = 0; u < m; ++u) {
+              z[2 * (u + (size_t)l * m)] = buf[2 * u + (size_t)l * L.m];
+              z[2 * (u + (size_t)l * m) + 1] = buf[2 * u + 1 + (size_t)l * L.m];
+            }
+          o.bytes(z.data(), z.size() * sizeof(double));
+          buf.resize((size_t)L.n * k);
+          HMX_CHECK(hipMemcpy(buf.data(), st.data() + L.b, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+          z.assign(2 * (size_t)n * k, 0.0);
+          for (int l = 0; l < k; ++l)
+            for (int v = 0; v < n; ++v) z[2 * (v + (size_t)l * n)] = buf[2 * v + (size_t)l * L.n];
+          o.bytes(z.data(), z.size() * sizeof(double));
+          o.put<int>(0);
+          o.put<int>(0);
+        }
+        continue;
+      }
+      o.put<int>(0);
+      buf.resize((size_t)L.m * L.n);
+      HMX_CHECK(hipMemcpy(buf.data(), st.data() + L.a, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+      z.resize(2 * (size_t)m * n);
+      auto at = [&](int i, int j) { return buf[i + (size_t)j * L.m]; };
+      for (int v = 0; v < n; ++v)
+        for (int u = 0; u < m; ++u) {
+          z[2 * (u + (size_t)v * m)] = 0.5 * (at(2 * u, 2 * v) + at(2 * u + 1, 2 * v + 1));
+          z[2 * (u + (size_t)v * m) + 1] = 0.5 * (at(2 * u + 1, 2 * v) - at(2 * u, 2 * v + 1));
+        }
+      o.bytes(z.data(), z.size() * sizeof(double));
+    }
+  }
+
   static void write_data(Matrix& M, hmat_iostream f, void* ud) {
     if (!M.assembled) throw std::runtime_error("write_data: matrix not assembled");
+    check_complex(M);
     Out o{f, ud};
     const std::vector<int> rk = ranks(M);
-    if (use32(M))
+    if (M.embedded)
+      write_complex_leaves(M, o, rk);
+    else if (use32(M))
       write_leaves<float>(M, o, M.s32, rk);
     else
       write_leaves<double>(M, o, M.s64, rk);
@@ -235,6 +300,9 @@ struct Serializer {
     if (fact != -1 && fact != hmat_factorization_lu && fact != hmat_factorization_llt &&
         fact != hmat_factorization_ldlt)
       throw std::runtime_error("read_struct: only unfactorised, LU, LDLt and LLt matrices are supported");
+    const bool complex = value_type == HMAT_DOUBLE_COMPLEX;
+    if (complex && fact != -1)
+      throw std::runtime_error("read_struct: a factorised complex matrix is not provided");
     std::unique_ptr<ClusterTree> rt = read_cluster_tree(in);
     in.end_marker();
     std::unique_ptr<ClusterTree> ct = read_cluster_tree(in);
@@ -278,6 +346,27 @@ struct Serializer {
           if (q < hint.size()) hint[q] = std::max(d.rank, 0);
           ++q;
         }
+    }
+    if (complex) {
+      // the real representation of the stream's complex matrix: embedded
+      // trees and block tree, a complex rank k taking real rank 2k
+      std::shared_ptr<ClusterTree> er(embed_cluster_tree(*rows));
+      std::shared_ptr<ClusterTree> ec = sym_tree ? er : std::shared_ptr<ClusterTree>(embed_cluster_tree(*cols));
+      std::unique_ptr<BlockTree> et = embed_block_tree(*bt, *rows, *er, *cols, *ec);
+      for (int& h : hint) h *= 2;
+      auto* M = new Matrix(er.get(), ec.get(), false, std::move(et), HMAT_DOUBLE_PRECISION,
+                           2 * rank_cap, hint);
+      M->embedded = true;
+      M->zrows = rows;
+      M->zcols = cols;
+      M->erows = er;
+      M->ecols = ec;
+      M->own_rows = true;
+      M->own_cols = true;
+      rt.release();
+      if (!sym_tree) ct.release();
+      M->lowrank_eps = eps;
+      return M;
     }
     auto* M = new Matrix(rows, cols, lower, std::move(bt), value_type, rank_cap, hint);
     M->own_rows = true;
@@ -345,10 +434,67 @@ struct Serializer {
     }
   }
 
+  // complex leaves into the real representation: an Rk (A, B) of complex
+  // rank k becomes the real Rk (R(A), R(conj B)) of rank 2k, R(X) the 2m x 2k
+  // real matrix of the 2 x 2 blocks [[Re x, -Im x], [Im x, Re x]] (R(A B^T) =
+  // R(A) R(conj B)^T); a full leaf Z becomes R(Z)
+  static void read_complex_leaves(Matrix& M, In& in) {
+    Store<double>& st = M.s64;
+    std::vector<double> image(st.elems, 0.0);
+    std::vector<int> rk(st.hdesc.size());
+    std::vector<double> z;
+    auto embed = [](const double* z, int rows, int cols, bool conj, double* out, int ld) {
+      for (int l = 0; l < cols; ++l)
+        for (int u = 0; u < rows; ++u) {
+          const double re = z[2 * (u + (size_t)l * rows)];
+          const double im = conj ? -z[2 * (u + (size_t)l * rows) + 1] : z[2 * (u + (size_t)l * rows) + 1];
+          out[2 * u + (size_t)(2 * l) * ld] = re;
+          out[2 * u + 1 + (size_t)(2 * l) * ld] = im;
+          out[2 * u + (size_t)(2 * l + 1) * ld] = -im;
+          out[2 * u + 1 + (size_t)(2 * l + 1) * ld] = re;
+        }
+    };
+    for (const Block* b : M.bt->leaves) {
+      const LeafDesc& L = st.hdesc[b->leaf];
+      const int m = L.m / 2, n = L.n / 2;
+      const int header = in.get<int>();
+      if (L.kind == 1) {
+        const int k = header;
+        if (k < 0 || 2 * k > L.cap)
+          throw std::runtime_error("read_data: Rk rank " + std::to_string(k) + " above the leaf capacity");
+        rk[b->leaf] = 2 * k;
+        if (k > 0) {
+          z.resize(2 * (size_t)m * k);
+          in.bytes(z.data(), z.size() * sizeof(double));
+          embed(z.data(), m, k, false, image.data() + L.a, L.m);
+          z.resize(2 * (size_t)n * k);
+          in.bytes(z.data(), z.size() * sizeof(double));
+          embed(z.data(), n, k, true, image.data() + L.b, L.n);
+          in.get<int>();
+          in.get<int>();
+        }
+        continue;
+      }
+      rk[b->leaf] = -1;
+      if (header & 1) continue;  // null full block
+      if (header & 6) throw std::runtime_error("read_data: factor data in an unfactorised complex matrix");
+      z.resize(2 * (size_t)m * n);
+      in.bytes(z.data(), z.size() * sizeof(double));
+      embed(z.data(), m, n, false, image.data() + L.a, L.m);
+    }
+    in.end_marker();
+    M.init_store(st);
+    HMX_CHECK(hipMemcpy(st.arena.p, image.data(), image.size() * sizeof(double), hipMemcpyHostToDevice));
+    HMX_CHECK(hipMemcpy(st.ranks.p, rk.data(), rk.size() * sizeof(int), hipMemcpyHostToDevice));
+    st.valid = true;
+  }
+
   static void read_data(Matrix& M, hmat_iostream f, void* ud) {
     In in{f, ud};
     M.device_init();
-    if (M.value_type == HMAT_SIMPLE_PRECISION)
+    if (M.embedded)
+      read_complex_leaves(M, in);
+    else if (M.value_type == HMAT_SIMPLE_PRECISION)
       read_leaves<float>(M, in, M.s32);
     else
       read_leaves<double>(M, in, M.s64);

@@ -620,7 +620,7 @@ class HMatrix:
         m = object.__new__(HMatrix)
         m.tree = _BorrowedTree(r.value)
         m.iface, m.value_type, m.handle, m._kernel = iface, value_type, h, None
-        m.lower_symmetric = bool(r.value == c.value)
+        m.lower_symmetric = bool(r.value == c.value) and value_type != HMAT_DOUBLE_COMPLEX
         m._consumed = pos[0]
         return m
 

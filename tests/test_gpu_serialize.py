@@ -144,3 +144,85 @@ def test_malformed_streams_raise():
         g.read_data(bytes(bad))
     g.close()
     h.close()
+
+
+def _complex_leaves(data, table):
+    """The complex matrix (hmat numbering) of a complex data stream: leaves in
+    preorder, a full leaf as flag + m n complex values, an Rk leaf as rank k +
+    (m + n) k complex values + two flags (MatrixDataMarshaller with Z_t);
+    `table` gives each leaf's embedded shape and offsets."""
+    nr = int(max(r[2] + r[0] for r in table)) // 2
+    nc = int(max(r[3] + r[1] for r in table)) // 2
+    Z = np.zeros((nr, nc), dtype=np.complex128)
+    pos = 0
+    for m2, n2, r2, c2, kind, _ in table:
+        m, n, r0, c0 = int(m2) // 2, int(n2) // 2, int(r2) // 2, int(c2) // 2
+        head = struct.unpack_from("<i", data, pos)[0]
+        pos += 4
+        if kind == 0:
+            v = np.frombuffer(data, "<c16", m * n, pos).reshape(n, m).T
+            pos += 16 * m * n
+            Z[r0:r0 + m, c0:c0 + n] = v
+        elif head > 0:
+            a = np.frombuffer(data, "<c16", m * head, pos).reshape(head, m).T
+            pos += 16 * m * head
+            b = np.frombuffer(data, "<c16", n * head, pos).reshape(head, n).T
+            pos += 16 * n * head + 8
+            Z[r0:r0 + m, c0:c0 + n] = a @ b.T
+    assert pos == len(data)
+    return Z
+
+
+@pytest.mark.gpu
+def test_complex_round_trip():
+    """A complex Helmholtz matrix in the complex stream format: the structure
+    bytes equal the oracle's MatrixStructMarshaller::write of the complex
+    partition (scalar type 3, the caller's trees, each Rk leaf at its rank);
+    the data stream holds the complex leaves (equal to the stored matrix to the
+    compression); read back into the real representation, the matrix is that
+    of the stream to rounding and its gemv is the original's."""
+    from scipy.spatial import cKDTree
+    n, leaf = 2000, 64
+    say(f"complex round trip n={n} leaf={leaf}")
+    pts = ho.cylinder_points(n, 1.0, 0.8, 5)
+    dd, _ = cKDTree(pts).query(pts, 2)
+    hreg = float(dd[:, 1].mean())
+    k = 2 * np.pi / (10 * hreg)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    h = hmat_amd.HMatrix(t, lower_symmetric=False, value_type=hmat_amd.HMAT_DOUBLE_COMPLEX)
+    h.assemble(kind=hmat_amd.KERNEL_HELMHOLTZ, wavenumber=k, reg=hreg, nugget=1.0, eps=EPS,
+               method="aca_partial")
+    s, d = h.write_struct(), h.write_data()
+    assert struct.unpack_from("<ii", s) == (3, -1)
+    tab = h.leaf_table()
+    root, idx = ho.build_cluster_tree(pts, leaf)
+    assert np.array_equal(idx, t.indices)
+    bt = ho.build_block_tree(root, root, 2.0, False)
+    ranks = [int(r[5]) for r in tab if r[4] != 0]
+    ref = ho.write_struct(bt, root, root, idx, idx, pts, EPS, value_type=3, lower_symmetric=False,
+                          ranks=ranks)
+    say(f"structure {len(s)} bytes (oracle {len(ref)}), data {len(d)} bytes")
+    assert s == ref
+    Zs = _complex_leaves(d, tab)
+    H = h.to_dense()
+    dev = np.linalg.norm(Zs - H) / np.linalg.norm(H)
+    say(f"stream's complex matrix vs the stored matrix {dev:.2e}")
+    assert dev <= EPS
+    g = hmat_amd.HMatrix.read_struct(s, hmat_amd.HMAT_DOUBLE_COMPLEX)
+    assert g._consumed == len(s)
+    assert g.read_data(d) == len(d)
+    G = g.to_dense()
+    back = np.linalg.norm(G - Zs) / np.linalg.norm(Zs)
+    x = np.random.default_rng(4).standard_normal((n, 2)) + 1j * np.random.default_rng(5).standard_normal((n, 2))
+    gx, hx = g.gemv(x), h.gemv(x)
+    gv = np.linalg.norm(gx - hx) / np.linalg.norm(hx)
+    say(f"read matrix vs the stream's {back:.2e}; gemv vs the original's {gv:.2e}")
+    assert back <= 1e-14 and gv <= EPS
+    # factorised complex matrices are not written; a real reader refuses the stream
+    with pytest.raises(hmat_amd.EngineError):
+        hmat_amd.HMatrix.read_struct(s, hmat_amd.HMAT_DOUBLE_PRECISION)
+    h.factorize(hmat_amd.FACT_LU)
+    with pytest.raises(hmat_amd.EngineError):
+        h.write_struct()
+    g.close()
+    h.close()
