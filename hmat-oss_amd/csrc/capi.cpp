@@ -375,7 +375,6 @@ int u_set_fp(hmat_matrix_t*, hmat_fp_settings_t) { return unsupported("SetFPComp
 int u_set_fp2(hmat_matrix_t*, float, int, hmat_FPcompress_t, bool, bool) { return unsupported("SetFPCompressionSettingsParams"); }
 int u_dump(hmat_matrix_t*, char*) { return unsupported("dump_info"); }
 int u_set_ct(hmat_matrix_t*, const hmat_cluster_tree_t*, const hmat_cluster_tree_t*) { return unsupported("set_cluster_trees"); }
-int u_values(struct hmat_get_values_context_t*) { return unsupported("get_block/get_values"); }
 
 // get_block c_wrapping.hpp:832-868 (UncompressedBlock): the rows [row_offset,
 // +row_size) x cols [col_offset, +col_size) block in hmat numbering,
@@ -632,6 +631,73 @@ int destroy_z(hmat_matrix_t* h) {
   if (c && c != r) delete c;
   return 0;
 }
+// get_block / get_values of a complex matrix: entry (p, q) of the complex
+// matrix (hmat numbering) is read from the real representation's 2 x 2 block
+// at rows 2p, 2p + 1 and columns 2q, 2q + 1 (the average of the two places
+// holding Re and of those holding Im, as the compression keeps them to eps)
+void complex_entries(Matrix* m, const std::vector<int>& rp, const std::vector<int>& cp, zd* out) {
+  if (m->factorization != -1)
+    throw std::runtime_error("get_block/get_values: the factor of a complex matrix is held as the real "
+                             "representation's LU, which has no complex entries");
+  const int nr = (int)rp.size(), nc = (int)cp.size();
+  std::vector<int> er(2 * (size_t)nr), ec(2 * (size_t)nc);
+  for (int i = 0; i < nr; ++i) er[2 * i] = 2 * rp[i], er[2 * i + 1] = 2 * rp[i] + 1;
+  for (int j = 0; j < nc; ++j) ec[2 * j] = 2 * cp[j], ec[2 * j + 1] = 2 * cp[j] + 1;
+  std::vector<double> v(4 * (size_t)nr * nc);
+  m->get_entries(er, ec, v.data(), 2 * nr);
+  const size_t ld = 2 * (size_t)nr;
+  for (int j = 0; j < nc; ++j)
+    for (int i = 0; i < nr; ++i) {
+      const double a = v[2 * i + 2 * j * ld], b = v[2 * i + 1 + 2 * j * ld];
+      const double c = v[2 * i + (2 * j + 1) * ld], d = v[2 * i + 1 + (2 * j + 1) * ld];
+      out[i + (size_t)j * nr] = zd(0.5 * (a + d), 0.5 * (b - c));
+    }
+}
+int get_block_z(struct hmat_get_values_context_t* ctx) {
+  return guard([&] {
+    Matrix* m = MZ(ctx->matrix);
+    const hmx::ClusterTree& R = *m->zrows;
+    const hmx::ClusterTree& Cl = *m->zcols;
+    if (ctx->row_offset < 0 || ctx->col_offset < 0 || ctx->row_offset + ctx->row_size > R.n ||
+        ctx->col_offset + ctx->col_size > Cl.n)
+      throw std::runtime_error("get_block: block out of range");
+    std::vector<int> rp(ctx->row_size), cp(ctx->col_size);
+    for (int i = 0; i < ctx->row_size; ++i) rp[i] = ctx->row_offset + i;
+    for (int j = 0; j < ctx->col_size; ++j) cp[j] = ctx->col_offset + j;
+    std::vector<zd> v((size_t)ctx->row_size * ctx->col_size);
+    complex_entries(m, rp, cp, v.data());
+    zd* out = static_cast<zd*>(ctx->values);
+    if (ctx->renumber_rows) {
+      if (ctx->row_offset != 0 || ctx->row_size != R.n) throw std::runtime_error("Cannot renumber");
+      for (int j = 0; j < ctx->col_size; ++j)
+        for (int i = 0; i < ctx->row_size; ++i)
+          out[R.indices[i] + (size_t)j * ctx->row_size] = v[i + (size_t)j * ctx->row_size];
+    } else {
+      std::copy(v.begin(), v.end(), out);
+    }
+    ctx->row_indices = const_cast<int*>(R.indices.data()) + ctx->row_offset;
+    ctx->col_indices = const_cast<int*>(Cl.indices.data()) + ctx->col_offset;
+  });
+}
+int get_values_z(struct hmat_get_values_context_t* ctx) {
+  return guard([&] {
+    Matrix* m = MZ(ctx->matrix);
+    const hmx::ClusterTree& R = *m->zrows;
+    const hmx::ClusterTree& Cl = *m->zcols;
+    std::vector<int> rp(ctx->row_size), cp(ctx->col_size);
+    for (int i = 0; i < ctx->row_size; ++i) {
+      const int u = ctx->row_indices[i] - 1;
+      if (u < 0 || u >= R.n) throw std::runtime_error("get_values: row index out of range");
+      rp[i] = R.indices_rev[u];
+    }
+    for (int j = 0; j < ctx->col_size; ++j) {
+      const int u = ctx->col_indices[j] - 1;
+      if (u < 0 || u >= Cl.n) throw std::runtime_error("get_values: column index out of range");
+      cp[j] = Cl.indices_rev[u];
+    }
+    complex_entries(m, rp, cp, static_cast<zd*>(ctx->values));
+  });
+}
 int u_logdet_z(hmat_matrix_t*, double*) { return unsupported("logdet of a complex matrix"); }
 int u_diag_z(hmat_matrix_t*, void*, int) { return unsupported("extract_diagonal of a complex matrix"); }
 int u_tri_z(hmat_matrix_t*, int, void*, int) {
@@ -668,8 +734,8 @@ void hmat_init_mi355x_interface(hmat_interface_t* i, hmat_value_t type) {
       i->extract_diagonal = u_diag_z;
       i->solve_lower_triangular = u_tri_z;
       i->solve_lower_triangular_dense = u_tri_z;
-      i->get_block = u_values;
-      i->get_values = u_values;
+      i->get_block = get_block_z;
+      i->get_values = get_values_z;
       // the stream holds the complex matrix (unfactorised); the engine reads
       // it into the real representation
       i->read_struct = read_struct<HMAT_DOUBLE_COMPLEX>;

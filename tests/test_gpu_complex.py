@@ -294,3 +294,44 @@ def test_complex_svd_compression():
     say(f"complex svd: vs exact kernel {err:.2e}, compressed size / oracle's {ratio:.4f}")
     assert err <= EPS and abs(ratio - 1) <= 0.02
     h.close()
+
+
+@pytest.mark.gpu
+def test_complex_get_block_and_get_values():
+    """get_block (hmat numbering, renumber_rows) and get_values (1-based user
+    indices) of a complex matrix equal the complex matrix of the stored real
+    representation (to_dense), and get_values equals the exact kernel to the
+    compression."""
+    n, leaf = 2000, 64
+    say(f"complex get_block / get_values n={n}")
+    pts, hreg, k = problem(n, 0.8, 9)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    h = hmat_amd.HMatrix(t, lower_symmetric=False, value_type=hmat_amd.HMAT_DOUBLE_COMPLEX)
+    h.assemble(kind=hmat_amd.KERNEL_HELMHOLTZ, wavenumber=k, reg=hreg, nugget=1.0, eps=EPS,
+               method="aca_partial")
+    H = h.to_dense()            # hmat numbering
+    idx = t.indices
+    B = h.get_block(300, 500, 1200, 400)
+    e1 = np.abs(B - H[300:800, 1200:1600]).max()
+    F = h.get_block(0, n, 700, 64, renumber_rows=True)
+    Fu = np.empty_like(F)
+    Fu[idx] = H[:, 700:764]
+    e2 = np.abs(F - Fu).max()
+    rng = np.random.default_rng(2)
+    ur, uc = rng.choice(n, 37, replace=False), rng.choice(n, 29, replace=False)
+    V = h.get_values(ur, uc)
+    inv = np.empty(n, dtype=np.int64)
+    inv[idx] = np.arange(n)
+    e3 = np.abs(V - H[np.ix_(inv[ur], inv[uc])]).max()
+    ker = ho.HelmholtzKernel(pts, k, hreg, 1.0)
+    ex = ker.block(ur, uc)
+    e4 = np.linalg.norm(V - ex) / np.linalg.norm(ex)
+    say(f"get_block vs to_dense {e1:.2e}, renumbered {e2:.2e}; get_values vs to_dense {e3:.2e}, "
+        f"vs the exact kernel {e4:.2e}")
+    scale = np.abs(H).max()
+    assert e1 <= 1e-14 * scale and e2 <= 1e-14 * scale and e3 <= 1e-14 * scale
+    assert e4 <= 5 * EPS
+    h.factorize(hmat_amd.FACT_LU)
+    with pytest.raises(hmat_amd.EngineError):
+        h.get_block(0, 10, 0, 10)
+    h.close()
