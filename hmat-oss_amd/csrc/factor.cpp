@@ -2317,7 +2317,10 @@ class Exec {
 
   void maybe_flush() {
     if (live_ != 0) return;
-    if (tasks_.size() >= kMaxTasks || fs_.temp_used > fs_.temp_budget / 2) flush(false);
+    // with a progress reporter, segments of at most kMaxTasksProgress tasks so
+    // that reports come at least every few seconds on large factorisations
+    const size_t limit = M_.progress && M_.progress->update ? kMaxTasksProgress : kMaxTasks;
+    if (tasks_.size() >= limit || fs_.temp_used > fs_.temp_budget / 2) flush(false);
   }
 
   void flush(bool forced) {
@@ -2484,6 +2487,7 @@ class Exec {
   // segments of at most this many tasks, so that the device starts early and
   // the host builds the next segment while the device runs the current one
   static constexpr size_t kMaxTasks = 250000;
+  static constexpr size_t kMaxTasksProgress = 20000;
   std::vector<std::shared_ptr<SegmentPlan>> inflight_;
   std::thread submitter_;          // queues the launches of the last segment
   std::exception_ptr submit_err_;
