@@ -301,12 +301,19 @@ void Matrix::assemble_pass(const KernelSpec& k, const CompressionSpec& comp) {
                                       : "assembly: the Helmholtz kernel needs a complex matrix");
   // very large complex blocks first: their ranks may raise their leaves'
   // capacities, which changes the storage layout
+  // (a block whose dense image would take more than a third of the free
+  // device memory stays with partial ACA)
   std::vector<int> giant;
-  if (embedded && comp.method != 0)
+  if (embedded && comp.method != 0) {
+    size_t freeb = 0, total = 0;
+    HMX_CHECK(hipMemGetInfo(&freeb, &total));
     for (size_t i = 0; i < bt->leaves.size(); ++i) {
       const LeafDesc& L = s64.hdesc[i];
-      if (L.kind == 1 && std::min(L.m, L.n) >= giant_rows) giant.push_back((int)i);
+      if (L.kind == 1 && std::min(L.m, L.n) >= giant_rows &&
+          (double)L.m * L.n * sizeof(double) <= freeb / 3.0)
+        giant.push_back((int)i);
     }
+  }
   if (!giant.empty()) {
     compress_giants(giant, rp, cp, kp, std::max(comp.eps, lowrank_eps));
     bool grow = false;
