@@ -521,9 +521,11 @@ void fill(hmat_interface_t* i) {
 // are; complex scalars with a nonzero imaginary part are applied on the host
 // (gemv) or refused where the real engine would have to take them.
 using zd = std::complex<double>;
+using zc = std::complex<float>;   // single-precision complex (C)
 
+template <typename Z>
 double real_scalar(const void* a, const char* what) {
-  const zd v = *static_cast<const zd*>(a);
+  const Z v = *static_cast<const Z*>(a);
   if (v.imag() != 0.0)
     throw std::runtime_error(std::string(what) +
                              ": a complex scalar with a nonzero imaginary part is not provided");
@@ -536,6 +538,7 @@ Matrix* MZ(hmat_matrix_t* h) {
   return m;
 }
 
+template <int VT>
 hmat_matrix_t* create_empty_z(const hmat_cluster_tree_t* r, const hmat_cluster_tree_t* c,
                               int lower_sym, hmat_admissibility_t* cond) {
   try {
@@ -545,7 +548,8 @@ hmat_matrix_t* create_empty_z(const hmat_cluster_tree_t* r, const hmat_cluster_t
     hmx::AdmissibilityParams p;
     if (cond) p = reinterpret_cast<Admissibility*>(cond)->p;
     auto* m = Matrix::create_complex(reinterpret_cast<const hmx::ClusterTree*>(r),
-                                     reinterpret_cast<const hmx::ClusterTree*>(c), p, g_rank_cap);
+                                     reinterpret_cast<const hmx::ClusterTree*>(c), p, g_rank_cap,
+                                     nullptr, nullptr, VT);
     return reinterpret_cast<hmat_matrix_t*>(m);
   } catch (const std::exception& e) {
     fail(e);
@@ -572,49 +576,58 @@ int factorize_z(hmat_matrix_t* h, hmat_factorization_context_t* ctx) {
 // y = alpha A x + beta y with complex alpha, beta: real ones go to the device
 // as they are, others through t = A x on the device and the combination on
 // the host
+template <typename Z>
 void gemv_z_run(Matrix* m, char trans, const void* alpha, void* x, const void* beta, void* y,
                 int nrhs, bool permute) {
+  using R = typename Z::value_type;
   if (trans != 'N') throw std::runtime_error("gemv: only 'N' for complex matrices");
-  const zd a = *static_cast<const zd*>(alpha), b = *static_cast<const zd*>(beta);
-  if (a.imag() == 0.0 && b.imag() == 0.0) {
-    const double ar = a.real(), br = b.real();
+  const Z a = *static_cast<const Z*>(alpha), b = *static_cast<const Z*>(beta);
+  if (a.imag() == R(0) && b.imag() == R(0)) {
+    const R ar = a.real(), br = b.real();
     m->gemv_host('N', &ar, x, &br, y, nrhs, permute);
     return;
   }
-  std::vector<zd> t((size_t)m->zrows->n * nrhs);
-  const double one = 1.0, zero = 0.0;
+  std::vector<Z> t((size_t)m->zrows->n * nrhs);
+  const R one = 1, zero = 0;
   m->gemv_host('N', &one, x, &zero, t.data(), nrhs, permute);
-  zd* yz = static_cast<zd*>(y);
+  Z* yz = static_cast<Z*>(y);
   for (size_t i = 0; i < t.size(); ++i) yz[i] = a * t[i] + b * yz[i];
 }
+template <typename Z>
 int gemv_z(char trans, void* alpha, hmat_matrix_t* h, void* x, void* beta, void* y, int nrhs) {
-  return guard([&] { gemv_z_run(MZ(h), trans, alpha, x, beta, y, nrhs, true); });
+  return guard([&] { gemv_z_run<Z>(MZ(h), trans, alpha, x, beta, y, nrhs, true); });
 }
+template <typename Z>
 int gemm_scalar_z(char trans, void* alpha, hmat_matrix_t* h, void* x, void* beta, void* y, int nrhs) {
-  return guard([&] { gemv_z_run(MZ(h), trans, alpha, x, beta, y, nrhs, false); });
+  return guard([&] { gemv_z_run<Z>(MZ(h), trans, alpha, x, beta, y, nrhs, false); });
 }
+template <typename Z>
 int gemm_dense_z(char trans_b, char trans_x, char side, const void* alpha, hmat_matrix_t* h,
                  void* x, const void* beta, void* y, int nrhs) {
   return guard([&] {
     if (side != 'L' || trans_x != 'N')
       throw std::runtime_error("gemm_dense: only side 'L' with trans_x 'N' is supported");
-    gemv_z_run(MZ(h), trans_b, alpha, x, beta, y, nrhs, false);
+    gemv_z_run<Z>(MZ(h), trans_b, alpha, x, beta, y, nrhs, false);
   });
 }
+template <typename Z>
 int scale_z(void* alpha, hmat_matrix_t* h) {
-  return guard([&] { MZ(h)->scale(real_scalar(alpha, "scale")); });
+  return guard([&] { MZ(h)->scale(real_scalar<Z>(alpha, "scale")); });
 }
+template <typename Z>
 int add_identity_z(hmat_matrix_t* h, void* alpha) {
-  return guard([&] { MZ(h)->add_identity(real_scalar(alpha, "add_identity")); });
+  return guard([&] { MZ(h)->add_identity(real_scalar<Z>(alpha, "add_identity")); });
 }
+template <typename Z>
 int axpy_z(void* a, hmat_matrix_t* x, hmat_matrix_t* y) {
-  return guard([&] { MZ(y)->axpy_from(real_scalar(a, "axpy"), *MZ(x)); });
+  return guard([&] { MZ(y)->axpy_from(real_scalar<Z>(a, "axpy"), *MZ(x)); });
 }
+template <typename Z>
 int gemm_z(char ta, char tb, const void* alpha, hmat_matrix_t* a, hmat_matrix_t* b,
            const void* beta, hmat_matrix_t* c) {
   return guard([&] {
     if (ta != 'N' || tb != 'N') throw std::runtime_error("gemm: only 'N', 'N' for complex matrices");
-    MZ(c)->gemm_from('N', 'N', real_scalar(alpha, "gemm"), *MZ(a), *MZ(b), real_scalar(beta, "gemm"));
+    MZ(c)->gemm_from('N', 'N', real_scalar<Z>(alpha, "gemm"), *MZ(a), *MZ(b), real_scalar<Z>(beta, "gemm"));
   });
 }
 int get_cluster_trees_z(hmat_matrix_t* h, const hmat_cluster_tree_t** r, const hmat_cluster_tree_t** c) {
@@ -635,7 +648,8 @@ int destroy_z(hmat_matrix_t* h) {
 // matrix (hmat numbering) is read from the real representation's 2 x 2 block
 // at rows 2p, 2p + 1 and columns 2q, 2q + 1 (the average of the two places
 // holding Re and of those holding Im, as the compression keeps them to eps)
-void complex_entries(Matrix* m, const std::vector<int>& rp, const std::vector<int>& cp, zd* out) {
+template <typename Z>
+void complex_entries(Matrix* m, const std::vector<int>& rp, const std::vector<int>& cp, Z* out) {
   if (m->factorization != -1)
     throw std::runtime_error("get_block/get_values: the factor of a complex matrix is held as the real "
                              "representation's LU, which has no complex entries");
@@ -650,9 +664,10 @@ void complex_entries(Matrix* m, const std::vector<int>& rp, const std::vector<in
     for (int i = 0; i < nr; ++i) {
       const double a = v[2 * i + 2 * j * ld], b = v[2 * i + 1 + 2 * j * ld];
       const double c = v[2 * i + (2 * j + 1) * ld], d = v[2 * i + 1 + (2 * j + 1) * ld];
-      out[i + (size_t)j * nr] = zd(0.5 * (a + d), 0.5 * (b - c));
+      out[i + (size_t)j * nr] = Z(0.5 * (a + d), 0.5 * (b - c));
     }
 }
+template <typename Z>
 int get_block_z(struct hmat_get_values_context_t* ctx) {
   return guard([&] {
     Matrix* m = MZ(ctx->matrix);
@@ -664,9 +679,9 @@ int get_block_z(struct hmat_get_values_context_t* ctx) {
     std::vector<int> rp(ctx->row_size), cp(ctx->col_size);
     for (int i = 0; i < ctx->row_size; ++i) rp[i] = ctx->row_offset + i;
     for (int j = 0; j < ctx->col_size; ++j) cp[j] = ctx->col_offset + j;
-    std::vector<zd> v((size_t)ctx->row_size * ctx->col_size);
+    std::vector<Z> v((size_t)ctx->row_size * ctx->col_size);
     complex_entries(m, rp, cp, v.data());
-    zd* out = static_cast<zd*>(ctx->values);
+    Z* out = static_cast<Z*>(ctx->values);
     if (ctx->renumber_rows) {
       if (ctx->row_offset != 0 || ctx->row_size != R.n) throw std::runtime_error("Cannot renumber");
       for (int j = 0; j < ctx->col_size; ++j)
@@ -679,6 +694,7 @@ int get_block_z(struct hmat_get_values_context_t* ctx) {
     ctx->col_indices = const_cast<int*>(Cl.indices.data()) + ctx->col_offset;
   });
 }
+template <typename Z>
 int get_values_z(struct hmat_get_values_context_t* ctx) {
   return guard([&] {
     Matrix* m = MZ(ctx->matrix);
@@ -695,13 +711,43 @@ int get_values_z(struct hmat_get_values_context_t* ctx) {
       if (u < 0 || u >= Cl.n) throw std::runtime_error("get_values: column index out of range");
       cp[j] = Cl.indices_rev[u];
     }
-    complex_entries(m, rp, cp, static_cast<zd*>(ctx->values));
+    complex_entries(m, rp, cp, static_cast<Z*>(ctx->values));
   });
 }
 int u_logdet_z(hmat_matrix_t*, double*) { return unsupported("logdet of a complex matrix"); }
 int u_diag_z(hmat_matrix_t*, void*, int) { return unsupported("extract_diagonal of a complex matrix"); }
 int u_tri_z(hmat_matrix_t*, int, void*, int) {
   return unsupported("solve_lower_triangular of a complex matrix (LLt only)");
+}
+
+// A complex table: the real table of the component type, with the entries
+// that see complex scalars, vectors or trees replaced
+template <int VT, typename T, typename Z, int ZT>
+void fill_complex(hmat_interface_t* i) {
+  fill<VT, T>(i);
+  i->value_type = (hmat_value_t)ZT;
+  i->create_empty_hmatrix_admissibility = create_empty_z<VT>;
+  i->factorize_generic = factorize_z;
+  i->destroy = destroy_z;
+  i->gemv = gemv_z<Z>;
+  i->gemm_scalar = gemm_scalar_z<Z>;
+  i->gemm_dense = gemm_dense_z<Z>;
+  i->scale = scale_z<Z>;
+  i->add_identity = add_identity_z<Z>;
+  i->axpy = axpy_z<Z>;
+  i->gemm = gemm_z<Z>;
+  i->vector_reorder = vector_order<Z, false>;
+  i->vector_restore = vector_order<Z, true>;
+  i->get_cluster_trees = get_cluster_trees_z;
+  i->logdet = u_logdet_z;
+  i->extract_diagonal = u_diag_z;
+  i->solve_lower_triangular = u_tri_z;
+  i->solve_lower_triangular_dense = u_tri_z;
+  i->get_block = get_block_z<Z>;
+  i->get_values = get_values_z<Z>;
+  // the stream holds the complex matrix (unfactorised); the engine reads it
+  // into the real representation
+  i->read_struct = read_struct<ZT>;
 }
 
 }  // namespace
@@ -713,42 +759,11 @@ void hmat_init_mi355x_interface(hmat_interface_t* i, hmat_value_t type) {
     case HMAT_SIMPLE_PRECISION: fill<HMAT_SIMPLE_PRECISION, float>(i); break;
     case HMAT_DOUBLE_PRECISION: fill<HMAT_DOUBLE_PRECISION, double>(i); break;
     case HMAT_DOUBLE_COMPLEX:
-      // the real FP64 table, with the entries that see complex scalars,
-      // vectors or trees replaced
-      fill<HMAT_DOUBLE_PRECISION, double>(i);
-      i->value_type = type;
-      i->create_empty_hmatrix_admissibility = create_empty_z;
-      i->factorize_generic = factorize_z;
-      i->destroy = destroy_z;
-      i->gemv = gemv_z;
-      i->gemm_scalar = gemm_scalar_z;
-      i->gemm_dense = gemm_dense_z;
-      i->scale = scale_z;
-      i->add_identity = add_identity_z;
-      i->axpy = axpy_z;
-      i->gemm = gemm_z;
-      i->vector_reorder = vector_order<zd, false>;
-      i->vector_restore = vector_order<zd, true>;
-      i->get_cluster_trees = get_cluster_trees_z;
-      i->logdet = u_logdet_z;
-      i->extract_diagonal = u_diag_z;
-      i->solve_lower_triangular = u_tri_z;
-      i->solve_lower_triangular_dense = u_tri_z;
-      i->get_block = get_block_z;
-      i->get_values = get_values_z;
-      // the stream holds the complex matrix (unfactorised); the engine reads
-      // it into the real representation
-      i->read_struct = read_struct<HMAT_DOUBLE_COMPLEX>;
+      fill_complex<HMAT_DOUBLE_PRECISION, double, zd, HMAT_DOUBLE_COMPLEX>(i);
       break;
-    default:
-      // single-precision complex: the table is filled, creation reports the limitation
-      fill<HMAT_DOUBLE_PRECISION, double>(i);
-      i->value_type = type;
-      i->create_empty_hmatrix_admissibility = [](const hmat_cluster_tree_t*, const hmat_cluster_tree_t*, int,
-                                                 hmat_admissibility_t*) -> hmat_matrix_t* {
-        unsupported("single-precision complex arithmetic");
-        return nullptr;
-      };
+    case HMAT_SIMPLE_COMPLEX:
+      fill_complex<HMAT_SIMPLE_PRECISION, float, zc, HMAT_SIMPLE_COMPLEX>(i);
+      break;
   }
 }
 

@@ -65,14 +65,14 @@ Matrix::Matrix(const ClusterTree* r, const ClusterTree* c, bool sym, std::unique
 
 Matrix* Matrix::create_complex(const ClusterTree* r, const ClusterTree* c,
                                const AdmissibilityParams& adm, int rank_cap,
-                               std::shared_ptr<ClusterTree> er, std::shared_ptr<ClusterTree> ec) {
+                               std::shared_ptr<ClusterTree> er, std::shared_ptr<ClusterTree> ec,
+                               int vt) {
   if (!er) er = std::shared_ptr<ClusterTree>(embed_cluster_tree(*r));
   if (!ec) ec = c == r ? er : std::shared_ptr<ClusterTree>(embed_cluster_tree(*c));
   auto zt = build_block_tree(*r, *c, adm, false);
   auto et = embed_block_tree(*zt, *r, *er, *c, *ec);
   // a complex rank k is a real rank 2k in the embedding
-  auto* m = new Matrix(er.get(), ec.get(), false, std::move(et), HMAT_DOUBLE_PRECISION,
-                       2 * rank_cap, {});
+  auto* m = new Matrix(er.get(), ec.get(), false, std::move(et), vt, 2 * rank_cap, {});
   m->adm = adm;
   m->embedded = true;
   m->zrows = r;

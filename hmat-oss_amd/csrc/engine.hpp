@@ -102,15 +102,17 @@ class Matrix {
   AdmissibilityParams adm;
   ~Matrix();
 
-  // A complex (Z) matrix over the caller's trees r x c: the engine stores the
-  // 2 x 2 real representation [[Re, -Im], [Im, Re]] of every entry over the
-  // embedded trees (embed_cluster_tree), so that every kernel runs in real
-  // FP64 arithmetic and a complex vector (interleaved re, im) is a real vector
-  // of the embedded numbering.  er / ec: embedded trees to share (clones).
+  // A complex (Z, or C with vt = HMAT_SIMPLE_PRECISION) matrix over the
+  // caller's trees r x c: the engine stores the 2 x 2 real representation
+  // [[Re, -Im], [Im, Re]] of every entry over the embedded trees
+  // (embed_cluster_tree), so that every kernel runs in real arithmetic of the
+  // type vt and a complex vector (interleaved re, im) is a real vector of the
+  // embedded numbering.  er / ec: embedded trees to share (clones).
   static Matrix* create_complex(const ClusterTree* r, const ClusterTree* c,
                                 const AdmissibilityParams& adm, int rank_cap,
                                 std::shared_ptr<ClusterTree> er = nullptr,
-                                std::shared_ptr<ClusterTree> ec = nullptr);
+                                std::shared_ptr<ClusterTree> ec = nullptr,
+                                int vt = HMAT_DOUBLE_PRECISION);
   bool embedded = false;                    // a complex matrix (see create_complex)
   const ClusterTree* zrows = nullptr;       // the caller's complex trees
   const ClusterTree* zcols = nullptr;

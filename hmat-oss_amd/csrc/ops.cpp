@@ -36,7 +36,7 @@ std::vector<int> leaves_where(const BlockTree& bt, bool diagonal_only) {
 
 Matrix* Matrix::clone(bool structure_only) {
   if (!assembled && !structure_only) throw std::runtime_error("copy: matrix not assembled");
-  auto* c = embedded ? create_complex(zrows, zcols, adm, rk_cap_max / 2, erows, ecols)
+  auto* c = embedded ? create_complex(zrows, zcols, adm, rk_cap_max / 2, erows, ecols, value_type)
                      : new Matrix(rows, cols, lower_sym, adm, value_type, rk_cap_max);
   try {
     c->lowrank_eps = lowrank_eps;

@@ -49,8 +49,9 @@ struct In {
 int type_code(int value_type) {
   if (value_type == HMAT_SIMPLE_PRECISION) return 0;
   if (value_type == HMAT_DOUBLE_PRECISION) return 1;
+  if (value_type == HMAT_SIMPLE_COMPLEX) return 2;
   if (value_type == HMAT_DOUBLE_COMPLEX) return 3;
-  throw std::runtime_error("serialization: single-precision complex matrices are not provided");
+  throw std::runtime_error("serialization: unknown scalar type");
 }
 
 // MatrixStructMarshaller::write(ClusterTree) serialization.cpp:34-53
@@ -151,7 +152,9 @@ struct Serializer {
     check_complex(M);
     Out o{f, ud};
     const int fact = written_factorization(M);
-    o.put<int>(type_code(M.embedded ? HMAT_DOUBLE_COMPLEX : M.value_type));
+    o.put<int>(type_code(!M.embedded ? M.value_type
+                                     : M.value_type == HMAT_SIMPLE_PRECISION ? HMAT_SIMPLE_COMPLEX
+                                                                             : HMAT_DOUBLE_COMPLEX));
     o.put<int>(fact);
     write_cluster_tree(o, M.embedded ? *M.zrows : *M.rows);
     write_cluster_tree(o, M.embedded ? *M.zcols : *M.cols);
@@ -231,10 +234,11 @@ struct Serializer {
     }
   }
 
-  // the complex leaves of an embedded matrix (see check_complex)
-  static void write_complex_leaves(Matrix& M, Out& o, const std::vector<int>& rk) {
-    const Store<double>& st = M.s64;
-    std::vector<double> buf, z;
+  // the complex leaves of an embedded matrix (see check_complex), components
+  // of type T (the store's)
+  template <typename T>
+  static void write_complex_leaves(Matrix& M, Out& o, const Store<T>& st, const std::vector<int>& rk) {
+    std::vector<T> buf, z;
     for (const Block* b : M.bt->leaves) {
       const LeafDesc& L = st.hdesc[b->leaf];
       const int m = L.m / 2, n = L.n / 2;
@@ -243,20 +247,20 @@ struct Serializer {
         o.put<int>(k);
         if (k > 0) {
           buf.resize((size_t)L.m * k);
-          HMX_CHECK(hipMemcpy(buf.data(), st.data() + L.a, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
-          z.assign(2 * (size_t)m * k, 0.0);
+          HMX_CHECK(hipMemcpy(buf.data(), st.data() + L.a, buf.size() * sizeof(T), hipMemcpyDeviceToHost));
+          z.assign(2 * (size_t)m * k, T(0));
           for (int l = 0; l < k; ++l)
             for (int u = 0; u < m; ++u) {
               z[2 * (u + (size_t)l * m)] = buf[2 * u + (size_t)l * L.m];
               z[2 * (u + (size_t)l * m) + 1] = buf[2 * u + 1 + (size_t)l * L.m];
             }
-          o.bytes(z.data(), z.size() * sizeof(double));
+          o.bytes(z.data(), z.size() * sizeof(T));
           buf.resize((size_t)L.n * k);
-          HMX_CHECK(hipMemcpy(buf.data(), st.data() + L.b, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
-          z.assign(2 * (size_t)n * k, 0.0);
+          HMX_CHECK(hipMemcpy(buf.data(), st.data() + L.b, buf.size() * sizeof(T), hipMemcpyDeviceToHost));
+          z.assign(2 * (size_t)n * k, T(0));
           for (int l = 0; l < k; ++l)
             for (int v = 0; v < n; ++v) z[2 * (v + (size_t)l * n)] = buf[2 * v + (size_t)l * L.n];
-          o.bytes(z.data(), z.size() * sizeof(double));
+          o.bytes(z.data(), z.size() * sizeof(T));
           o.put<int>(0);
           o.put<int>(0);
         }
@@ -264,15 +268,15 @@ struct Serializer {
       }
       o.put<int>(0);
       buf.resize((size_t)L.m * L.n);
-      HMX_CHECK(hipMemcpy(buf.data(), st.data() + L.a, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+      HMX_CHECK(hipMemcpy(buf.data(), st.data() + L.a, buf.size() * sizeof(T), hipMemcpyDeviceToHost));
       z.resize(2 * (size_t)m * n);
-      auto at = [&](int i, int j) { return buf[i + (size_t)j * L.m]; };
+      auto at = [&](int i, int j) { return (double)buf[i + (size_t)j * L.m]; };
       for (int v = 0; v < n; ++v)
         for (int u = 0; u < m; ++u) {
-          z[2 * (u + (size_t)v * m)] = 0.5 * (at(2 * u, 2 * v) + at(2 * u + 1, 2 * v + 1));
-          z[2 * (u + (size_t)v * m) + 1] = 0.5 * (at(2 * u + 1, 2 * v) - at(2 * u, 2 * v + 1));
+          z[2 * (u + (size_t)v * m)] = (T)(0.5 * (at(2 * u, 2 * v) + at(2 * u + 1, 2 * v + 1)));
+          z[2 * (u + (size_t)v * m) + 1] = (T)(0.5 * (at(2 * u + 1, 2 * v) - at(2 * u, 2 * v + 1)));
         }
-      o.bytes(z.data(), z.size() * sizeof(double));
+      o.bytes(z.data(), z.size() * sizeof(T));
     }
   }
 
@@ -281,8 +285,10 @@ struct Serializer {
     check_complex(M);
     Out o{f, ud};
     const std::vector<int> rk = ranks(M);
-    if (M.embedded)
-      write_complex_leaves(M, o, rk);
+    if (M.embedded && use32(M))
+      write_complex_leaves<float>(M, o, M.s32, rk);
+    else if (M.embedded)
+      write_complex_leaves<double>(M, o, M.s64, rk);
     else if (use32(M))
       write_leaves<float>(M, o, M.s32, rk);
     else
@@ -300,7 +306,8 @@ struct Serializer {
     if (fact != -1 && fact != hmat_factorization_lu && fact != hmat_factorization_llt &&
         fact != hmat_factorization_ldlt)
       throw std::runtime_error("read_struct: only unfactorised, LU, LDLt and LLt matrices are supported");
-    const bool complex = value_type == HMAT_DOUBLE_COMPLEX;
+    const bool complex = value_type == HMAT_DOUBLE_COMPLEX || value_type == HMAT_SIMPLE_COMPLEX;
+    const int real_type = value_type == HMAT_SIMPLE_COMPLEX ? HMAT_SIMPLE_PRECISION : HMAT_DOUBLE_PRECISION;
     if (complex && fact != -1)
       throw std::runtime_error("read_struct: a factorised complex matrix is not provided");
     std::unique_ptr<ClusterTree> rt = read_cluster_tree(in);
@@ -354,8 +361,7 @@ struct Serializer {
       std::shared_ptr<ClusterTree> ec = sym_tree ? er : std::shared_ptr<ClusterTree>(embed_cluster_tree(*cols));
       std::unique_ptr<BlockTree> et = embed_block_tree(*bt, *rows, *er, *cols, *ec);
       for (int& h : hint) h *= 2;
-      auto* M = new Matrix(er.get(), ec.get(), false, std::move(et), HMAT_DOUBLE_PRECISION,
-                           2 * rank_cap, hint);
+      auto* M = new Matrix(er.get(), ec.get(), false, std::move(et), real_type, 2 * rank_cap, hint);
       M->embedded = true;
       M->zrows = rows;
       M->zcols = cols;
@@ -438,16 +444,16 @@ struct Serializer {
   // rank k becomes the real Rk (R(A), R(conj B)) of rank 2k, R(X) the 2m x 2k
   // real matrix of the 2 x 2 blocks [[Re x, -Im x], [Im x, Re x]] (R(A B^T) =
   // R(A) R(conj B)^T); a full leaf Z becomes R(Z)
-  static void read_complex_leaves(Matrix& M, In& in) {
-    Store<double>& st = M.s64;
-    std::vector<double> image(st.elems, 0.0);
+  template <typename T>
+  static void read_complex_leaves(Matrix& M, In& in, Store<T>& st) {
+    std::vector<T> image(st.elems, T(0));
     std::vector<int> rk(st.hdesc.size());
-    std::vector<double> z;
-    auto embed = [](const double* z, int rows, int cols, bool conj, double* out, int ld) {
+    std::vector<T> z;
+    auto embed = [](const T* z, int rows, int cols, bool conj, T* out, int ld) {
       for (int l = 0; l < cols; ++l)
         for (int u = 0; u < rows; ++u) {
-          const double re = z[2 * (u + (size_t)l * rows)];
-          const double im = conj ? -z[2 * (u + (size_t)l * rows) + 1] : z[2 * (u + (size_t)l * rows) + 1];
+          const T re = z[2 * (u + (size_t)l * rows)];
+          const T im = conj ? -z[2 * (u + (size_t)l * rows) + 1] : z[2 * (u + (size_t)l * rows) + 1];
           out[2 * u + (size_t)(2 * l) * ld] = re;
           out[2 * u + 1 + (size_t)(2 * l) * ld] = im;
           out[2 * u + (size_t)(2 * l + 1) * ld] = -im;
@@ -465,10 +471,10 @@ struct Serializer {
         rk[b->leaf] = 2 * k;
         if (k > 0) {
           z.resize(2 * (size_t)m * k);
-          in.bytes(z.data(), z.size() * sizeof(double));
+          in.bytes(z.data(), z.size() * sizeof(T));
           embed(z.data(), m, k, false, image.data() + L.a, L.m);
           z.resize(2 * (size_t)n * k);
-          in.bytes(z.data(), z.size() * sizeof(double));
+          in.bytes(z.data(), z.size() * sizeof(T));
           embed(z.data(), n, k, true, image.data() + L.b, L.n);
           in.get<int>();
           in.get<int>();
@@ -479,12 +485,12 @@ struct Serializer {
       if (header & 1) continue;  // null full block
       if (header & 6) throw std::runtime_error("read_data: factor data in an unfactorised complex matrix");
       z.resize(2 * (size_t)m * n);
-      in.bytes(z.data(), z.size() * sizeof(double));
+      in.bytes(z.data(), z.size() * sizeof(T));
       embed(z.data(), m, n, false, image.data() + L.a, L.m);
     }
     in.end_marker();
     M.init_store(st);
-    HMX_CHECK(hipMemcpy(st.arena.p, image.data(), image.size() * sizeof(double), hipMemcpyHostToDevice));
+    HMX_CHECK(hipMemcpy(st.arena.p, image.data(), image.size() * sizeof(T), hipMemcpyHostToDevice));
     HMX_CHECK(hipMemcpy(st.ranks.p, rk.data(), rk.size() * sizeof(int), hipMemcpyHostToDevice));
     st.valid = true;
   }
@@ -492,8 +498,10 @@ struct Serializer {
   static void read_data(Matrix& M, hmat_iostream f, void* ud) {
     In in{f, ud};
     M.device_init();
-    if (M.embedded)
-      read_complex_leaves(M, in);
+    if (M.embedded && M.value_type == HMAT_SIMPLE_PRECISION)
+      read_complex_leaves<float>(M, in, M.s32);
+    else if (M.embedded)
+      read_complex_leaves<double>(M, in, M.s64);
     else if (M.value_type == HMAT_SIMPLE_PRECISION)
       read_leaves<float>(M, in, M.s32);
     else

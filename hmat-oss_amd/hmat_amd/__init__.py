@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libhmat_mi355x.so")
 
 HMAT_SIMPLE_PRECISION, HMAT_DOUBLE_PRECISION, HMAT_SIMPLE_COMPLEX, HMAT_DOUBLE_COMPLEX = 0, 1, 2, 3
 _NP_TYPE = {HMAT_SIMPLE_PRECISION: np.float32, HMAT_DOUBLE_PRECISION: np.float64,
-            HMAT_DOUBLE_COMPLEX: np.complex128}
+            HMAT_SIMPLE_COMPLEX: np.complex64, HMAT_DOUBLE_COMPLEX: np.complex128}
 FACT_NONE, FACT_LU, FACT_LDLT, FACT_LLT, FACT_HODLR, FACT_HODLRSYM = -1, 0, 1, 2, 3, 4
 KERNEL_EXPONENTIAL, KERNEL_GAUSSIAN, KERNEL_SHIFTED_EXPONENTIAL = 0, 1, 2
 # derivatives with respect to the correlation length (for the GP likelihood gradient)
@@ -316,7 +316,7 @@ class HMatrix:
 
     @property
     def is_complex(self):
-        return self.value_type == HMAT_DOUBLE_COMPLEX
+        return self.value_type in (HMAT_SIMPLE_COMPLEX, HMAT_DOUBLE_COMPLEX)
 
     @property
     def dtype(self):
@@ -457,6 +457,8 @@ class HMatrix:
         is component c of hmat DoF p), otherwise its nearest complex matrix
         (Re and Im averaged over the two places that hold them)."""
         if self.is_complex:
+            if self.value_type == HMAT_SIMPLE_COMPLEX:
+                precision = 1    # the single-precision store holds the matrix
             d = np.zeros((2 * self.n, 2 * self.n))
             for m, n, r0, c0, kind, rank, a, b in self.leaves(precision):
                 d[r0:r0 + m, c0:c0 + n] = a if kind == 0 else (a @ b.T if rank > 0 else 0.0)
@@ -503,7 +505,8 @@ class HMatrix:
     # ---- whole-matrix slots ----------------------------------------------------
     def _scalar(self, v):
         if self.is_complex:
-            return (C.c_double * 2)(complex(v).real, complex(v).imag)
+            ct = C.c_double if self.value_type == HMAT_DOUBLE_COMPLEX else C.c_float
+            return (ct * 2)(complex(v).real, complex(v).imag)
         return (C.c_double if self.value_type else C.c_float)(v)
 
     def copy(self, structure_only=False):
@@ -620,7 +623,8 @@ class HMatrix:
         m = object.__new__(HMatrix)
         m.tree = _BorrowedTree(r.value)
         m.iface, m.value_type, m.handle, m._kernel = iface, value_type, h, None
-        m.lower_symmetric = bool(r.value == c.value) and value_type != HMAT_DOUBLE_COMPLEX
+        m.lower_symmetric = bool(r.value == c.value) and value_type not in (HMAT_SIMPLE_COMPLEX,
+                                                                            HMAT_DOUBLE_COMPLEX)
         m._consumed = pos[0]
         return m
 

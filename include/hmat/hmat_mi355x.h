@@ -27,13 +27,13 @@ extern "C" {
 #endif
 
 /* Replaces hmat_init_default_interface (c_default_interface.cpp:242).
- * HMAT_DOUBLE_COMPLEX matrices are held as the 2 x 2 real representation
- * [[Re, -Im], [Im, Re]] of every entry (real FP64 kernels throughout); complex
- * arrays are passed as the reference passes them (interleaved re, im).  They
- * are stored whole and factorised by H-LU; logdet, extract_diagonal,
- * serialization and complex scalars with a nonzero imaginary part in scale /
- * add_identity / axpy / gemm are not provided for them.  HMAT_SIMPLE_COMPLEX
- * is not provided. */
+ * HMAT_DOUBLE_COMPLEX (HMAT_SIMPLE_COMPLEX) matrices are held as the 2 x 2
+ * real representation [[Re, -Im], [Im, Re]] of every entry (real FP64 (FP32)
+ * kernels throughout); complex arrays are passed as the reference passes them
+ * (interleaved re, im).  They are stored whole and factorised by H-LU;
+ * logdet, extract_diagonal, serialization of a factorised complex matrix and
+ * complex scalars with a nonzero imaginary part in scale / add_identity /
+ * axpy / gemm are not provided for them. */
 HMAT_API void hmat_init_mi355x_interface(hmat_interface_t *i, hmat_value_t type);
 
 typedef enum {

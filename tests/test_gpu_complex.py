@@ -335,3 +335,68 @@ def test_complex_get_block_and_get_values():
     with pytest.raises(hmat_amd.EngineError):
         h.get_block(0, 10, 0, 10)
     h.close()
+
+
+@pytest.mark.gpu
+def test_single_precision_complex():
+    """HMAT_SIMPLE_COMPLEX (C): the real representation in FP32 (assembled in
+    FP64, stored in FP32), against the exact kernel, the dense algebra of its
+    own stored matrix, and the FP64 complex matrix of the same partition;
+    get_values and the complex stream format (scalar type 2) included."""
+    import struct
+    n, leaf = 2000, 64
+    say(f"single-precision complex n={n}")
+    pts, hreg, k = problem(n, 0.8, 21)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    hz = hmat_amd.HMatrix(t, lower_symmetric=False, value_type=hmat_amd.HMAT_DOUBLE_COMPLEX)
+    hc = hmat_amd.HMatrix(t, lower_symmetric=False, value_type=hmat_amd.HMAT_SIMPLE_COMPLEX)
+    for h in (hz, hc):
+        h.assemble(kind=hmat_amd.KERNEL_HELMHOLTZ, wavenumber=k, reg=hreg, nugget=1.0, eps=EPS,
+                   method="aca_partial")
+    idx = t.indices
+    Hc, Hz = hc.to_dense(), hz.to_dense()
+    ker = ho.HelmholtzKernel(pts, k, hreg, 1.0)
+    exact = ker.block(idx, idx)
+    e_exact = np.linalg.norm(Hc - exact) / np.linalg.norm(exact)
+    e_z = np.linalg.norm(Hc - Hz) / np.linalg.norm(Hz)
+    say(f"C matrix vs exact kernel {e_exact:.2e}, vs the Z matrix {e_z:.2e}")
+    assert e_exact <= 2 * EPS and e_z <= 1e-6
+    assert np.array_equal(hc.partition(), hz.partition())
+    rng = np.random.default_rng(8)
+    x = (rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))).astype(np.complex64)
+    D, Du = embedded_user(hc, idx)
+    ax = complex_view(Du @ real_view(x.astype(np.complex128)))
+    y = hc.gemv(x)
+    assert y.dtype == np.complex64
+    g = np.linalg.norm(y - ax) / np.linalg.norm(ax)
+    y2 = hc.gemv(x, alpha=0.5 - 1.5j)
+    g2 = np.linalg.norm(y2 - (0.5 - 1.5j) * ax) / np.linalg.norm(ax)
+    ur, uc = rng.choice(n, 23, replace=False), rng.choice(n, 31, replace=False)
+    inv = np.empty(n, dtype=np.int64)
+    inv[idx] = np.arange(n)
+    V = hc.get_values(ur, uc)
+    gv = np.abs(V - Hc[np.ix_(inv[ur], inv[uc])]).max() / np.abs(Hc).max()
+    say(f"gemv vs dense product of the stored matrix {g:.2e}, complex alpha {g2:.2e}; "
+        f"get_values {gv:.2e}")
+    assert g <= 1e-5 and g2 <= 1e-5 and gv <= 1e-6
+    # the stream: scalar type 2, complex<float> leaves; read back
+    s, d = hc.write_struct(), hc.write_data()
+    assert struct.unpack_from("<ii", s) == (2, -1)
+    r = hmat_amd.HMatrix.read_struct(s, hmat_amd.HMAT_SIMPLE_COMPLEX)
+    assert r.read_data(d) == len(d)
+    back = np.linalg.norm(r.to_dense() - Hc) / np.linalg.norm(Hc)
+    say(f"read-back C matrix vs the original {back:.2e}")
+    # the stream keeps the first column of each 2 x 2 block of the Rk leaves,
+    # which the compression keeps equal to the other to eps (to_dense averages)
+    assert back <= EPS
+    r.close()
+    # H-LU in FP32 and its solve
+    hc.factorize(hmat_amd.FACT_LU)
+    b = (rng.standard_normal((n, 4)) + 1j * rng.standard_normal((n, 4))).astype(np.complex64)
+    xs = hc.solve(b)
+    xe = complex_view(np.linalg.solve(Du, real_view(b.astype(np.complex128))))
+    rel = np.linalg.norm(xs - xe) / np.linalg.norm(xe)
+    say(f"FP32 H-LU solve vs dense solve of the stored matrix {rel:.2e}")
+    assert rel <= 5e-3
+    hc.close()
+    hz.close()
