@@ -662,6 +662,12 @@ void Matrix::assemble_host_pass(const HostFunctions& f, const CompressionSpec& c
 // plan serves every nrhs and only the scratch buffers depend on it.
 void Matrix::build_gemv_plan() {
   GemvPlan& P = plan_;
+  P.max_rows = 0;
+  // the y pass takes destination leaf clusters of at most 512 rows: a matrix
+  // with wider ones has no H-gemv plan (its gemv reports it, the rest works)
+  P.built = false;
+  for (const Cluster* c : rows->leaves)
+    if (c->size > 512) return;
   const int nrhs = 1;
   const auto& D = s64.hdesc;
   const int nleaf = (int)D.size();
@@ -718,7 +724,7 @@ void Matrix::build_gemv_plan() {
   std::vector<YContrib> contrib;
   std::vector<int64_t> slist;
   for (size_t q = 0; q < RL.size(); ++q) {
-    if (RL[q]->size > 64 * 4) throw std::runtime_error("gemv: leaf clusters above 256 rows");
+    P.max_rows = std::max(P.max_rows, RL[q]->size);
     YItem it{RL[q]->offset, RL[q]->size, (int)contrib.size(), 0, (int)slist.size(), 0};
     // full leaves first, then the Rk slices, which the y pass stages in batches
     std::stable_partition(per[q].begin(), per[q].end(), [](const YContrib& c) { return c.kind == 0; });
@@ -761,7 +767,8 @@ void Matrix::gemv_launch(double alpha, const T* x, double beta, T* y, int nrhs, 
   launch_gemv_zreduce(P.zred.as<ZReduce>(), P.nzred, ranks, zpart, P.zitems.as<ZItem>(), nrhs,
                       desc, z, stream);
   launch_gemv_ypass<T>(desc, ranks, st.data(), P.yitems.as<YItem>(), P.ny,
-                       P.contrib.as<YContrib>(), z, slots, x, ldx, nrhs, acc, rows->n, stream);
+                       P.contrib.as<YContrib>(), z, slots, x, ldx, nrhs, acc, rows->n, P.max_rows,
+                       stream);
   launch_gemv_yfinal<T>(P.yitems.as<YItem>(), P.ny, P.slotlist.as<int64_t>(), slots, acc,
                         rows->n, y, ldy, nrhs, alpha, beta, stream);
 }
@@ -772,6 +779,7 @@ void Matrix::gemv_device(double alpha, const T* x, double beta, T* y, int nrhs, 
   if (!st.valid) throw std::runtime_error("gemv: matrix data not available in this precision");
   if (nrhs < 1) throw std::runtime_error("gemv: nrhs must be positive");
   if (!plan_.built) build_gemv_plan();
+  if (!plan_.built) throw std::runtime_error("gemv: leaf clusters above 512 rows are not provided");
   GemvPlan& P = plan_;
   if (P.nrhs < nrhs) {
     HMX_CHECK(hipStreamSynchronize(stream));
@@ -789,7 +797,8 @@ template <typename T>
 void Matrix::gemv_device_scratch(double alpha, const T* x, double beta, T* y, int nrhs,
                                  const Store<T>& st, double* scratch, size_t scratch_elems) {
   if (!st.valid) throw std::runtime_error("gemv: matrix data not available in this precision");
-  if (!plan_.built) throw std::runtime_error("gemv: no H-gemv plan (matrix not assembled)");
+  if (!plan_.built)
+    throw std::runtime_error("gemv: no H-gemv plan (matrix not assembled, or leaf clusters above 512 rows)");
   if (nrhs < 1) throw std::runtime_error("gemv: nrhs must be positive");
   const size_t need = gemv_scratch_elems(nrhs);
   if (need > scratch_elems)

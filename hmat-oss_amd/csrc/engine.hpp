@@ -80,6 +80,7 @@ struct GemvPlan {
   int nrhs = 0;       // scratch buffers below hold this many right-hand sides
   DevBuf zitems, zred, yitems, contrib, slotlist;
   int nz = 0, nzred = 0, ny = 0;
+  int max_rows = 0;   // rows of the largest destination leaf cluster
   int64_t zpart1 = 0, z1 = 0, slot1 = 0;  // scratch doubles per right-hand side
   DevBuf zpart, z, slots, acc;
 };

@@ -127,11 +127,11 @@ __global__ void k_zreduce(const ZReduce* red, const int* ranks, const double* zp
   }
 }
 
-// y pass.  One workgroup per destination leaf cluster I (at most 256 rows).
+// y pass.  One workgroup per destination leaf cluster I (at most 128 kQ rows).
 // The cluster's contribution list is first described in LDS (one parallel
 // load of its records), so the loops below issue data loads only.
 //  * Full leaves (I x J) are staged in LDS in column chunks with x_J; threads
-//    < 128 form M x_J (rows t, t + 128), and for a mirrored leaf threads >= 128
+//    < 128 form M x_J (rows t + 128 q), and for a mirrored leaf threads >= 128
 //    form M^T x_I column by column into the leaf's slot.  No cross-lane
 //    reductions.
 //  * Rk slices (A z, or B z' of a mirrored Rk leaf) are taken four at a time,
@@ -145,15 +145,17 @@ template <int kR>
 constexpr int y_tile() { return kR == 1 ? 3840 : 5120; }
 constexpr int kYMeta = 128;    // contribution records described in LDS at a time
 
-template <typename T, int kR>
+template <typename T, int kR, int kQ>
 __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const int* ranks,
                                                   const T* arena, const YItem* items,
                                                   const YContrib* contrib, const double* z,
                                                   double* slots, const T* x, int ldx, int nrhs,
                                                   double* acc_out, int ldacc) {
   constexpr int kYTile = y_tile<kR>();
+  constexpr int kRows = 128 * kQ;  // rows of a destination leaf cluster (at most)
+  static_assert(5 * kRows <= kYTile, "the partial sums are gathered in the leaf tile");
   __shared__ double tile[kYTile];
-  __shared__ double xI[256 * kR];  // x_I, x_J: 256 rows per right-hand side
+  __shared__ double xI[kRows * kR];  // x_I: kRows rows, x_J: 256 columns per right-hand side
   __shared__ double xJ[256 * kR];
   __shared__ const T* mp[kYMeta];      // full: leaf data; rk: first row of the slice
   __shared__ const double* mz[kYMeta]; // rk: z; full mirrored: slot (as double*)
@@ -167,15 +169,16 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
     __syncthreads();
     for (int e = tid; e < it.len * kR; e += kBlock) {
       const int i = e % it.len, r = e / it.len;
-      xI[i + r * 256] = r < nr ? (double)x[it.r0 + i + (size_t)(rc + r) * ldx] : 0.0;
+      xI[i + r * kRows] = r < nr ? (double)x[it.r0 + i + (size_t)(rc + r) * ldx] : 0.0;
     }
-    double accf[2][kR];  // full leaves: rows h, h + 128 (threads < 128)
-    double accr[4][kR];  // Rk slices: rows lane + 64 q
+    double accf[kQ][kR];      // full leaves: rows h + 128 q (threads < 128)
+    double accr[2 * kQ][kR];  // Rk slices: rows lane + 64 q
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
-      accf[0][r] = accf[1][r] = 0.0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) accr[q][r] = 0.0;
+      for (int q = 0; q < kQ; ++q) accf[q][r] = 0.0;
+#pragma unroll
+      for (int q = 0; q < 2 * kQ; ++q) accr[q][r] = 0.0;
     }
     for (int cb = it.beg; cb < it.end; cb += kYMeta) {
       const int nc = it.end - cb < kYMeta ? it.end - cb : kYMeta;
@@ -252,7 +255,7 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
           if (kPrefetch && j0 + cw_max < n) load_chunk(j0 + cw_max);
           if (lo) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < kQ; ++q) {
               const int i = h + 128 * q;
               if (i >= m) continue;
               const double* row = tile + i;
@@ -273,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
               for (int i = 0; i < m; ++i) {
                 const double v = col[i];
 #pragma unroll
-                for (int r = 0; r < kR; ++r) t[r] += v * xI[i + r * 256];
+                for (int r = 0; r < kR; ++r) t[r] += v * xI[i + r * kRows];
               }
 #pragma unroll
               for (int r = 0; r < kR; ++r)
@@ -297,7 +300,7 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
             for (int r = 0; r < kR; ++r) zl[r] = r < nr ? zz[l + (size_t)r * cap] : 0.0;
             const T* pl = P + (size_t)l * ld;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < 2 * kQ; ++q) {
               const int i = lane + 64 * q;
               if (i < it.len) {
                 const double v = (double)pl[i];
@@ -328,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
           const T* pa = P0 + (size_t)(l < k0 ? l : 0) * ld0;
           const T* pb = P1 + (size_t)(l < k1 ? l : 0) * ld1;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < 2 * kQ; ++q) {
             const int i = lane + 64 * q;
             if (i < it.len) {
               const double va = l < k0 ? (double)pa[i] : 0.0;
@@ -345,16 +348,17 @@ __global__ void __launch_bounds__(kBlock) k_ypass(const LeafDesc* desc, const in
     for (int r = 0; r < kR; ++r) {
       __syncthreads();
       if (lo) {
-        tile[4 * 256 + h] = accf[0][r];
-        tile[4 * 256 + h + 128] = accf[1][r];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) tile[4 * kRows + h + 128 * q] = accf[q][r];
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) tile[wv * 256 + lane + 64 * q] = accr[q][r];
+      for (int q = 0; q < 2 * kQ; ++q) tile[wv * kRows + lane + 64 * q] = accr[q][r];
       __syncthreads();
       if (r < nr)
         for (int i = tid; i < it.len; i += kBlock)
           acc_out[it.r0 + i + (size_t)(rc + r) * ldacc] =
-              tile[4 * 256 + i] + ((tile[i] + tile[256 + i]) + (tile[512 + i] + tile[768 + i]));
+              tile[4 * kRows + i] +
+              ((tile[i] + tile[kRows + i]) + (tile[2 * kRows + i] + tile[3 * kRows + i]));
     }
   }
 }
@@ -503,14 +507,27 @@ void launch_gemv_zreduce(const ZReduce* red, int count, const int* ranks, const 
 template <typename T>
 void launch_gemv_ypass(const LeafDesc* desc, const int* ranks, const T* arena, const YItem* items,
                        int count, const YContrib* contrib, const double* z, double* slots,
-                       const T* x, int ldx, int nrhs, double* acc, int ldacc, hipStream_t s) {
+                       const T* x, int ldx, int nrhs, double* acc, int ldacc, int max_rows,
+                       hipStream_t s) {
   if (count == 0) return;
-  if (nrhs == 1)
-    hipLaunchKernelGGL((k_ypass<T, 1>), dim3(count), dim3(kBlock), 0, s, desc, ranks, arena, items,
-                       contrib, z, slots, x, ldx, nrhs, acc, ldacc);
-  else
-    hipLaunchKernelGGL((k_ypass<T, kRhsChunk>), dim3(count), dim3(kBlock), 0, s, desc, ranks, arena,
-                       items, contrib, z, slots, x, ldx, nrhs, acc, ldacc);
+  if (max_rows > 512) throw std::runtime_error("gemv: leaf clusters above 512 rows");
+  // destination leaf clusters of at most 256 rows (the usual case) or 512
+  // (twice the registers and the x_I staging, fewer workgroups per CU)
+  if (max_rows <= 256) {
+    if (nrhs == 1)
+      hipLaunchKernelGGL((k_ypass<T, 1, 2>), dim3(count), dim3(kBlock), 0, s, desc, ranks, arena,
+                         items, contrib, z, slots, x, ldx, nrhs, acc, ldacc);
+    else
+      hipLaunchKernelGGL((k_ypass<T, kRhsChunk, 2>), dim3(count), dim3(kBlock), 0, s, desc, ranks,
+                         arena, items, contrib, z, slots, x, ldx, nrhs, acc, ldacc);
+  } else {
+    if (nrhs == 1)
+      hipLaunchKernelGGL((k_ypass<T, 1, 4>), dim3(count), dim3(kBlock), 0, s, desc, ranks, arena,
+                         items, contrib, z, slots, x, ldx, nrhs, acc, ldacc);
+    else
+      hipLaunchKernelGGL((k_ypass<T, kRhsChunk, 4>), dim3(count), dim3(kBlock), 0, s, desc, ranks,
+                         arena, items, contrib, z, slots, x, ldx, nrhs, acc, ldacc);
+  }
   HMX_CHECK(hipGetLastError());
 }
 
@@ -529,7 +546,7 @@ void launch_gemv_yfinal(const YItem* items, int count, const int64_t* slot_list,
                                      const T*, int, int, double*, hipStream_t);                 \
   template void launch_gemv_ypass<T>(const LeafDesc*, const int*, const T*, const YItem*, int, \
                                      const YContrib*, const double*, double*, const T*, int,   \
-                                     int, double*, int, hipStream_t);                           \
+                                     int, double*, int, int, hipStream_t);                      \
   template void launch_gemv_yfinal<T>(const YItem*, int, const int64_t*, const double*,        \
                                       const double*, int, T*, int, int, double, double,         \
                                       hipStream_t);

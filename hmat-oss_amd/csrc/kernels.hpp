@@ -139,9 +139,11 @@ void launch_gemv_zreduce(const ZReduce* red, int count, const int* ranks, const 
                          const ZItem* items, int nrhs, const LeafDesc* desc, double* z,
                          hipStream_t s);
 template <typename T>
+// (max_rows: the largest destination leaf cluster, at most 512 rows)
 void launch_gemv_ypass(const LeafDesc* desc, const int* ranks, const T* arena, const YItem* items,
                        int count, const YContrib* contrib, const double* z, double* slots,
-                       const T* x, int ldx, int nrhs, double* acc, int ldacc, hipStream_t s);
+                       const T* x, int ldx, int nrhs, double* acc, int ldacc, int max_rows,
+                       hipStream_t s);
 template <typename T>
 void launch_gemv_yfinal(const YItem* items, int count, const int64_t* slot_list,
                         const double* slots, const double* acc, int ldacc, T* y, int ldy,

@@ -152,7 +152,8 @@ def test_svd_compression_every_size():
 # chunks and the 476-row remainders of the 1500-row panels
 @pytest.mark.gpu
 @pytest.mark.parametrize("nrhs,n,height,leaf", [(1, 4000, 1.6, 128), (3, 4000, 1.6, 128),
-                                                (9, 4000, 1.6, 128), (1, 6000, 8.0, 64)])
+                                                (9, 4000, 1.6, 128), (1, 6000, 8.0, 64),
+                                                (1, 4000, 1.6, 400), (9, 4000, 1.6, 400)])
 def test_gemv_device_matches_dense(nrhs, n, height, leaf):
     import torch
     say(f"gemv device nrhs={nrhs} n={n} leaf={leaf}")
@@ -238,3 +239,54 @@ def test_rank_capacity_grows_during_assembly():
     assert ranks.max() > 4 and same and diff <= 1e-14
     small.close()
     ref.close()
+
+
+@pytest.mark.gpu
+def test_gemv_leaf_clusters_up_to_512_rows():
+    """Leaf clusters above 256 rows take the y pass with 512-row destination
+    items: non-symmetric storage with leaf 512 and a complex matrix with leaf
+    256 (512 rows of its real representation), against the dense product of
+    the stored matrix; 625-row clusters are refused."""
+    say("gemv with leaf clusters up to 512 rows")
+    pts = ho.cylinder_points(5000, 1.0, 2.0, 3)
+    t = hmat_amd.ClusterTree(pts, 512)
+    h = hmat_amd.HMatrix(t, lower_symmetric=False)
+    h.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    assert max(int(r[0]) for r in h.leaf_table() if r[4] == 0) > 256
+    D = h.to_dense()
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((h.n, 9))
+    idx = t.indices
+    ref = np.empty_like(x)
+    ref[idx] = D @ x[idx]
+    e1 = np.linalg.norm(h.gemv(x) - ref) / np.linalg.norm(ref)
+    e2 = np.linalg.norm(h.gemv(x[:, :1]) - ref[:, :1]) / np.linalg.norm(ref[:, :1])
+    h.close()
+    from scipy.spatial import cKDTree
+    dd, _ = cKDTree(pts).query(pts, 2)
+    hreg = float(dd[:, 1].mean())
+    tz = hmat_amd.ClusterTree(pts, 256)
+    hz = hmat_amd.HMatrix(tz, lower_symmetric=False, value_type=hmat_amd.HMAT_DOUBLE_COMPLEX)
+    hz.assemble(kind=hmat_amd.KERNEL_HELMHOLTZ, wavenumber=2 * np.pi / (10 * hreg), reg=hreg,
+                nugget=1.0, eps=EPS, method="aca_partial")
+    Dz = hz.to_dense(embedded=True)
+    xz = rng.standard_normal((hz.n, 2)) + 1j * rng.standard_normal((hz.n, 2))
+    e = np.empty(2 * hz.n, dtype=np.int64)
+    e[0::2], e[1::2] = 2 * tz.indices, 2 * tz.indices + 1
+    xr = np.empty((2 * hz.n, 2))
+    xr[0::2], xr[1::2] = xz.real, xz.imag
+    yr = np.empty_like(xr)
+    yr[e] = Dz @ xr[e]
+    refz = yr[0::2] + 1j * yr[1::2]
+    e3 = np.linalg.norm(hz.gemv(xz) - refz) / np.linalg.norm(refz)
+    hz.close()
+    say(f"leaf 512 non-symmetric: 9 RHS {e1:.2e}, 1 RHS {e2:.2e}; complex leaf 256: {e3:.2e}")
+    assert e1 <= 1e-13 and e2 <= 1e-13 and e3 <= 1e-13
+    # 625-row leaf clusters (5000 points halved three times, leaf 700): the
+    # matrix assembles, its H-gemv is refused
+    t6 = hmat_amd.ClusterTree(pts, 700)
+    h6 = hmat_amd.HMatrix(t6, lower_symmetric=False)
+    h6.assemble(length=0.5, nugget=1e-2, eps=EPS)
+    with pytest.raises(hmat_amd.EngineError, match="512"):
+        h6.gemv(x)
+    h6.close()
