@@ -207,6 +207,11 @@ int solve_generic(hmat_matrix_t* h, const struct hmat_solve_context_t* ctx) {
 }
 
 // solve_lower_triangular(_dense) c_wrapping.hpp:793-820
+// transpose c_wrapping.hpp:574-583 (HMatrix::transpose)
+int transpose(hmat_matrix_t* h) {
+  return guard([&] { M(h)->transpose(); });
+}
+
 int solve_lower_triangular(hmat_matrix_t* h, int transpose, void* b, int nrhs) {
   return guard([&] { M(h)->solve_host(b, nrhs, true, transpose ? 2 : 1); });
 }
@@ -471,7 +476,7 @@ void fill(hmat_interface_t* i) {
   i->solve_mat = solve_mat_h;
   i->solve_systems = solve_systems;
   i->solve_dense = solve_dense;
-  i->transpose = u_h;
+  i->transpose = transpose;
   i->scale = scale_matrix<T>;
   i->truncate = truncate_matrix;
   i->vector_reorder = vector_order<T, false>;

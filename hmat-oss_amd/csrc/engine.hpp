@@ -237,6 +237,7 @@ class Matrix {
 
   // whole-matrix operations (ops.cpp; truncate_all / axpy_from in factor.cpp)
   Matrix* clone(bool structure_only);       // same trees and partition; null leaves if asked
+  void transpose();                         // A <- A^T (unfactorised, one cluster tree)
   double norm();                            // Frobenius norm of the whole matrix
   void scale(double alpha);                 // A <- alpha A
   void add_identity(double alpha);          // A <- A + alpha I

@@ -149,6 +149,28 @@ __global__ void k_pick_scale(const double* V, int ldv, int rows, int first, int 
   }
 }
 
+// HMatrix::transpose: one workgroup per item.  kind 0: dst (m x n, ld m) =
+// src^T (src n x m, ld n), entries whose row and column parities differ
+// negated when conj (the real representation of a complex transpose); kind
+// 1: rows 2i + 1 of the m x n panel at dst negated in place.
+template <typename T>
+__global__ void k_leaf_transpose(const TransposeItem* items, T* arena, const T* copy) {
+  const TransposeItem it = items[blockIdx.x];
+  T* dst = arena + it.dst;
+  const size_t total = (size_t)it.m * it.n;
+  if (it.kind == 0) {
+    const T* src = copy + it.src;
+    for (size_t e = threadIdx.x; e < total; e += blockDim.x) {
+      const int i = (int)(e % it.m), j = (int)(e / it.m);
+      const T v = src[j + (size_t)i * it.n];
+      dst[e] = (it.conj && ((i ^ j) & 1)) ? -v : v;
+    }
+  } else {
+    for (size_t e = threadIdx.x; e < total; e += blockDim.x)
+      if ((e % it.m) & 1) dst[e] = -dst[e];
+  }
+}
+
 unsigned grid_for(size_t n) {
   const size_t b = (n + 255) / 256;
   return (unsigned)(b < 65536 ? (b ? b : 1) : 65536);
@@ -159,6 +181,16 @@ void launch_gauss(double* out, size_t n, uint64_t seed, hipStream_t s) {
   hipLaunchKernelGGL(k_gauss, dim3(grid_for(n)), dim3(256), 0, s, out, n, seed);
   HMX_CHECK(hipGetLastError());
 }
+
+template <typename T>
+void launch_leaf_transpose(const TransposeItem* items, int count, T* arena, const T* copy,
+                           hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_leaf_transpose<T>, dim3(count), dim3(256), 0, s, items, arena, copy);
+  HMX_CHECK(hipGetLastError());
+}
+template void launch_leaf_transpose<double>(const TransposeItem*, int, double*, const double*, hipStream_t);
+template void launch_leaf_transpose<float>(const TransposeItem*, int, float*, const float*, hipStream_t);
 
 void launch_pick_scale(const double* V, int ldv, int rows, int first, int count,
                        const double* lam, double e, double* out, hipStream_t s) {

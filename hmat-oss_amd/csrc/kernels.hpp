@@ -113,6 +113,20 @@ template <typename T>
 void launch_leaf_get_diag(const LeafDesc* desc, const int* list, int count, const T* arena,
                           double* out, hipStream_t s);
 
+// Transposition of the leaves (Matrix::transpose): kind 0, a full leaf
+// dst (m x n) = the partner's block at src (n x m) transposed, read from a
+// copy of the arena (with conj, the real representation of a complex
+// transpose: entries of differing row / column parity negated); kind 1, the
+// odd rows of the m x n panel at dst negated.
+struct TransposeItem {
+  int64_t dst, src;
+  int m, n;
+  int kind, conj;
+};
+template <typename T>
+void launch_leaf_transpose(const TransposeItem* items, int count, T* arena, const T* copy,
+                           hipStream_t s);
+
 // Whole-device compression of very large blocks (giant.cpp): seeded standard
 // normal numbers; the last `count` columns of V (ld ldv, eigenvalues lam
 // ascending), largest first, column j scaled by lam[j]^e, into out (ld rows).

@@ -2,12 +2,15 @@
 // and copy_struct (HMatInterface::copy, c_wrapping.hpp:98-108), norm
 // (HMatrix::normSqr h_matrix.cpp:624-646: off-diagonal blocks of symmetric
 // storage count twice), scale (c_wrapping.hpp:431), add_identity
-// (c_wrapping.hpp:393), extract_diagonal (c_wrapping.hpp:760-773) and the
-// entries of get_block / get_values (c_wrapping.hpp:832-890).
+// (c_wrapping.hpp:393), extract_diagonal (c_wrapping.hpp:760-773), transpose
+// (c_wrapping.hpp:574-583) and the entries of get_block / get_values
+// (c_wrapping.hpp:832-890).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <stdexcept>
+#include <type_traits>
 #include <utility>
 
 #include "engine.hpp"
@@ -90,6 +93,83 @@ Matrix* Matrix::clone(bool structure_only) {
     throw;
   }
   return c;
+}
+
+// HMatrix::transpose h_matrix.cpp:2036-2039 (transposeData + transposeMeta)
+// for a square matrix over one cluster tree: the admissibility is symmetric,
+// so leaf (I, J) has the partner (J, I) of the same kind.  An Rk leaf takes
+// its partner's panels exchanged, (A B^T)^T = B A^T (descriptor offsets,
+// capacities and ranks swap; no data moves), a full leaf its partner's block
+// transposed.  Lower-symmetric storage is its own transpose.  A complex
+// matrix gets its transpose (not the conjugate transpose): in the real
+// representation the real transpose followed by the conjugation P R P,
+// P = diag(1, -1, 1, -1, ...), i.e. odd rows of both Rk panels negated.
+void Matrix::transpose() {
+  if (!assembled) throw std::runtime_error("transpose: matrix not assembled");
+  if (factorization != -1) throw std::runtime_error("transpose: factorised matrices are not provided");
+  if (lower_sym) return;
+  if (rows != cols) throw std::runtime_error("transpose: only square matrices over one cluster tree");
+  std::map<std::pair<const Cluster*, const Cluster*>, int> at;
+  for (const Block* b : bt->leaves) at[{b->rows, b->cols}] = b->leaf;
+  std::vector<int> partner(bt->leaves.size());
+  for (const Block* b : bt->leaves) {
+    auto it = at.find({b->cols, b->rows});
+    if (it == at.end() || bt->leaves[it->second]->kind != b->kind)
+      throw std::runtime_error("transpose: the partition is not symmetric");
+    partner[b->leaf] = it->second;
+  }
+  HMX_CHECK(hipStreamSynchronize(stream));
+  auto run = [&](auto* tag, auto& st) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    if (!st.valid) return;
+    std::vector<int> rk(st.hdesc.size());
+    HMX_CHECK(hipMemcpy(rk.data(), st.ranks.p, rk.size() * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<LeafDesc> nd = st.hdesc;
+    std::vector<int> nrk = rk;
+    std::vector<TransposeItem> items;
+    for (size_t l = 0; l < nd.size(); ++l) {
+      const LeafDesc& P = st.hdesc[partner[l]];
+      LeafDesc& L = nd[l];
+      if (L.kind == 0) {
+        items.push_back(TransposeItem{L.a, P.a, L.m, L.n, 0, embedded ? 1 : 0});
+      } else {
+        L.a = P.b;
+        L.b = P.a;
+        L.cap = P.cap;
+        nrk[l] = rk[partner[l]];
+      }
+    }
+    if (embedded)
+      for (size_t l = 0; l < nd.size(); ++l)
+        if (nd[l].kind == 1 && nrk[l] > 0) {
+          items.push_back(TransposeItem{nd[l].a, 0, nd[l].m, nrk[l], 1, 1});
+          items.push_back(TransposeItem{nd[l].b, 0, nd[l].n, nrk[l], 1, 1});
+        }
+    DevBuf copy, ditems;
+    copy.alloc(st.elems * sizeof(T));
+    HMX_CHECK(hipMemcpyAsync(copy.p, st.arena.p, st.elems * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    upload(ditems, items.data(), items.size() * sizeof(TransposeItem), stream);
+    launch_leaf_transpose<T>(ditems.as<TransposeItem>(), (int)items.size(), st.data(),
+                             copy.as<T>(), stream);
+    st.hdesc = nd;
+    upload(st.desc, nd.data(), nd.size() * sizeof(LeafDesc), stream);
+    HMX_CHECK(hipMemcpyAsync(st.ranks.p, nrk.data(), nrk.size() * sizeof(int), hipMemcpyHostToDevice,
+                             stream));
+    HMX_CHECK(hipStreamSynchronize(stream));
+  };
+  run((double*)nullptr, s64);
+  run((float*)nullptr, s32);
+  // the other store's layout follows (both share one arena layout)
+  if (s64.valid && !s32.valid) s32.hdesc = s64.hdesc;
+  if (s32.valid && !s64.valid) s64.hdesc = s32.hdesc;
+  if (!cap_hint_.empty()) {
+    std::vector<int> h(cap_hint_.size());
+    for (size_t l = 0; l < h.size(); ++l) h[l] = cap_hint_[partner[l]];
+    cap_hint_ = h;
+  }
+  plan_.built = false;
+  plan_.nrhs = 0;
+  build_gemv_plan();
 }
 
 double Matrix::norm() {

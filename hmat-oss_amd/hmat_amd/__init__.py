@@ -135,7 +135,7 @@ _SLOTS = [
     ("copy_struct", _COPY), ("norm", _NORM), ("logdet", _LOGDET), ("inverse", _vp),
     ("factorize_generic", _FACTOR), ("destroy", _P_INT), ("get_child", _vp),
     ("destroy_child", _vp), ("solve_generic", _SOLVE_GEN), ("solve_mat", _SOLVE_MAT),
-    ("solve_systems", _SOLVE), ("solve_dense", _SOLVE), ("transpose", _vp), ("scale", _SCALAR_H),
+    ("solve_systems", _SOLVE), ("solve_dense", _SOLVE), ("transpose", _P_INT), ("scale", _SCALAR_H),
     ("truncate", _P_INT), ("vector_reorder", _REORDER), ("vector_restore", _REORDER),
     ("gemm", _GEMM_H), ("axpy", _AXPY), ("trsm", _vp), ("gemv", _GEMV), ("gemm_scalar", _GEMV),
     ("full_gemm", _vp), ("gemm_dense", _GEMM_DENSE), ("add_identity", _SCALAR_H), ("init", _INIT),
@@ -520,6 +520,11 @@ class HMatrix:
         c.tree, c.iface, c.value_type = self.tree, self.iface, self.value_type
         c.handle, c.lower_symmetric, c._kernel = h, self.lower_symmetric, self._kernel
         return c
+
+    def transpose(self):
+        """hmat_interface_t.transpose: A <- A^T in place (for a complex matrix
+        the transpose, not the conjugate transpose)."""
+        _check(self.iface.transpose(self.handle), "transpose")
 
     def norm(self):
         r = self.iface.norm(self.handle)

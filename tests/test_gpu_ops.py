@@ -177,3 +177,52 @@ def test_get_block_and_get_values():
     print(f"get_block full {e1:.1e}, sub {e2:.1e}, renumbered {e3:.1e}; get_values {e4:.1e}; factor {e5:.1e}")
     assert max(e1, e2, e3, e4) <= 1e-14 * np.abs(D).max() and e5 <= 1e-14 * np.abs(F).max()
     h.close()
+
+
+@pytest.mark.gpu
+def test_transpose():
+    """hmat_interface_t.transpose (HMatrix::transpose): the stored matrix of a
+    non-symmetric real matrix becomes its exact transpose (leaves exchanged
+    with their partners, data moved, nothing recomputed) and its gemv the
+    transposed product; a complex matrix becomes its transpose (not the
+    conjugate transpose); symmetric storage is unchanged; factors are refused."""
+    n, leaf = 3000, 64
+    say(f"transpose n={n}")
+    pts = ho.cylinder_points(n, 1.0, 1.2, 6)
+    t = hmat_amd.ClusterTree(pts, leaf)
+    h = hmat_amd.HMatrix(t, lower_symmetric=False)
+    h.assemble(kind=hmat_amd.KERNEL_SHIFTED_EXPONENTIAL, length=0.5, nugget=1e-2, shift=0.3,
+               eps=1e-4)
+    D = h.to_dense()
+    h.transpose()
+    Dt = h.to_dense()
+    assert np.array_equal(Dt, D.T)
+    x = np.random.default_rng(1).standard_normal((n, 3))
+    idx = t.indices
+    ref = np.empty_like(x)
+    ref[idx] = D.T @ x[idx]
+    e = np.linalg.norm(h.gemv(x) - ref) / np.linalg.norm(ref)
+    h.transpose()
+    assert np.array_equal(h.to_dense(), D)
+    from scipy.spatial import cKDTree
+    dd, _ = cKDTree(pts).query(pts, 2)
+    hreg = float(dd[:, 1].mean())
+    z = hmat_amd.HMatrix(t, lower_symmetric=False, value_type=hmat_amd.HMAT_DOUBLE_COMPLEX)
+    z.assemble(kind=hmat_amd.KERNEL_HELMHOLTZ, wavenumber=2 * np.pi / (10 * hreg), reg=hreg,
+               nugget=1.0, eps=1e-4, method="aca_partial")
+    Z = z.to_dense()
+    z.transpose()
+    ez = np.abs(z.to_dense() - Z.T).max() / np.abs(Z).max()
+    say(f"gemv of the transpose {e:.2e}; complex transpose {ez:.2e}")
+    assert e <= 1e-13 and ez <= 1e-15
+    s = hmat_amd.HMatrix(t)
+    s.assemble(length=0.5, nugget=1e-2, eps=1e-4)
+    Ds = s.to_dense()
+    s.transpose()
+    assert np.array_equal(s.to_dense(), Ds)
+    s.factorize(hmat_amd.FACT_LLT)
+    h.factorize(hmat_amd.FACT_LU)
+    with pytest.raises(hmat_amd.EngineError):
+        h.transpose()
+    for m in (h, z, s):
+        m.close()
